@@ -1,0 +1,119 @@
+/*
+ * tincr_tda -- MI355X (gfx950) LC-TDA excited-state hot path, C ABI.
+ *
+ * Drop-in for tincr's src/excited_states path (Rust).  Each entry point names
+ * the reference interface it replaces.  Conventions:
+ *   - all arrays are caller-owned, host memory, row-major FP64;
+ *   - layouts are the reference's: q_ov [n_atoms, n_occ*n_virt] with column
+ *     i*n_virt+a, q_oo [n_atoms, n_occ^2], q_vv [n_atoms, n_virt^2], trial /
+ *     CI vectors [n_occ*n_virt, k] (one column per vector, row i*n_virt+a);
+ *   - the library copies inputs to the GPU and owns all device memory;
+ *   - every call returns a status (TDA_OK == 0); tda_last_error() explains;
+ *   - distinct contexts are independent (one HIP stream each); one context
+ *     must not be used from two threads at once.
+ */
+#ifndef TINCR_TDA_H
+#define TINCR_TDA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TDA_OK 0
+#define TDA_ERR_NOT_CONVERGED 1 /* outputs hold the last Ritz pairs (davidson.rs:80,184 DavidsonError) */
+#define TDA_ERR_BAD_ARG 2
+#define TDA_ERR_OOM 3
+#define TDA_ERR_HIP 4
+#define TDA_ERR_STATE 5 /* call order violated (e.g. run before set_system) */
+
+/* precision of the bulk products (q_ov / gamma / exchange contractions) */
+#define TDA_PREC_FP64 0 /* reference arithmetic */
+#define TDA_PREC_FP32 1 /* FP32 products, FP64 subspace + final FP64 correction */
+#define TDA_PREC_BF16 2 /* BF16 exchange (per-atom scaled, FP32 accumulate) */
+#define TDA_PREC_FP16 3 /* FP16 exchange (per-atom scaled, FP32 accumulate) */
+
+typedef struct tda_ctx tda_ctx;
+
+typedef struct tda_stats {
+  int32_t n_iter;           /* Davidson iterations, all phases */
+  int32_t n_iter_fp64;      /* iterations of the FP64 correction phase */
+  int32_t n_products;       /* single-vector A.x products, all precisions */
+  int32_t n_products_fp64;  /* of which in FP64 */
+  int32_t converged;        /* 1 if every root met the tolerance in FP64 */
+  int32_t n_atom_blocks;    /* atom blocks per exchange (HBM budget) */
+  double max_residual;      /* max FP64 residual norm of the returned roots */
+  double t_total_ms;        /* tda_run wall time */
+  double t_qbuild_ms;       /* q build + gamma_lr.q_oo (inside tda_run if not prepared) */
+  double t_products_ms;     /* sum of A.x product time (device events) */
+  double t_exchange_ms;     /* sum of exchange-stage GEMM time (device events) */
+  int32_t n_exchange_launches;
+  double exchange_flops;    /* algorithmic flops of those launches */
+} tda_stats;
+
+/* Create a context on HIP device `device`.  hbm_budget_bytes bounds the
+ * exchange intermediates (0 = 16 GiB).  precision: TDA_PREC_*. */
+int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx** out);
+void tda_ctx_destroy(tda_ctx* ctx);
+const char* tda_last_error(const tda_ctx* ctx);
+int tda_set_precision(tda_ctx* ctx, int precision);
+
+/* Replaces trans_charges (src/excited_states/transition_charges.rs:8-15).
+ * orbs, s: [n_ao, n_ao]; orbs_per_atom: [n_atoms] (AOs of each atom are
+ * contiguous); occupied MOs occ_begin..occ_begin+n_occ-1 and virtual MOs
+ * virt_begin..virt_begin+n_virt-1 (contiguous ranges, as :21-27 uses them).
+ * Outputs q_ov [n_atoms, n_occ*n_virt], q_oo [n_atoms, n_occ^2],
+ * q_vv [n_atoms, n_virt^2] (any may be NULL). */
+int tda_trans_charges(tda_ctx* ctx, int n_atoms, int n_ao, const int32_t* orbs_per_atom, const double* orbs,
+                      const double* s, int occ_begin, int n_occ, int virt_begin, int n_virt, double* q_ov,
+                      double* q_oo, double* q_vv);
+
+/* Replaces orbe_differences (src/excited_states/utils.rs:8-24):
+ * omega[i*n_virt+a] = virt[a] - occ[i]. */
+int tda_orbe_differences(const double* occ, int n_occ, const double* virt, int n_virt, double* omega);
+
+/* Loads the inputs System/Monomer::prepare_tda reads from its Properties
+ * (src/excited_states/tda/system.rs:10-52, tda/monomer.rs:8-53): orbs, S,
+ * orbe, gamma [n_atoms^2], gamma_lr [n_atoms^2] (NULL => no long-range
+ * correction, as gammafunction_lc == None, solvers/mod.rs:314), xyz
+ * [n_atoms, 3] in bohr (used for transition dipoles, tda/moments.rs:7-14). */
+int tda_set_system(tda_ctx* ctx, int n_atoms, int n_ao, const int32_t* orbs_per_atom, const double* orbs,
+                   const double* s, const double* orbe, int occ_begin, int n_occ, int virt_begin, int n_virt,
+                   const double* gamma, const double* gamma_lr, const double* xyz);
+
+/* prepare_tda: builds q_ov/q_oo/q_vv and omega on the GPU (and
+ * gamma_lr.q_oo, once per system instead of once per product call as
+ * solvers/mod.rs:328-331 does). */
+int tda_prepare(tda_ctx* ctx);
+
+/* Copies the prepared charges / omega back (reference layouts; NULL skips). */
+int tda_get_charges(tda_ctx* ctx, double* q_ov, double* q_oo, double* q_vv, double* omega);
+
+/* DavidsonEngine::compute_products without the product cache
+ * (src/excited_states/solvers/mod.rs:20-34, :280-369): ax = A x for the k
+ * columns of x [ov, k], in the context's precision. */
+int tda_compute_products(tda_ctx* ctx, const double* x, int k, double* ax);
+
+/* DavidsonEngine::precondition (solvers/mod.rs:373-380):
+ * out = r / (w - omega), entries with |w - omega| < 1e-4 divided by 1. */
+int tda_precondition(tda_ctx* ctx, const double* r, double w, double* out);
+
+/* System::run_tda / Monomer::run_tda (src/excited_states/tda/mod.rs:19-131):
+ * guess (solvers/utils.rs:17-26), Davidson (solvers/davidson.rs:53-199),
+ * q_trans = q_ov X, Mulliken transition dipoles and oscillator strengths
+ * (tda/moments.rs:7-24).  Outputs: omega [n_roots] (ci_eigenvalues), X
+ * [ov, n_roots] (ci_coefficients), q_trans [n_atoms, n_roots], tr_dipoles
+ * [3, n_roots], f [n_roots]; any output may be NULL.  Calls tda_prepare
+ * first if needed.  In a mixed precision the Davidson runs on low-precision
+ * products and then continues with FP64 products from the Ritz vectors until
+ * the FP64 residuals meet `tolerance`. */
+int tda_run(tda_ctx* ctx, int n_roots, int max_iter, double tolerance, double* omega, double* X, double* q_trans,
+            double* tr_dipoles, double* f, tda_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TINCR_TDA_H */

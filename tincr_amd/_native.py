@@ -1,0 +1,140 @@
+"""ctypes binding of libtincr_tda.so (include/tincr_tda.h).
+
+The library must have been built (``python -m tincr_amd.build``); loading
+fails loudly otherwise -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libtincr_tda.so")
+
+TDA_OK = 0
+TDA_ERR_NOT_CONVERGED = 1
+TDA_ERR_BAD_ARG = 2
+TDA_ERR_OOM = 3
+TDA_ERR_HIP = 4
+TDA_ERR_STATE = 5
+
+PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "fp16": 3}
+
+# every symbol include/tincr_tda.h declares
+EXPORTS = [
+    "tda_ctx_create", "tda_ctx_destroy", "tda_last_error", "tda_set_precision",
+    "tda_trans_charges", "tda_orbe_differences", "tda_set_system", "tda_prepare",
+    "tda_get_charges", "tda_compute_products", "tda_precondition", "tda_run",
+]
+
+
+class TdaStats(ctypes.Structure):
+    _fields_ = [
+        ("n_iter", ctypes.c_int32),
+        ("n_iter_fp64", ctypes.c_int32),
+        ("n_products", ctypes.c_int32),
+        ("n_products_fp64", ctypes.c_int32),
+        ("converged", ctypes.c_int32),
+        ("n_atom_blocks", ctypes.c_int32),
+        ("max_residual", ctypes.c_double),
+        ("t_total_ms", ctypes.c_double),
+        ("t_qbuild_ms", ctypes.c_double),
+        ("t_products_ms", ctypes.c_double),
+        ("t_exchange_ms", ctypes.c_double),
+        ("n_exchange_launches", ctypes.c_int32),
+        ("exchange_flops", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class TdaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build the HIP library first (python -m tincr_amd.build)")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    D = ctypes.POINTER(ctypes.c_double)
+    I32 = ctypes.POINTER(ctypes.c_int32)
+    i = ctypes.c_int
+    lib.tda_ctx_create.argtypes = [i, ctypes.c_size_t, i, ctypes.POINTER(P)]
+    lib.tda_ctx_destroy.argtypes = [P]
+    lib.tda_ctx_destroy.restype = None
+    lib.tda_last_error.argtypes = [P]
+    lib.tda_last_error.restype = ctypes.c_char_p
+    lib.tda_set_precision.argtypes = [P, i]
+    lib.tda_trans_charges.argtypes = [P, i, i, I32, D, D, i, i, i, i, D, D, D]
+    lib.tda_orbe_differences.argtypes = [D, i, D, i, D]
+    lib.tda_set_system.argtypes = [P, i, i, I32, D, D, D, i, i, i, i, D, D, D]
+    lib.tda_prepare.argtypes = [P]
+    lib.tda_get_charges.argtypes = [P, D, D, D, D]
+    lib.tda_compute_products.argtypes = [P, D, i, D]
+    lib.tda_precondition.argtypes = [P, D, ctypes.c_double, D]
+    lib.tda_run.argtypes = [P, i, i, ctypes.c_double, D, D, D, D, D, ctypes.POINTER(TdaStats)]
+    for name in EXPORTS:
+        if name not in ("tda_ctx_destroy", "tda_last_error"):
+            getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def dptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One device context (one HIP stream)."""
+
+    def __init__(self, device=0, precision="fp64", hbm_budget_bytes=0):
+        self.lib = load()
+        self.handle = ctypes.c_void_p()
+        rc = self.lib.tda_ctx_create(device, hbm_budget_bytes, PREC[precision], ctypes.byref(self.handle))
+        if rc != TDA_OK:
+            raise TdaError(rc, "tda_ctx_create failed")
+        self.precision = precision
+
+    def close(self):
+        if self.handle:
+            self.lib.tda_ctx_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc, allow=()):
+        if rc != TDA_OK and rc not in allow:
+            raise TdaError(rc, self.lib.tda_last_error(self.handle).decode())
+        return rc
+
+    def set_precision(self, precision):
+        self.check(self.lib.tda_set_precision(self.handle, PREC[precision]))
+        self.precision = precision

@@ -1,0 +1,60 @@
+// LDS-tiled MFMA GEMM for gfx950 with generic (two-level) index maps.
+//
+//   C[cm(m) + cn(n)]  =  epilogue( alpha * sum_k A[am(m) + ak(k)] * B[bk(k) + bn(n)] )
+//
+// The contraction layouts of the TDA path (transition-charge build, Coulomb
+// projections, the three-stage exchange and the Davidson subspace algebra)
+// are all expressed through these maps, so one kernel family serves them.
+#pragma once
+#include "common.h"
+
+namespace tda {
+
+enum Epi : int {
+  EPI_STORE = 0,  // C = alpha*acc*scale[z] + beta*C
+  EPI_DIAG = 1,   // C = alpha*acc + diag[dmap(n)] * X[xm(m) + xn(n)]
+};
+
+struct GemmArgs {
+  const void* A;
+  const void* B;
+  void* C;
+  int64_t M, N, K;
+  IdxMap am, ak, bk, bn, cm, cn;
+  // batching over blockIdx.z (outer) and split-K (inner)
+  int batch;
+  int64_t a_batch, b_batch, c_batch;
+  int ksplit;
+  int64_t k_chunk;  // multiple of BK
+  // split-K partials: ws[(z * M + m) * N + n], accumulator type
+  void* ws;
+  double alpha, beta;
+  const float* batch_scale;  // optional: multiplies alpha per batch index
+  // EPI_DIAG
+  const double* diag;
+  IdxMap dmap;
+  const double* xsrc;
+  IdxMap xm, xn;
+};
+
+// Element type of an operand / result buffer.
+enum DType : int { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2, DT_F16 = 3 };
+
+// Launch C = op(A*B).  `in` is the type of A and B (also selects the MFMA),
+// `out` the type of C.  Chooses tile shape and split-K from the problem size
+// unless `ksplit_hint` > 0.  When ksplit > 1 a workspace of
+// ksplit*batch*M*N accumulators is taken from `ws` (caller sized, see
+// gemm_workspace_bytes) and a reduction kernel applies the epilogue.
+struct GemmPlan {
+  int cfg;
+  int ksplit;
+  int64_t k_chunk;
+};
+
+GemmPlan gemm_plan(DType in, int64_t M, int64_t N, int64_t K, int batch, bool a_kc, bool b_kc,
+                   int ksplit_hint = 0);
+size_t gemm_workspace_bytes(DType in, const GemmPlan& p, int64_t M, int64_t N, int batch);
+void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs args,
+                 const GemmPlan& plan, hipStream_t stream);
+
+}  // namespace tda

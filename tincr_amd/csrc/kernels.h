@@ -1,0 +1,65 @@
+// Launchers of the non-GEMM kernels (qbuild.hip, stream.hip, eigh.hip).
+#pragma once
+#include "common.h"
+
+namespace tda {
+
+// ---- transition charges (transition_charges.rs:8-98) ----
+// out[A][p][q] = 1/2 sum_{mu in A} (C[mu][pb+p] SC[mu][qb+q] + C[mu][qb+q] SC[mu][pb+p])
+// for p < P, q < Q; rows p in [P, P_rows) and columns q in [Q, Q_pad) are zero.
+// C, SC: [n_ao][ld] row-major FP64.  ao_off: [n_atoms+1] (device).
+void launch_qblock(const double* C, const double* SC, int64_t ld, const int* ao_off, int n_atoms,
+                   int pb, int P, int P_rows, int qb, int Q, int Q_pad, double* out, hipStream_t s);
+// CT pair variant (ct_pair.rs:78-123): rows of atoms in [atom_lo, atom_hi) of a
+// combined index space:  out[A][i][a] = 1/2 sum_mu L[mu][i] R[mu][a]
+void launch_qprod(const double* Lm, int64_t ldl, const double* Rm, int64_t ldr, const int* ao_off,
+                  int atom_lo, int n_atoms, int P, int P_rows, int Q, int Q_pad, double* out,
+                  hipStream_t s);
+
+// per-atom max |x| over blocks of `len` elements -> amax[n] (float)
+void launch_block_amax(const double* x, int64_t len, int n, float* amax, hipStream_t s);
+// dst[b][e] = (T) (src[b][e] * inv_scale[b])   (inv_scale may be null => 1)
+void launch_convert(const double* src, int64_t len, int n, const float* inv_scale, void* dst, int dtype,
+                    hipStream_t s);
+void launch_convert_f32_to(const float* src, int64_t n, void* dst, int dtype, hipStream_t s);
+
+// ---- layout conversions ----
+// host order x[(i*v+a)*k + c]  <->  device xd[c*ovp + i*vp + a] (pads zero)
+void launch_pack_x(const double* src, int o, int v, int vp, int k, double* dst, hipStream_t s);
+void launch_unpack_x(const double* src, int o, int v, int vp, int k, double* dst, hipStream_t s);
+// delta eps:  de[i*vp + a] = orbe[vb+a] - orbe[ob+i]  (a < v), 0 in pads
+void launch_delta_eps(const double* orbe, int ob, int o, int vb, int v, int vp, double* de, hipStream_t s);
+// dst[k][L] (double) -> (float/half/bf16)
+void launch_cast_vectors(const double* src, int64_t n, void* dst, int dtype, hipStream_t s);
+
+// ---- Davidson helpers (davidson.rs:53-185, solvers/mod.rs:373-380) ----
+// out = r / d,  d = w - de,  |d| < 1e-4 -> 1.0
+void launch_precondition(const double* r, double w, const double* de, int64_t L, double* out, hipStream_t s);
+// P[c] = precondition(R[idx[c]], u[idx[c]])   idx, u on device
+void launch_precondition_many(const double* R, const double* u, const int* idx, int nc, const double* de,
+                              int64_t L, double* P, hipStream_t s);
+// R[r][e] = AVY[r][e] - u[r] * RZ[r][e]  (in place on AVY)
+void launch_residual(double* AVY, const double* RZ, const double* u, int n, int64_t L, hipStream_t s);
+// norms[r] = || X[r] ||  (deterministic two-pass), partial: >= n*256 doubles
+void launch_norms(const double* X, int n, int64_t L, double* partial, double* norms, hipStream_t s);
+// classical Gram-Schmidt of one new vector against accepted new vectors
+//   d[s]  = Q[s] . P_t           for s < *count
+//   W_t  -= sum_s d[s] Q[s];  norm = ||W_t||
+//   if norm > 1e-8: Q[*count] = W_t / norm; ++*count
+void launch_gs_step(const double* Pt, double* Wt, double* Q, int64_t L, int* count, double* scratch,
+                    int max_new, hipStream_t s);
+void launch_copy(const double* src, double* dst, int64_t n, hipStream_t s);
+void launch_fill(double* dst, double v, int64_t n, hipStream_t s);
+// byte-wise zero fill by a kernel (graph-capture safe, unlike hipMemsetAsync)
+void launch_zero_bytes(void* dst, size_t bytes, hipStream_t s);
+// tiny dense: C[M][N] = A[M][K] . B[K][N]   (FP64, row-major, small sizes)
+void launch_small_gemm(const double* A, const double* B, double* C, int M, int N, int K, hipStream_t s);
+
+// ---- subspace eigensolver (eigh, davidson.rs:93) ----
+// H: m x m symmetric (upper triangle read), leading dim ldh; work >= eigh_work_doubles(m)
+// Outputs the n lowest eigenvalues (ascending) and eigenvectors Y[r][0..m) (row per root).
+size_t eigh_work_doubles(int m);
+void launch_eigh(const double* H, int ldh, int m, int n, double* work, double* evals, double* Y, int ldy,
+                 hipStream_t s);
+
+}  // namespace tda

@@ -1,0 +1,489 @@
+// Streaming (HBM-bound) kernels of the TDA path: transition-charge build,
+// layout conversions and the Davidson vector updates.
+#include "kernels.h"
+
+namespace tda {
+
+static inline int grid_for(int64_t n, int per_block = 256, int cap = 16384) {
+  int64_t g = ceil_div(n, per_block);
+  if (g < 1) g = 1;
+  return (int)std::min<int64_t>(g, cap);
+}
+
+// ---------------------------------------------------------------------------
+// transition charges  q^A_pq = 1/2 sum_{mu in A}(C_mu,p SC_mu,q + C_mu,q SC_mu,p)
+// (transition_charges.rs:38-95).  Block = 256 threads along q, QB_ROWS rows p.
+// ---------------------------------------------------------------------------
+constexpr int QB_ROWS = 8;
+
+__global__ void __launch_bounds__(256) qblock_kernel(const double* __restrict__ C, const double* __restrict__ SC,
+                                                     int64_t ld, const int* __restrict__ ao_off, int pb, int P,
+                                                     int P_rows, int qb, int Q, int Q_pad, double* __restrict__ out) {
+  const int A = blockIdx.z;
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  const int p0 = blockIdx.y * QB_ROWS;
+  if (q >= Q_pad) return;
+  const int mu0 = ao_off[A], mu1 = ao_off[A + 1];
+  double acc[QB_ROWS];
+#pragma unroll
+  for (int r = 0; r < QB_ROWS; ++r) acc[r] = 0.0;
+  if (q < Q) {
+    for (int mu = mu0; mu < mu1; ++mu) {
+      const double cq = C[mu * ld + qb + q];
+      const double scq = SC[mu * ld + qb + q];
+#pragma unroll
+      for (int r = 0; r < QB_ROWS; ++r) {
+        const int p = p0 + r;
+        if (p < P) {
+          const double cp = C[mu * ld + pb + p];
+          const double scp = SC[mu * ld + pb + p];
+          acc[r] += 0.5 * (cp * scq + cq * scp);
+        }
+      }
+    }
+  }
+  double* o = out + (int64_t)A * P_rows * Q_pad;
+#pragma unroll
+  for (int r = 0; r < QB_ROWS; ++r) {
+    const int p = p0 + r;
+    if (p < P_rows) o[(int64_t)p * Q_pad + q] = (p < P) ? acc[r] : 0.0;
+  }
+}
+
+void launch_qblock(const double* C, const double* SC, int64_t ld, const int* ao_off, int n_atoms, int pb, int P,
+                   int P_rows, int qb, int Q, int Q_pad, double* out, hipStream_t s) {
+  if (n_atoms == 0 || P_rows == 0 || Q_pad == 0) return;
+  dim3 grid((unsigned)ceil_div(Q_pad, 256), (unsigned)ceil_div(P_rows, QB_ROWS), (unsigned)n_atoms);
+  qblock_kernel<<<grid, 256, 0, s>>>(C, SC, ld, ao_off, pb, P, P_rows, qb, Q, Q_pad, out);
+  TDA_LAUNCH_CHECK();
+}
+
+// out[A][i][a] = 1/2 sum_{mu in A} L[mu][i] R[mu][a]  (ct_pair.rs:99-121)
+__global__ void __launch_bounds__(256) qprod_kernel(const double* __restrict__ Lm, int64_t ldl,
+                                                    const double* __restrict__ Rm, int64_t ldr,
+                                                    const int* __restrict__ ao_off, int atom_lo, int P, int P_rows,
+                                                    int Q, int Q_pad, double* __restrict__ out) {
+  const int A = blockIdx.z;
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  const int p0 = blockIdx.y * QB_ROWS;
+  if (q >= Q_pad) return;
+  const int mu0 = ao_off[A], mu1 = ao_off[A + 1];
+  double acc[QB_ROWS];
+#pragma unroll
+  for (int r = 0; r < QB_ROWS; ++r) acc[r] = 0.0;
+  if (q < Q) {
+    for (int mu = mu0; mu < mu1; ++mu) {
+      const double rq = Rm[mu * ldr + q];
+#pragma unroll
+      for (int r = 0; r < QB_ROWS; ++r) {
+        const int p = p0 + r;
+        if (p < P) acc[r] += Lm[mu * ldl + p] * rq;
+      }
+    }
+  }
+  double* o = out + (int64_t)(atom_lo + A) * P_rows * Q_pad;
+#pragma unroll
+  for (int r = 0; r < QB_ROWS; ++r) {
+    const int p = p0 + r;
+    if (p < P_rows) o[(int64_t)p * Q_pad + q] = (p < P) ? 0.5 * acc[r] : 0.0;
+  }
+}
+
+void launch_qprod(const double* Lm, int64_t ldl, const double* Rm, int64_t ldr, const int* ao_off, int atom_lo,
+                  int n_atoms, int P, int P_rows, int Q, int Q_pad, double* out, hipStream_t s) {
+  if (n_atoms == 0 || P_rows == 0 || Q_pad == 0) return;
+  dim3 grid((unsigned)ceil_div(Q_pad, 256), (unsigned)ceil_div(P_rows, QB_ROWS), (unsigned)n_atoms);
+  qprod_kernel<<<grid, 256, 0, s>>>(Lm, ldl, Rm, ldr, ao_off, atom_lo, P, P_rows, Q, Q_pad, out);
+  TDA_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// per-block max |x| and scaled conversions (per-atom scaling of low-precision
+// operands)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) block_amax_kernel(const double* __restrict__ x, int64_t len,
+                                                         float* __restrict__ amax) {
+  const double* p = x + (int64_t)blockIdx.x * len;
+  double m = 0.0;
+  for (int64_t e = threadIdx.x; e < len; e += 256) m = fmax(m, fabs(p[e]));
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    amax[blockIdx.x] = (float)r;
+  }
+}
+
+void launch_block_amax(const double* x, int64_t len, int n, float* amax, hipStream_t s) {
+  if (n == 0) return;
+  block_amax_kernel<<<n, 256, 0, s>>>(x, len, amax);
+  TDA_LAUNCH_CHECK();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) convert_kernel(const double* __restrict__ src, int64_t len, int n,
+                                                      const float* __restrict__ inv_scale, T* __restrict__ dst) {
+  const int64_t total = len * n;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    double v = src[t];
+    if (inv_scale) v *= (double)inv_scale[t / len];
+    dst[t] = to_type<T>(v);
+  }
+}
+
+void launch_convert(const double* src, int64_t len, int n, const float* inv_scale, void* dst, int dtype,
+                    hipStream_t s) {
+  const int64_t total = len * n;
+  if (total == 0) return;
+  const int g = grid_for(total);
+  switch (dtype) {
+    case PREC_FP32: convert_kernel<float><<<g, 256, 0, s>>>(src, len, n, inv_scale, (float*)dst); break;
+    case PREC_BF16: convert_kernel<__bf16><<<g, 256, 0, s>>>(src, len, n, inv_scale, (__bf16*)dst); break;
+    case PREC_FP16: convert_kernel<_Float16><<<g, 256, 0, s>>>(src, len, n, inv_scale, (_Float16*)dst); break;
+    default: convert_kernel<double><<<g, 256, 0, s>>>(src, len, n, inv_scale, (double*)dst); break;
+  }
+  TDA_LAUNCH_CHECK();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) convert_f32_kernel(const float* __restrict__ src, int64_t n, T* __restrict__ dst) {
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+    dst[t] = to_type<T>((double)src[t]);
+}
+
+void launch_convert_f32_to(const float* src, int64_t n, void* dst, int dtype, hipStream_t s) {
+  if (n == 0) return;
+  const int g = grid_for(n);
+  switch (dtype) {
+    case PREC_BF16: convert_f32_kernel<__bf16><<<g, 256, 0, s>>>(src, n, (__bf16*)dst); break;
+    case PREC_FP16: convert_f32_kernel<_Float16><<<g, 256, 0, s>>>(src, n, (_Float16*)dst); break;
+    default: throw Error{2, "convert_f32_to: bad dtype"};
+  }
+  TDA_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// layout conversions
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) pack_x_kernel(const double* __restrict__ src, int o, int v, int vp, int k,
+                                                     double* __restrict__ dst) {
+  const int64_t ovp = (int64_t)o * vp;
+  const int64_t total = ovp * k;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    const int c = (int)(t / ovp);
+    const int64_t e = t % ovp;
+    const int i = (int)(e / vp), a = (int)(e % vp);
+    dst[t] = (a < v) ? src[((int64_t)i * v + a) * k + c] : 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(256) unpack_x_kernel(const double* __restrict__ src, int o, int v, int vp, int k,
+                                                       double* __restrict__ dst) {
+  const int64_t ov = (int64_t)o * v;
+  const int64_t total = ov * k;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    const int64_t ia = t / k;
+    const int c = (int)(t % k);
+    const int i = (int)(ia / v), a = (int)(ia % v);
+    dst[t] = src[(int64_t)c * o * vp + (int64_t)i * vp + a];
+  }
+}
+
+void launch_pack_x(const double* src, int o, int v, int vp, int k, double* dst, hipStream_t s) {
+  const int64_t total = (int64_t)o * vp * k;
+  if (total == 0) return;
+  pack_x_kernel<<<grid_for(total), 256, 0, s>>>(src, o, v, vp, k, dst);
+  TDA_LAUNCH_CHECK();
+}
+
+void launch_unpack_x(const double* src, int o, int v, int vp, int k, double* dst, hipStream_t s) {
+  const int64_t total = (int64_t)o * v * k;
+  if (total == 0) return;
+  unpack_x_kernel<<<grid_for(total), 256, 0, s>>>(src, o, v, vp, k, dst);
+  TDA_LAUNCH_CHECK();
+}
+
+__global__ void delta_eps_kernel(const double* __restrict__ orbe, int ob, int o, int vb, int v, int vp,
+                                 double* __restrict__ de) {
+  const int64_t total = (int64_t)o * vp;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    const int i = (int)(t / vp), a = (int)(t % vp);
+    de[t] = (a < v) ? orbe[vb + a] - orbe[ob + i] : 0.0;
+  }
+}
+
+void launch_delta_eps(const double* orbe, int ob, int o, int vb, int v, int vp, double* de, hipStream_t s) {
+  const int64_t total = (int64_t)o * vp;
+  if (total == 0) return;
+  delta_eps_kernel<<<grid_for(total), 256, 0, s>>>(orbe, ob, o, vb, v, vp, de);
+  TDA_LAUNCH_CHECK();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) cast_kernel(const double* __restrict__ src, int64_t n, T* __restrict__ dst) {
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+    dst[t] = to_type<T>(src[t]);
+}
+
+void launch_cast_vectors(const double* src, int64_t n, void* dst, int dtype, hipStream_t s) {
+  if (n == 0) return;
+  const int g = grid_for(n);
+  switch (dtype) {
+    case PREC_FP32: cast_kernel<float><<<g, 256, 0, s>>>(src, n, (float*)dst); break;
+    case PREC_BF16: cast_kernel<__bf16><<<g, 256, 0, s>>>(src, n, (__bf16*)dst); break;
+    case PREC_FP16: cast_kernel<_Float16><<<g, 256, 0, s>>>(src, n, (_Float16*)dst); break;
+    default: throw Error{2, "cast_vectors: bad dtype"};
+  }
+  TDA_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Davidson helpers
+// ---------------------------------------------------------------------------
+__device__ inline double precond_denom(double w, double de) {
+  const double d = w - de;
+  return fabs(d) < 0.0001 ? 1.0 : d;  // solvers/mod.rs:378
+}
+
+__global__ void __launch_bounds__(256) precondition_kernel(const double* __restrict__ r, double w,
+                                                           const double* __restrict__ de, int64_t L,
+                                                           double* __restrict__ out) {
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256)
+    out[t] = r[t] / precond_denom(w, de[t]);
+}
+
+void launch_precondition(const double* r, double w, const double* de, int64_t L, double* out, hipStream_t s) {
+  if (L == 0) return;
+  precondition_kernel<<<grid_for(L), 256, 0, s>>>(r, w, de, L, out);
+  TDA_LAUNCH_CHECK();
+}
+
+__global__ void __launch_bounds__(256) precondition_many_kernel(const double* __restrict__ R,
+                                                                const double* __restrict__ u,
+                                                                const int* __restrict__ idx, int nc,
+                                                                const double* __restrict__ de, int64_t L,
+                                                                double* __restrict__ P) {
+  const int c = blockIdx.y;
+  const int r = idx[c];
+  const double w = u[r];
+  const double* src = R + (int64_t)r * L;
+  double* dst = P + (int64_t)c * L;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256)
+    dst[t] = src[t] / precond_denom(w, de[t]);
+}
+
+void launch_precondition_many(const double* R, const double* u, const int* idx, int nc, const double* de, int64_t L,
+                              double* P, hipStream_t s) {
+  if (nc == 0 || L == 0) return;
+  dim3 grid((unsigned)grid_for(L, 256, 4096), (unsigned)nc);
+  precondition_many_kernel<<<grid, 256, 0, s>>>(R, u, idx, nc, de, L, P);
+  TDA_LAUNCH_CHECK();
+}
+
+__global__ void __launch_bounds__(256) residual_kernel(double* __restrict__ AVY, const double* __restrict__ RZ,
+                                                       const double* __restrict__ u, int64_t L) {
+  const int r = blockIdx.y;
+  const double w = u[r];
+  double* a = AVY + (int64_t)r * L;
+  const double* z = RZ + (int64_t)r * L;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256)
+    a[t] = a[t] - z[t] * w;
+}
+
+void launch_residual(double* AVY, const double* RZ, const double* u, int n, int64_t L, hipStream_t s) {
+  if (n == 0 || L == 0) return;
+  dim3 grid((unsigned)grid_for(L, 256, 4096), (unsigned)n);
+  residual_kernel<<<grid, 256, 0, s>>>(AVY, RZ, u, L);
+  TDA_LAUNCH_CHECK();
+}
+
+// Deterministic two-pass vector norms: NB_NORM partial sums per vector.
+constexpr int NB_NORM = 256;
+
+__global__ void __launch_bounds__(256) norm_partial_kernel(const double* __restrict__ X, int64_t L,
+                                                           double* __restrict__ partial) {
+  const int r = blockIdx.y;
+  const double* x = X + (int64_t)r * L;
+  double acc = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256) {
+    const double v = x[t];
+    acc += v * v;
+  }
+  acc = wave_sum(acc);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[(int64_t)r * NB_NORM + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void __launch_bounds__(256) norm_final_kernel(const double* __restrict__ partial, int n,
+                                                         double* __restrict__ norms) {
+  const int r = blockIdx.x;
+  double acc = partial[(int64_t)r * NB_NORM + threadIdx.x];
+  acc = wave_sum(acc);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) norms[r] = sqrt((red[0] + red[1]) + (red[2] + red[3]));
+}
+
+void launch_norms(const double* X, int n, int64_t L, double* partial, double* norms, hipStream_t s) {
+  if (n == 0) return;
+  norm_partial_kernel<<<dim3(NB_NORM, n), 256, 0, s>>>(X, L, partial);
+  TDA_LAUNCH_CHECK();
+  norm_final_kernel<<<n, 256, 0, s>>>(partial, n, norms);
+  TDA_LAUNCH_CHECK();
+}
+
+// ---- Gram-Schmidt step for one new vector (davidson.rs:162-168) ----
+// scratch layout: [0, NB_NORM*max_new) dot partials, then d[max_new],
+// then norm partials [NB_NORM], then {norm, accept}.
+__global__ void __launch_bounds__(256) gs_dots_kernel(const double* __restrict__ Pt, const double* __restrict__ Q,
+                                                      int64_t L, const int* __restrict__ count,
+                                                      double* __restrict__ scratch) {
+  const int s = blockIdx.y;
+  if (s >= *count) return;
+  const double* q = Q + (int64_t)s * L;
+  double acc = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256)
+    acc += q[t] * Pt[t];
+  acc = wave_sum(acc);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) scratch[(int64_t)s * NB_NORM + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void __launch_bounds__(256) gs_dots_final_kernel(const int* __restrict__ count, int max_new,
+                                                            double* __restrict__ scratch) {
+  const int s = blockIdx.x;
+  if (s >= *count) return;
+  double acc = wave_sum(scratch[(int64_t)s * NB_NORM + threadIdx.x]);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) scratch[(int64_t)NB_NORM * max_new + s] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void __launch_bounds__(256) gs_update_kernel(double* __restrict__ Wt, const double* __restrict__ Q,
+                                                        int64_t L, const int* __restrict__ count, int max_new,
+                                                        double* __restrict__ scratch) {
+  const int cnt = *count;
+  const double* d = scratch + (int64_t)NB_NORM * max_new;
+  double acc = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256) {
+    double w = Wt[t];
+    for (int s = 0; s < cnt; ++s) w -= d[s] * Q[(int64_t)s * L + t];
+    Wt[t] = w;
+    acc += w * w;
+  }
+  acc = wave_sum(acc);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    scratch[(int64_t)NB_NORM * max_new + max_new + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void __launch_bounds__(256) gs_norm_final_kernel(int max_new, double* __restrict__ scratch) {
+  double* np = scratch + (int64_t)NB_NORM * max_new + max_new;
+  double acc = wave_sum(np[threadIdx.x]);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double nrm = sqrt((red[0] + red[1]) + (red[2] + red[3]));
+    np[NB_NORM] = nrm;
+    np[NB_NORM + 1] = (nrm > 1.0e-8) ? 1.0 : 0.0;  // davidson.rs:165
+  }
+}
+
+__global__ void __launch_bounds__(256) gs_append_kernel(const double* __restrict__ Wt, double* __restrict__ Q,
+                                                        int64_t L, const int* __restrict__ count, int max_new,
+                                                        const double* __restrict__ scratch) {
+  const double* np = scratch + (int64_t)NB_NORM * max_new + max_new;
+  if (np[NB_NORM + 1] == 0.0) return;
+  const double nrm = np[NB_NORM];
+  double* q = Q + (int64_t)(*count) * L;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256)
+    q[t] = Wt[t] / nrm;
+}
+
+__global__ void gs_count_kernel(int* count, int max_new, const double* __restrict__ scratch) {
+  const double* np = scratch + (int64_t)NB_NORM * max_new + max_new;
+  if (threadIdx.x == 0 && np[NB_NORM + 1] != 0.0) *count += 1;
+}
+
+void launch_gs_step(const double* Pt, double* Wt, double* Q, int64_t L, int* count, double* scratch, int max_new,
+                    hipStream_t s) {
+  gs_dots_kernel<<<dim3(NB_NORM, max_new), 256, 0, s>>>(Pt, Q, L, count, scratch);
+  TDA_LAUNCH_CHECK();
+  gs_dots_final_kernel<<<max_new, 256, 0, s>>>(count, max_new, scratch);
+  TDA_LAUNCH_CHECK();
+  gs_update_kernel<<<NB_NORM, 256, 0, s>>>(Wt, Q, L, count, max_new, scratch);
+  TDA_LAUNCH_CHECK();
+  gs_norm_final_kernel<<<1, 256, 0, s>>>(max_new, scratch);
+  TDA_LAUNCH_CHECK();
+  gs_append_kernel<<<grid_for(L, 256, 4096), 256, 0, s>>>(Wt, Q, L, count, max_new, scratch);
+  TDA_LAUNCH_CHECK();
+  gs_count_kernel<<<1, 64, 0, s>>>(count, max_new, scratch);
+  TDA_LAUNCH_CHECK();
+}
+
+__global__ void copy_kernel(const double* __restrict__ src, double* __restrict__ dst, int64_t n) {
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) dst[t] = src[t];
+}
+__global__ void fill_kernel(double* __restrict__ dst, double v, int64_t n) {
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) dst[t] = v;
+}
+void launch_copy(const double* src, double* dst, int64_t n, hipStream_t s) {
+  if (n == 0) return;
+  copy_kernel<<<grid_for(n), 256, 0, s>>>(src, dst, n);
+  TDA_LAUNCH_CHECK();
+}
+void launch_fill(double* dst, double v, int64_t n, hipStream_t s) {
+  if (n == 0) return;
+  fill_kernel<<<grid_for(n), 256, 0, s>>>(dst, v, n);
+  TDA_LAUNCH_CHECK();
+}
+
+__global__ void zero_words_kernel(uint32_t* __restrict__ dst, int64_t nw) {
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nw; t += (int64_t)gridDim.x * 256) dst[t] = 0u;
+}
+__global__ void zero_tail_kernel(unsigned char* __restrict__ dst, int64_t nb) {
+  if (threadIdx.x < nb) dst[threadIdx.x] = 0;
+}
+void launch_zero_bytes(void* dst, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return;
+  const int64_t nw = (int64_t)(bytes / 4);
+  if (nw) {
+    zero_words_kernel<<<grid_for(nw), 256, 0, s>>>((uint32_t*)dst, nw);
+    TDA_LAUNCH_CHECK();
+  }
+  const int64_t tail = (int64_t)(bytes % 4);
+  if (tail) {
+    zero_tail_kernel<<<1, 64, 0, s>>>((unsigned char*)dst + nw * 4, tail);
+    TDA_LAUNCH_CHECK();
+  }
+}
+
+__global__ void small_gemm_kernel(const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C,
+                                  int M, int N, int K) {
+  const int64_t total = (int64_t)M * N;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    const int m = (int)(t / N), n = (int)(t % N);
+    double acc = 0.0;
+    for (int k = 0; k < K; ++k) acc += A[(int64_t)m * K + k] * B[(int64_t)k * N + n];
+    C[t] = acc;
+  }
+}
+void launch_small_gemm(const double* A, const double* B, double* C, int M, int N, int K, hipStream_t s) {
+  const int64_t total = (int64_t)M * N;
+  if (total == 0) return;
+  small_gemm_kernel<<<grid_for(total), 256, 0, s>>>(A, B, C, M, N, K);
+  TDA_LAUNCH_CHECK();
+}
+
+}  // namespace tda

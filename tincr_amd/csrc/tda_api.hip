@@ -1,0 +1,877 @@
+// C ABI, device context and the Davidson driver of the LC-TDA path.
+//
+// Device layouts (all row-major; o_p, v_p = n_occ, n_virt rounded up to 64):
+//   trial / product vectors  X[k][o][v_p]           (ovp = o * v_p per vector)
+//   q_ov                     [N][o][v_p]
+//   q_oo, G = gamma_lr q_oo  [N][o_p][o_p]
+//   q_vv                     [N][v_p][v_p]
+//   exchange intermediate T  [k][v_p][NB*o_p]  per atom block of NB atoms
+// Pads are zero, so every dot product over ovp equals the one over o*v.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "../../include/tincr_tda.h"
+#include "gemm.h"
+#include "kernels.h"
+
+using namespace tda;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  // returns true when (re)allocated
+  bool ensure(size_t n) {
+    if (n <= bytes) return false;
+    release();
+    if (n == 0) return false;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      bytes = 0;
+      throw Error{TDA_ERR_OOM, "hipMalloc of " + std::to_string(n) + " bytes failed"};
+    }
+    bytes = n;
+    return true;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const { return (T*)p; }
+};
+
+DType dtype_of(int prec) {
+  switch (prec) {
+    case PREC_FP64: return DT_F64;
+    case PREC_FP32: return DT_F32;
+    case PREC_BF16: return DT_BF16;
+    default: return DT_F16;
+  }
+}
+
+struct Timer {
+  hipEvent_t a, b;
+  Timer() {
+    TDA_HIP(hipEventCreate(&a));
+    TDA_HIP(hipEventCreate(&b));
+  }
+  ~Timer() {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+};
+
+}  // namespace
+
+struct tda_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int prec = PREC_FP64;
+  size_t budget = (size_t)16 << 30;
+  std::string err;
+
+  // system
+  bool has_system = false, has_lc = false, prepared = false;
+  int N = 0, nao = 0, ob = 0, o = 0, vb = 0, v = 0, op = 0, vp = 0;
+  int64_t ovp = 0;
+  std::vector<double> xyz, h_de;
+  DevBuf C, S, SC, orbe, aooff, gamma, gamma_lr;
+  // prepared operators (FP64)
+  DevBuf qov, qoo, qvv, G, de;
+  // low precision copies
+  int lp_prec = -1;
+  DevBuf qov32, qvv_lp, G_lp, qvv_amax, G_amax, stage1_scale;
+  // workspaces
+  DevBuf ws_gemm, T, xlp, u, uu, vv32, ax_tmp, stage, stage2;
+  // Davidson state
+  DevBuf V, AV, H, Y, evals, RZ, R, P, W, norms, npart, idx, count, gs_scratch, eigw;
+  // instrumentation
+  bool timing = true;
+  double t_products_ms = 0, t_exchange_ms = 0, exchange_flops = 0;
+  int n_exchange_launches = 0, n_products = 0, n_products_fp64 = 0, n_blocks = 1;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_exch;
+  std::vector<double> ev_exch_flops;
+  std::vector<hipEvent_t> ev_pool;
+
+  ~tda_ctx() {
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
+    for (DevBuf* b : {&C, &S, &SC, &orbe, &aooff, &gamma, &gamma_lr, &qov, &qoo, &qvv, &G, &de, &qov32, &qvv_lp,
+                      &G_lp, &qvv_amax, &G_amax, &stage1_scale, &ws_gemm, &T, &xlp, &u, &uu, &vv32, &ax_tmp,
+                      &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &P, &W, &norms, &npart, &idx, &count,
+                      &gs_scratch, &eigw})
+      b->release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  hipEvent_t get_event() {
+    if (!ev_pool.empty()) {
+      hipEvent_t e = ev_pool.back();
+      ev_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    TDA_HIP(hipEventCreate(&e));
+    return e;
+  }
+  // Fold recorded exchange events into the counters (after a sync).
+  void harvest_events() {
+    for (size_t i = 0; i < ev_exch.size(); ++i) {
+      float ms = 0;
+      TDA_HIP(hipEventElapsedTime(&ms, ev_exch[i].first, ev_exch[i].second));
+      t_exchange_ms += ms;
+      exchange_flops += ev_exch_flops[i];
+      ++n_exchange_launches;
+      ev_pool.push_back(ev_exch[i].first);
+      ev_pool.push_back(ev_exch[i].second);
+    }
+    ev_exch.clear();
+    ev_exch_flops.clear();
+  }
+};
+
+namespace {
+
+void set_device(tda_ctx* c) { TDA_HIP(hipSetDevice(c->device)); }
+
+// ---------------------------------------------------------------------------
+// GEMM helper
+// ---------------------------------------------------------------------------
+GemmArgs gargs(const void* A, const void* B, void* C, int64_t M, int64_t N, int64_t K, IdxMap am, IdxMap ak,
+               IdxMap bk, IdxMap bn, IdxMap cm, IdxMap cn) {
+  GemmArgs g;
+  std::memset(&g, 0, sizeof(g));
+  g.A = A; g.B = B; g.C = C;
+  g.M = M; g.N = N; g.K = K;
+  g.am = am; g.ak = ak; g.bk = bk; g.bn = bn; g.cm = cm; g.cn = cn;
+  g.batch = 1;
+  g.ksplit = 1;
+  g.alpha = 1.0;
+  g.beta = 0.0;
+  g.dmap = plain(1); g.xm = plain(0); g.xn = plain(0);
+  return g;
+}
+
+void run_gemm(tda_ctx* c, DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g, int ksplit_hint = 0) {
+  GemmPlan plan = gemm_plan(in, g.M, g.N, g.K, g.batch < 1 ? 1 : g.batch, a_kc, b_kc, ksplit_hint);
+  size_t wsb = gemm_workspace_bytes(in, plan, g.M, g.N, g.batch < 1 ? 1 : g.batch);
+  if (wsb) {
+    c->ws_gemm.ensure(wsb);
+    g.ws = c->ws_gemm.p;
+  }
+  gemm_launch(in, out, epi, a_kc, b_kc, g, plan, c->stream);
+}
+
+// ---------------------------------------------------------------------------
+// prepare_tda: SC = S C, q blocks, delta eps, G = gamma_lr q_oo, LP copies
+// ---------------------------------------------------------------------------
+void build_charges(tda_ctx* c, const double* dC, const double* dSC, const int* dao, int n_atoms, int nao, int ob,
+                   int o, int vb, int v, int op, int vp, double* qov, double* qoo, double* qvv) {
+  if (qov) launch_qblock(dC, dSC, nao, dao, n_atoms, ob, o, o, vb, v, vp, qov, c->stream);
+  if (qoo) launch_qblock(dC, dSC, nao, dao, n_atoms, ob, o, op, ob, o, op, qoo, c->stream);
+  if (qvv) launch_qblock(dC, dSC, nao, dao, n_atoms, vb, v, vp, vb, v, vp, qvv, c->stream);
+}
+
+void gemm_sc(tda_ctx* c, const double* dS, const double* dC, double* dSC, int nao) {
+  // SC = S C  (transition_charges.rs:35)
+  GemmArgs g = gargs(dS, dC, dSC, nao, nao, nao, plain(nao), plain(1), plain(nao), plain(1), plain(nao), plain(1));
+  run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, g);
+}
+
+void prepare_lowprec(tda_ctx* c) {
+  if (c->prec == PREC_FP64 || c->lp_prec == c->prec) return;
+  const int N = c->N;
+  const size_t esz = prec_bytes(c->prec);
+  // Coulomb operand in FP32 for every mixed precision
+  c->qov32.ensure((size_t)N * c->ovp * 4);
+  launch_convert(c->qov.as<double>(), c->ovp, N, nullptr, c->qov32.p, PREC_FP32, c->stream);
+  if (c->has_lc) {
+    const int64_t lvv = (int64_t)c->vp * c->vp, loo = (int64_t)c->op * c->op;
+    c->qvv_lp.ensure((size_t)N * lvv * esz);
+    c->G_lp.ensure((size_t)N * loo * esz);
+    c->qvv_amax.ensure((size_t)N * 4);
+    c->G_amax.ensure((size_t)N * 4);
+    c->stage1_scale.ensure((size_t)N * 4 * 3);
+    float* inv_vv = c->stage1_scale.as<float>() + N;
+    float* inv_g = c->stage1_scale.as<float>() + 2 * N;
+    if (c->prec == PREC_FP32) {
+      launch_convert(c->qvv.as<double>(), lvv, N, nullptr, c->qvv_lp.p, PREC_FP32, c->stream);
+      launch_convert(c->G.as<double>(), loo, N, nullptr, c->G_lp.p, PREC_FP32, c->stream);
+    } else {
+      // per-atom scaling: stored q_vv^B / s_B and G^B / g_B; stage 1 output
+      // is multiplied by s_B * g_B so that stage 3 needs no rescaling.
+      launch_block_amax(c->qvv.as<double>(), lvv, N, c->qvv_amax.as<float>(), c->stream);
+      launch_block_amax(c->G.as<double>(), loo, N, c->G_amax.as<float>(), c->stream);
+      std::vector<float> svv(N), sg(N), sc(3 * (size_t)N);
+      TDA_HIP(hipMemcpyAsync(svv.data(), c->qvv_amax.p, N * 4, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipMemcpyAsync(sg.data(), c->G_amax.p, N * 4, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipStreamSynchronize(c->stream));
+      for (int b = 0; b < N; ++b) {
+        const float s1 = svv[b] > 0.f ? svv[b] : 1.f;
+        const float s2 = sg[b] > 0.f ? sg[b] : 1.f;
+        sc[b] = s1 * s2;
+        sc[N + b] = 1.f / s1;
+        sc[2 * N + b] = 1.f / s2;
+      }
+      TDA_HIP(hipMemcpyAsync(c->stage1_scale.p, sc.data(), sc.size() * 4, hipMemcpyHostToDevice, c->stream));
+      launch_convert(c->qvv.as<double>(), lvv, N, inv_vv, c->qvv_lp.p, c->prec, c->stream);
+      launch_convert(c->G.as<double>(), loo, N, inv_g, c->G_lp.p, c->prec, c->stream);
+      TDA_HIP(hipStreamSynchronize(c->stream));
+    }
+  }
+  c->lp_prec = c->prec;
+}
+
+void prepare(tda_ctx* c) {
+  TDA_CHECK(c->has_system, TDA_ERR_STATE, "tda_prepare: no system loaded");
+  if (c->prepared) {
+    prepare_lowprec(c);
+    return;
+  }
+  const int N = c->N, nao = c->nao;
+  c->SC.ensure((size_t)nao * nao * 8);
+  gemm_sc(c, c->S.as<double>(), c->C.as<double>(), c->SC.as<double>(), nao);
+  c->qov.ensure((size_t)N * c->ovp * 8);
+  double* qoo = nullptr;
+  double* qvv = nullptr;
+  if (c->has_lc) {
+    c->qoo.ensure((size_t)N * c->op * c->op * 8);
+    c->qvv.ensure((size_t)N * c->vp * c->vp * 8);
+    qoo = c->qoo.as<double>();
+    qvv = c->qvv.as<double>();
+  }
+  build_charges(c, c->C.as<double>(), c->SC.as<double>(), c->aooff.as<int>(), N, nao, c->ob, c->o, c->vb, c->v,
+                c->op, c->vp, c->qov.as<double>(), qoo, qvv);
+  c->de.ensure((size_t)c->ovp * 8);
+  launch_delta_eps(c->orbe.as<double>(), c->ob, c->o, c->vb, c->v, c->vp, c->de.as<double>(), c->stream);
+  if (c->has_lc) {
+    // G[B] = sum_C gamma_lr[B][C] q_oo[C]  (solvers/mod.rs:328-331, once per system)
+    const int64_t loo = (int64_t)c->op * c->op;
+    c->G.ensure((size_t)N * loo * 8);
+    GemmArgs g = gargs(c->gamma_lr.p, c->qoo.p, c->G.p, N, loo, N, plain(N), plain(1), plain(loo), plain(1),
+                       plain(loo), plain(1));
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, g);
+  }
+  TDA_HIP(hipStreamSynchronize(c->stream));
+  c->prepared = true;
+  c->lp_prec = -1;
+  prepare_lowprec(c);
+}
+
+// ---------------------------------------------------------------------------
+// A.x  (solvers/mod.rs:280-369).  X, AX: device [k][ovp] FP64.
+// ---------------------------------------------------------------------------
+void apply_A(tda_ctx* c, const double* X, int k, double* AX, int prec) {
+  if (k == 0) return;
+  const int N = c->N, o = c->o, v = c->v, op = c->op, vp = c->vp;
+  const int64_t ovp = c->ovp;
+  const bool lp = prec != PREC_FP64;
+  // ---- Coulomb: u = q_ov X ; vv = gamma u ; AX = de*X + 2 q_ov^T vv ----
+  c->u.ensure((size_t)N * k * 8);
+  c->uu.ensure((size_t)N * k * 8);
+  if (!lp) {
+    GemmArgs g = gargs(c->qov.p, X, c->u.p, N, k, ovp, plain(ovp), plain(1), plain(1), plain(ovp), plain(k), plain(1));
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
+  } else {
+    c->xlp.ensure((size_t)k * ovp * std::max<size_t>(4, prec_bytes(prec)));
+    launch_cast_vectors(X, (int64_t)k * ovp, c->xlp.p, PREC_FP32, c->stream);
+    GemmArgs g = gargs(c->qov32.p, c->xlp.p, c->u.p, N, k, ovp, plain(ovp), plain(1), plain(1), plain(ovp),
+                       plain(k), plain(1));
+    run_gemm(c, DT_F32, DT_F64, EPI_STORE, true, true, g);
+  }
+  launch_small_gemm(c->gamma.as<double>(), c->u.as<double>(), c->uu.as<double>(), N, k, N, c->stream);
+  {
+    GemmArgs g;
+    if (!lp) {
+      g = gargs(c->uu.p, c->qov.p, AX, k, ovp, N, plain(1), plain(k), plain(ovp), plain(1), plain(ovp), plain(1));
+    } else {
+      c->vv32.ensure((size_t)N * k * 4);
+      launch_convert(c->uu.as<double>(), (int64_t)N * k, 1, nullptr, c->vv32.p, PREC_FP32, c->stream);
+      g = gargs(c->vv32.p, c->qov32.p, AX, k, ovp, N, plain(1), plain(k), plain(ovp), plain(1), plain(ovp),
+                plain(1));
+    }
+    g.alpha = 2.0;  // singlet Coulomb factor (solvers/mod.rs:311)
+    g.diag = c->de.as<double>();
+    g.dmap = plain(1);
+    g.xsrc = X;
+    g.xm = plain(ovp);
+    g.xn = plain(1);
+    run_gemm(c, lp ? DT_F32 : DT_F64, DT_F64, EPI_DIAG, false, false, g);
+  }
+  if (!c->has_lc) return;
+  // ---- long-range exchange, blocked over atoms ----
+  const DType dt = lp ? dtype_of(prec) : DT_F64;
+  const size_t esz = lp ? prec_bytes(prec) : 8;
+  const void* Xin = X;
+  if (lp && prec != PREC_FP32) {
+    launch_cast_vectors(X, (int64_t)k * ovp, c->xlp.p, prec, c->stream);
+  }
+  if (lp) Xin = c->xlp.p;  // FP32 cast above serves PREC_FP32
+  const void* qvv = lp ? c->qvv_lp.p : c->qvv.p;
+  const void* G = lp ? c->G_lp.p : c->G.p;
+  const float* s1scale = (lp && prec != PREC_FP32) ? c->stage1_scale.as<float>() : nullptr;
+  size_t per_atom = (size_t)k * vp * op * esz;
+  size_t tb = std::max<size_t>(per_atom, std::min<size_t>(c->budget, (size_t)8 << 30));
+  int NB = (int)std::max<size_t>(1, std::min<size_t>((size_t)N, tb / per_atom));
+  c->n_blocks = (int)ceil_div(N, NB);
+  // T must hold finite values everywhere: its pad rows/columns are read
+  // (against zero G / q_vv pads) but never written.
+  if (c->T.ensure((size_t)NB * per_atom)) launch_zero_bytes(c->T.p, c->T.bytes, c->stream);
+  for (int B0 = 0; B0 < N; B0 += NB) {
+    const int nb = std::min(NB, N - B0);
+    const int64_t NBop = (int64_t)nb * op;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    if (c->timing) {
+      e0 = c->get_event();
+      TDA_HIP(hipEventRecord(e0, c->stream));
+    }
+    // stage 1: T[c][a][(B,j)] = sum_b q_vv[B][a][b] X[c][j][b]
+    {
+      GemmArgs g = gargs((const char*)qvv + (size_t)B0 * vp * vp * esz, Xin, c->T.p, v, (int64_t)k * o, vp,
+                         plain(vp), plain(1), plain(1), split(o, ovp, vp), plain(NBop), split(o, (int64_t)vp * NBop, 1));
+      g.batch = nb;
+      g.a_batch = (int64_t)vp * vp;
+      g.b_batch = 0;
+      g.c_batch = op;
+      if (s1scale) g.batch_scale = s1scale + B0;
+      run_gemm(c, dt, dt, EPI_STORE, true, true, g);
+    }
+    if (c->timing) {
+      e1 = c->get_event();
+      TDA_HIP(hipEventRecord(e1, c->stream));
+    }
+    // stage 3: AX[c][i][a] -= sum_(B,j) G[B][i][j] T[c][a][(B,j)]
+    {
+      GemmArgs g = gargs((const char*)G + (size_t)B0 * op * op * esz, c->T.p, AX, o, (int64_t)k * v, NBop,
+                         plain(op), split(op, (int64_t)op * op, 1), plain(1), split(v, (int64_t)vp * NBop, NBop),
+                         plain(vp), split(v, ovp, 1));
+      g.alpha = -1.0;
+      g.beta = 1.0;
+      run_gemm(c, dt, DT_F64, EPI_STORE, true, true, g);
+    }
+    if (c->timing) {
+      e2 = c->get_event();
+      TDA_HIP(hipEventRecord(e2, c->stream));
+      c->ev_exch.push_back({e0, e1});
+      c->ev_exch_flops.push_back(2.0 * v * (double)k * o * v * nb);
+      c->ev_exch.push_back({e1, e2});
+      c->ev_exch_flops.push_back(2.0 * o * (double)k * v * o * nb);
+    }
+  }
+}
+
+void products(tda_ctx* c, const double* X, int k, double* AX, int prec) {
+  // chunk so that one call never holds more than 64 vectors of workspace
+  for (int c0 = 0; c0 < k; c0 += 64) {
+    const int kc = std::min(64, k - c0);
+    apply_A(c, X + (size_t)c0 * c->ovp, kc, AX + (size_t)c0 * c->ovp, prec);
+  }
+  c->n_products += k;
+  if (prec == PREC_FP64) c->n_products_fp64 += k;
+}
+
+// ---------------------------------------------------------------------------
+// Davidson (solvers/davidson.rs:53-185) on device-resident vectors
+// ---------------------------------------------------------------------------
+struct DavResult {
+  bool converged = false;
+  int iters = 0;
+  double max_err = 0;
+};
+
+// Runs the block Davidson with products in `prec` starting from the m0
+// vectors already in V.  `first_can_converge` relaxes the i>0 rule of the
+// reference (used by the FP64 continuation, which starts from Ritz vectors).
+// stop_stall > 0: leave early when the max residual has not dropped below
+// 0.5x its best value for that many iterations (low-precision phase).
+DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec, bool first_can_converge,
+                   int stop_stall, std::vector<double>& h_evals, std::vector<double>& h_err) {
+  const int64_t L = c->ovp;
+  const int cap = 15 * n;  // max_space (davidson.rs:74)
+  double* V = c->V.as<double>();
+  double* AV = c->AV.as<double>();
+  double* H = c->H.as<double>();
+  double* Y = c->Y.as<double>();
+  double* RZ = c->RZ.as<double>();
+  double* R = c->R.as<double>();
+  double* P = c->P.as<double>();
+  double* W = c->W.as<double>();
+  int* d_count = c->count.as<int>();
+  int m = m0, n_done = 0, dim_sub = m0;
+  const int dim_sub_origin = m0;
+  DavResult res;
+  std::vector<int> idx;
+  double best = 1e300;
+  int stall = 0;
+  for (int it = 0; it < max_iter; ++it) {
+    res.iters = it + 1;
+    // 1. products for the new columns (cache semantics solvers/mod.rs:290-302)
+    if (m <= n_done) n_done = 0;
+    {
+      Timer t;
+      TDA_HIP(hipEventRecord(t.a, c->stream));
+      products(c, V + (size_t)n_done * L, m - n_done, AV + (size_t)n_done * L, prec);
+      TDA_HIP(hipEventRecord(t.b, c->stream));
+      // 1.1 projection H[p][q] = V_p . AV_q for the new q (davidson.rs:89)
+      GemmArgs g = gargs(V, AV + (size_t)n_done * L, H + n_done, m, m - n_done, L, plain(L), plain(1), plain(1),
+                         plain(L), plain(cap), plain(1));
+      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
+      n_done = m;
+      // 2. subspace eigenproblem (davidson.rs:93-97)
+      launch_eigh(H, cap, m, n, c->eigw.as<double>(), c->evals.as<double>(), Y, cap, c->stream);
+      // 3.1 Ritz vectors RZ = Y V, 3.2 residuals R = Y AV - u RZ (davidson.rs:101-104)
+      GemmArgs gr = gargs(Y, V, RZ, n, L, m, plain(cap), plain(1), plain(L), plain(1), plain(L), plain(1));
+      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, gr);
+      GemmArgs ga = gargs(Y, AV, R, n, L, m, plain(cap), plain(1), plain(L), plain(1), plain(L), plain(1));
+      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, ga);
+      launch_residual(R, RZ, c->evals.as<double>(), n, L, c->stream);
+      // 3.3 norms (davidson.rs:107)
+      launch_norms(R, n, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
+      h_evals.resize(n);
+      h_err.resize(n);
+      TDA_HIP(hipMemcpyAsync(h_evals.data(), c->evals.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipMemcpyAsync(h_err.data(), c->norms.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipStreamSynchronize(c->stream));
+      float ms = 0;
+      TDA_HIP(hipEventElapsedTime(&ms, t.a, t.b));
+      c->t_products_ms += ms;
+      c->harvest_events();
+    }
+    for (double e : h_err)
+      TDA_CHECK(std::isfinite(e), TDA_ERR_HIP, "Davidson: non-finite residual norm");
+    int roots_cvd = 0;
+    double max_err = 0;
+    for (double e : h_err) {
+      roots_cvd += (e < tol);  // davidson.rs:115-117
+      max_err = std::max(max_err, e);
+    }
+    res.max_err = max_err;
+    const int roots_lft = n - roots_cvd;
+    if (roots_lft == 0 && (it > 0 || first_can_converge)) {  // davidson.rs:121
+      res.converged = true;
+      return res;
+    }
+    if (stop_stall > 0) {
+      if (max_err < 0.5 * best) {
+        best = max_err;
+        stall = 0;
+      } else if (++stall >= stop_stall) {
+        return res;
+      }
+    }
+    if (dim_sub + roots_lft <= cap) {  // davidson.rs:146
+      idx.clear();
+      for (int r = 0; r < n; ++r)
+        if (h_err[r] > tol) idx.push_back(r);  // davidson.rs:150-153
+      const int nc = (int)idx.size();
+      dim_sub += roots_lft;  // davidson.rs:159 (counts dropped vectors too)
+      if (nc > 0) {
+        TDA_HIP(hipMemcpyAsync(c->idx.p, idx.data(), nc * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        launch_precondition_many(R, c->evals.as<double>(), c->idx.as<int>(), nc, c->de.as<double>(), L, P,
+                                 c->stream);
+        // classical Gram-Schmidt (davidson.rs:162-168): W = P - V (V^T P) ...
+        double* C0 = c->stage2.as<double>();
+        GemmArgs g0 = gargs(V, P, C0, m, nc, L, plain(L), plain(1), plain(1), plain(L), plain(nc), plain(1));
+        run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g0);
+        launch_copy(P, W, (int64_t)nc * L, c->stream);
+        GemmArgs g1 = gargs(C0, V, W, nc, L, m, plain(1), plain(nc), plain(L), plain(1), plain(L), plain(1));
+        g1.alpha = -1.0;
+        g1.beta = 1.0;
+        run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, g1);
+        // ... then against the accepted new vectors, in order, 1e-8 drop
+        launch_zero_bytes(d_count, sizeof(int), c->stream);
+        for (int j = 0; j < nc; ++j)
+          launch_gs_step(P + (size_t)j * L, W + (size_t)j * L, V + (size_t)m * L, L, d_count,
+                         c->gs_scratch.as<double>(), n, c->stream);
+        int h_count = 0;
+        TDA_HIP(hipMemcpyAsync(&h_count, d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        TDA_HIP(hipStreamSynchronize(c->stream));
+        m += h_count;
+      }
+    } else {
+      // collapse to the Ritz vectors (davidson.rs:172-176)
+      dim_sub = dim_sub_origin;
+      launch_copy(RZ, V, (int64_t)dim_sub * L, c->stream);
+      m = dim_sub;
+      n_done = 0;
+    }
+  }
+  return res;
+}
+
+void alloc_davidson(tda_ctx* c, int n) {
+  const int64_t L = c->ovp;
+  const int cap = 15 * n;
+  c->V.ensure((size_t)cap * L * 8);
+  c->AV.ensure((size_t)cap * L * 8);
+  c->H.ensure((size_t)cap * cap * 8);
+  c->Y.ensure((size_t)n * cap * 8);
+  c->evals.ensure((size_t)n * 8);
+  c->RZ.ensure((size_t)n * L * 8);
+  c->R.ensure((size_t)n * L * 8);
+  c->P.ensure((size_t)n * L * 8);
+  c->W.ensure((size_t)n * L * 8);
+  c->norms.ensure((size_t)n * 8);
+  c->npart.ensure((size_t)n * 256 * 8);
+  c->idx.ensure((size_t)n * sizeof(int));
+  c->count.ensure(sizeof(int) * 4);
+  c->gs_scratch.ensure(((size_t)256 * n + n + 256 + 8) * 8);
+  c->eigw.ensure(eigh_work_doubles(cap) * 8);
+  c->stage2.ensure((size_t)cap * n * 8);
+}
+
+void check_sizes(int n_atoms, int n_ao, const int32_t* opa, int occ_begin, int n_occ, int virt_begin, int n_virt) {
+  TDA_CHECK(n_atoms > 0 && n_ao > 0, TDA_ERR_BAD_ARG, "n_atoms and n_ao must be positive");
+  TDA_CHECK(opa != nullptr, TDA_ERR_BAD_ARG, "orbs_per_atom is NULL");
+  int64_t s = 0;
+  for (int i = 0; i < n_atoms; ++i) {
+    TDA_CHECK(opa[i] >= 0, TDA_ERR_BAD_ARG, "negative orbs_per_atom");
+    s += opa[i];
+  }
+  TDA_CHECK(s == n_ao, TDA_ERR_BAD_ARG, "sum(orbs_per_atom) != n_ao");
+  TDA_CHECK(n_occ > 0 && n_virt > 0, TDA_ERR_BAD_ARG, "n_occ and n_virt must be positive");
+  TDA_CHECK(occ_begin >= 0 && occ_begin + n_occ <= n_ao, TDA_ERR_BAD_ARG, "occupied range outside the MOs");
+  TDA_CHECK(virt_begin >= 0 && virt_begin + n_virt <= n_ao, TDA_ERR_BAD_ARG, "virtual range outside the MOs");
+}
+
+std::vector<int> ao_offsets(int n_atoms, const int32_t* opa) {
+  std::vector<int> off(n_atoms + 1, 0);
+  for (int i = 0; i < n_atoms; ++i) off[i + 1] = off[i] + opa[i];
+  return off;
+}
+
+template <typename F>
+int guarded(tda_ctx* c, F&& f) {
+  try {
+    if (c) set_device(c);
+    f();
+    return TDA_OK;
+  } catch (const Error& e) {
+    if (c) c->err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (c) c->err = "host allocation failed";
+    return TDA_ERR_OOM;
+  } catch (...) {
+    if (c) c->err = "unknown error";
+    return TDA_ERR_HIP;
+  }
+}
+
+void copy_unpad_rows(const double* src, int rows, int cols, int ld, double* dst) {
+  for (int r = 0; r < rows; ++r) std::memcpy(dst + (size_t)r * cols, src + (size_t)r * ld, (size_t)cols * 8);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx** out) {
+  if (!out) return TDA_ERR_BAD_ARG;
+  *out = nullptr;
+  if (precision < TDA_PREC_FP64 || precision > TDA_PREC_FP16) return TDA_ERR_BAD_ARG;
+  tda_ctx* c = nullptr;
+  try {
+    c = new tda_ctx();
+  } catch (...) {
+    return TDA_ERR_OOM;
+  }
+  c->device = device;
+  c->prec = precision;
+  if (hbm_budget_bytes) c->budget = hbm_budget_bytes;
+  int rc = guarded(c, [&] {
+    int ndev = 0;
+    TDA_HIP(hipGetDeviceCount(&ndev));
+    TDA_CHECK(device >= 0 && device < ndev, TDA_ERR_BAD_ARG, "no such HIP device");
+    TDA_HIP(hipSetDevice(device));
+    TDA_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  });
+  if (rc != TDA_OK) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return TDA_OK;
+}
+
+void tda_ctx_destroy(tda_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  delete c;
+}
+
+const char* tda_last_error(const tda_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int tda_set_precision(tda_ctx* c, int precision) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  if (precision < TDA_PREC_FP64 || precision > TDA_PREC_FP16) {
+    c->err = "bad precision";
+    return TDA_ERR_BAD_ARG;
+  }
+  c->prec = precision;
+  return TDA_OK;
+}
+
+int tda_orbe_differences(const double* occ, int n_occ, const double* virt, int n_virt, double* omega) {
+  if (!occ || !virt || !omega || n_occ < 0 || n_virt < 0) return TDA_ERR_BAD_ARG;
+  for (int i = 0; i < n_occ; ++i)
+    for (int a = 0; a < n_virt; ++a) omega[(size_t)i * n_virt + a] = virt[a] - occ[i];
+  return TDA_OK;
+}
+
+int tda_trans_charges(tda_ctx* c, int n_atoms, int n_ao, const int32_t* opa, const double* orbs, const double* s,
+                      int occ_begin, int n_occ, int virt_begin, int n_virt, double* q_ov, double* q_oo,
+                      double* q_vv) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] {
+    check_sizes(n_atoms, n_ao, opa, occ_begin, n_occ, virt_begin, n_virt);
+    TDA_CHECK(orbs && s, TDA_ERR_BAD_ARG, "orbs or s is NULL");
+    DevBuf dC, dS, dSC, dao, dq;
+    const size_t nn = (size_t)n_ao * n_ao;
+    dC.ensure(nn * 8);
+    dS.ensure(nn * 8);
+    dSC.ensure(nn * 8);
+    std::vector<int> off = ao_offsets(n_atoms, opa);
+    dao.ensure(off.size() * sizeof(int));
+    TDA_HIP(hipMemcpyAsync(dC.p, orbs, nn * 8, hipMemcpyHostToDevice, c->stream));
+    TDA_HIP(hipMemcpyAsync(dS.p, s, nn * 8, hipMemcpyHostToDevice, c->stream));
+    TDA_HIP(hipMemcpyAsync(dao.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    gemm_sc(c, dS.as<double>(), dC.as<double>(), dSC.as<double>(), n_ao);
+    struct Blk { double* out; int pb, P, qb, Q; };
+    Blk blks[3] = {{q_ov, occ_begin, n_occ, virt_begin, n_virt},
+                   {q_oo, occ_begin, n_occ, occ_begin, n_occ},
+                   {q_vv, virt_begin, n_virt, virt_begin, n_virt}};
+    for (const Blk& b : blks) {
+      if (!b.out) continue;
+      const size_t cnt = (size_t)n_atoms * b.P * b.Q;
+      dq.ensure(cnt * 8);
+      launch_qblock(dC.as<double>(), dSC.as<double>(), n_ao, dao.as<int>(), n_atoms, b.pb, b.P, b.P, b.qb, b.Q,
+                    b.Q, dq.as<double>(), c->stream);
+      TDA_HIP(hipMemcpyAsync(b.out, dq.p, cnt * 8, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipStreamSynchronize(c->stream));
+    }
+    TDA_HIP(hipStreamSynchronize(c->stream));
+    for (DevBuf* b : {&dC, &dS, &dSC, &dao, &dq}) b->release();
+  });
+}
+
+int tda_set_system(tda_ctx* c, int n_atoms, int n_ao, const int32_t* opa, const double* orbs, const double* s,
+                   const double* orbe, int occ_begin, int n_occ, int virt_begin, int n_virt, const double* gamma,
+                   const double* gamma_lr, const double* xyz) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] {
+    check_sizes(n_atoms, n_ao, opa, occ_begin, n_occ, virt_begin, n_virt);
+    TDA_CHECK(orbs && s && orbe && gamma && xyz, TDA_ERR_BAD_ARG, "NULL input array");
+    c->has_system = false;
+    c->prepared = false;
+    c->lp_prec = -1;
+    c->N = n_atoms;
+    c->nao = n_ao;
+    c->ob = occ_begin;
+    c->o = n_occ;
+    c->vb = virt_begin;
+    c->v = n_virt;
+    c->op = (int)round_up(n_occ, 64);
+    c->vp = (int)round_up(n_virt, 64);
+    c->ovp = (int64_t)c->o * c->vp;
+    c->has_lc = gamma_lr != nullptr;
+    const size_t nn = (size_t)n_ao * n_ao;
+    c->C.ensure(nn * 8);
+    c->S.ensure(nn * 8);
+    c->orbe.ensure((size_t)n_ao * 8);
+    c->gamma.ensure((size_t)n_atoms * n_atoms * 8);
+    std::vector<int> off = ao_offsets(n_atoms, opa);
+    c->aooff.ensure(off.size() * sizeof(int));
+    TDA_HIP(hipMemcpyAsync(c->C.p, orbs, nn * 8, hipMemcpyHostToDevice, c->stream));
+    TDA_HIP(hipMemcpyAsync(c->S.p, s, nn * 8, hipMemcpyHostToDevice, c->stream));
+    TDA_HIP(hipMemcpyAsync(c->orbe.p, orbe, (size_t)n_ao * 8, hipMemcpyHostToDevice, c->stream));
+    TDA_HIP(hipMemcpyAsync(c->gamma.p, gamma, (size_t)n_atoms * n_atoms * 8, hipMemcpyHostToDevice, c->stream));
+    TDA_HIP(hipMemcpyAsync(c->aooff.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (c->has_lc) {
+      c->gamma_lr.ensure((size_t)n_atoms * n_atoms * 8);
+      TDA_HIP(hipMemcpyAsync(c->gamma_lr.p, gamma_lr, (size_t)n_atoms * n_atoms * 8, hipMemcpyHostToDevice,
+                             c->stream));
+    }
+    c->xyz.assign(xyz, xyz + (size_t)n_atoms * 3);
+    c->h_de.resize((size_t)n_occ * n_virt);
+    for (int i = 0; i < n_occ; ++i)
+      for (int a = 0; a < n_virt; ++a) c->h_de[(size_t)i * n_virt + a] = orbe[virt_begin + a] - orbe[occ_begin + i];
+    TDA_HIP(hipStreamSynchronize(c->stream));
+    c->has_system = true;
+  });
+}
+
+int tda_prepare(tda_ctx* c) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] { prepare(c); });
+}
+
+int tda_get_charges(tda_ctx* c, double* q_ov, double* q_oo, double* q_vv, double* omega) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] {
+    prepare(c);
+    const int N = c->N, o = c->o, v = c->v, op = c->op, vp = c->vp;
+    if (q_ov) {
+      std::vector<double> h((size_t)N * c->ovp);
+      TDA_HIP(hipMemcpy(h.data(), c->qov.p, h.size() * 8, hipMemcpyDeviceToHost));
+      copy_unpad_rows(h.data(), N * o, v, vp, q_ov);
+    }
+    if ((q_oo || q_vv) && !c->has_lc) throw Error{TDA_ERR_STATE, "q_oo/q_vv are built only with gamma_lr"};
+    if (q_oo) {
+      std::vector<double> h((size_t)N * op * op);
+      TDA_HIP(hipMemcpy(h.data(), c->qoo.p, h.size() * 8, hipMemcpyDeviceToHost));
+      for (int A = 0; A < N; ++A) copy_unpad_rows(h.data() + (size_t)A * op * op, o, o, op, q_oo + (size_t)A * o * o);
+    }
+    if (q_vv) {
+      std::vector<double> h((size_t)N * vp * vp);
+      TDA_HIP(hipMemcpy(h.data(), c->qvv.p, h.size() * 8, hipMemcpyDeviceToHost));
+      for (int A = 0; A < N; ++A) copy_unpad_rows(h.data() + (size_t)A * vp * vp, v, v, vp, q_vv + (size_t)A * v * v);
+    }
+    if (omega) std::memcpy(omega, c->h_de.data(), c->h_de.size() * 8);
+  });
+}
+
+int tda_compute_products(tda_ctx* c, const double* x, int k, double* ax) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] {
+    TDA_CHECK(k >= 0 && (k == 0 || (x && ax)), TDA_ERR_BAD_ARG, "bad x/ax/k");
+    prepare(c);
+    if (k == 0) return;
+    const int64_t ov = (int64_t)c->o * c->v;
+    c->stage.ensure((size_t)ov * k * 8);
+    c->ax_tmp.ensure((size_t)c->ovp * k * 8 * 2);
+    double* dX = c->ax_tmp.as<double>();
+    double* dAX = dX + (size_t)c->ovp * k;
+    TDA_HIP(hipMemcpyAsync(c->stage.p, x, (size_t)ov * k * 8, hipMemcpyHostToDevice, c->stream));
+    launch_pack_x(c->stage.as<double>(), c->o, c->v, c->vp, k, dX, c->stream);
+    products(c, dX, k, dAX, c->prec);
+    launch_unpack_x(dAX, c->o, c->v, c->vp, k, c->stage.as<double>(), c->stream);
+    TDA_HIP(hipMemcpyAsync(ax, c->stage.p, (size_t)ov * k * 8, hipMemcpyDeviceToHost, c->stream));
+    TDA_HIP(hipStreamSynchronize(c->stream));
+    c->harvest_events();
+  });
+}
+
+int tda_precondition(tda_ctx* c, const double* r, double w, double* out) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] {
+    TDA_CHECK(c->has_system, TDA_ERR_STATE, "no system loaded");
+    TDA_CHECK(r && out, TDA_ERR_BAD_ARG, "NULL r/out");
+    const size_t ov = c->h_de.size();
+    for (size_t t = 0; t < ov; ++t) {
+      const double d = w - c->h_de[t];
+      out[t] = r[t] / (std::fabs(d) < 0.0001 ? 1.0 : d);
+    }
+  });
+}
+
+int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, double* X, double* q_trans,
+            double* tr_dipoles, double* f, tda_stats* stats) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  const auto t0 = std::chrono::steady_clock::now();
+  DavResult res, res64;
+  int rc = guarded(c, [&] {
+    TDA_CHECK(c->has_system, TDA_ERR_STATE, "tda_run: no system loaded");
+    const int64_t ov = (int64_t)c->o * c->v;
+    TDA_CHECK(n_roots >= 1 && n_roots <= 64 && n_roots <= ov, TDA_ERR_BAD_ARG,
+              "n_roots must be in 1..min(64, n_occ*n_virt)");
+    TDA_CHECK(max_iter >= 1 && tol > 0.0, TDA_ERR_BAD_ARG, "bad max_iter / tolerance");
+    c->t_products_ms = c->t_exchange_ms = c->exchange_flops = 0;
+    c->n_exchange_launches = c->n_products = c->n_products_fp64 = 0;
+    const auto tq0 = std::chrono::steady_clock::now();
+    prepare(c);
+    const double tq = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tq0).count();
+    const int n = n_roots;
+    const int64_t L = c->ovp;
+    alloc_davidson(c, n);
+    // initial subspace: unit vectors at the n smallest omega (solvers/utils.rs:17-26)
+    std::vector<int64_t> order(ov);
+    std::iota(order.begin(), order.end(), 0);
+    std::partial_sort(order.begin(), order.begin() + n, order.end(), [&](int64_t a, int64_t b) {
+      return c->h_de[a] < c->h_de[b] || (c->h_de[a] == c->h_de[b] && a < b);
+    });
+    launch_fill(c->V.as<double>(), 0.0, (int64_t)n * L, c->stream);
+    std::vector<double> one(1, 1.0);
+    for (int r = 0; r < n; ++r) {
+      const int64_t ia = order[r];
+      const int64_t pos = (ia / c->v) * c->vp + (ia % c->v);
+      TDA_HIP(hipMemcpyAsync(c->V.as<double>() + (size_t)r * L + pos, one.data(), 8, hipMemcpyHostToDevice,
+                             c->stream));
+    }
+    TDA_HIP(hipStreamSynchronize(c->stream));
+    std::vector<double> h_evals, h_err;
+    if (c->prec == PREC_FP64) {
+      res = davidson(c, n, n, max_iter, tol, PREC_FP64, false, 0, h_evals, h_err);
+      res64 = res;
+    } else {
+      // low-precision phase, then FP64 continuation from the Ritz vectors
+      res = davidson(c, n, n, max_iter, tol, c->prec, false, 3, h_evals, h_err);
+      launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
+      const int left = std::max(1, max_iter - res.iters);
+      res64 = davidson(c, n, n, left, tol, PREC_FP64, true, 0, h_evals, h_err);
+    }
+    // outputs (tda/mod.rs:88-126)
+    const int N = c->N;
+    if (omega) std::memcpy(omega, h_evals.data(), (size_t)n * 8);
+    if (X) {
+      c->stage.ensure((size_t)ov * n * 8);
+      launch_unpack_x(c->RZ.as<double>(), c->o, c->v, c->vp, n, c->stage.as<double>(), c->stream);
+      TDA_HIP(hipMemcpyAsync(X, c->stage.p, (size_t)ov * n * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    std::vector<double> qt((size_t)N * n);
+    c->u.ensure((size_t)N * n * 8);
+    GemmArgs g = gargs(c->qov.p, c->RZ.p, c->u.p, N, n, L, plain(L), plain(1), plain(1), plain(L), plain(n), plain(1));
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
+    TDA_HIP(hipMemcpyAsync(qt.data(), c->u.p, qt.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    TDA_HIP(hipStreamSynchronize(c->stream));
+    if (q_trans) std::memcpy(q_trans, qt.data(), qt.size() * 8);
+    std::vector<double> mu((size_t)3 * n, 0.0);
+    for (int d = 0; d < 3; ++d)
+      for (int r = 0; r < n; ++r) {
+        double s = 0;
+        for (int A = 0; A < N; ++A) s += c->xyz[(size_t)A * 3 + d] * qt[(size_t)A * n + r];
+        mu[(size_t)d * n + r] = s;
+      }
+    if (tr_dipoles) std::memcpy(tr_dipoles, mu.data(), mu.size() * 8);
+    if (f)
+      for (int r = 0; r < n; ++r) {
+        double m2 = 0;
+        for (int d = 0; d < 3; ++d) m2 += mu[(size_t)d * n + r] * mu[(size_t)d * n + r];
+        f[r] = 2.0 / 3.0 * h_evals[r] * m2;
+      }
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      stats->n_iter = res.iters + (c->prec == PREC_FP64 ? 0 : res64.iters);
+      stats->n_iter_fp64 = c->prec == PREC_FP64 ? res.iters : res64.iters;
+      stats->n_products = c->n_products;
+      stats->n_products_fp64 = c->n_products_fp64;
+      stats->converged = res64.converged ? 1 : 0;
+      stats->n_atom_blocks = c->n_blocks;
+      stats->max_residual = res64.max_err;
+      stats->t_qbuild_ms = tq;
+      stats->t_products_ms = c->t_products_ms;
+      stats->t_exchange_ms = c->t_exchange_ms;
+      stats->n_exchange_launches = c->n_exchange_launches;
+      stats->exchange_flops = c->exchange_flops;
+    }
+    if (!res64.converged) throw Error{TDA_ERR_NOT_CONVERGED, "Davidson Algorithm did not converge!"};
+  });
+  if (stats)
+    stats->t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+}  // extern "C"

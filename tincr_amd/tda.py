@@ -1,0 +1,165 @@
+"""Host-side mirror of tincr's excited-state interface, backed by the HIP library.
+
+Names, argument meaning and error behaviour follow the reference:
+
+* ``trans_charges``      src/excited_states/transition_charges.rs:8-15
+* ``orbe_differences``   src/excited_states/utils.rs:8-24
+* ``TdaSystem``          the DavidsonEngine of System / Monomer
+  (src/excited_states/solvers/mod.rs:20-34, :278-385) together with
+  ``prepare_tda`` (tda/system.rs:10-52) and ``run_tda`` (tda/mod.rs:74-131);
+  results land in ``properties`` under the reference's keys
+  (``ci_eigenvalues``, ``ci_coefficients``, ``q_trans``, ``tr_dipoles``,
+  ``oscillator_strengths``, ``q_ov``/``q_oo``/``q_vv``, ``omega``, ``homo``,
+  ``lumo``).
+* ``DavidsonError``      src/excited_states/solvers/davidson.rs:28-37 -- raised
+  where the reference's callers ``unwrap()`` a non-converged Davidson.
+
+All arithmetic runs on the GPU through libtincr_tda.so; there is no CPU path.
+"""
+import numpy as np
+
+from . import _native as nat
+
+
+class DavidsonError(RuntimeError):
+    """Davidson Algorithm did not converge!"""
+
+
+def _ranges(occ_indices, virt_indices):
+    occ_indices = list(occ_indices)
+    virt_indices = list(virt_indices)
+    ob, oe = occ_indices[0], occ_indices[-1] + 1
+    vb, ve = virt_indices[0], virt_indices[-1] + 1
+    return ob, oe - ob, vb, ve - vb
+
+
+_shared_ctx = None
+
+
+def _ctx():
+    global _shared_ctx
+    if _shared_ctx is None:
+        _shared_ctx = nat.Context(0, "fp64")
+    return _shared_ctx
+
+
+def trans_charges(n_atoms, orbs_per_atom, orbs, s, occ_indices, virt_indices, ctx=None):
+    """(q_ov [N, o*v], q_oo [N, o*o], q_vv [N, v*v]) on the GPU.
+
+    ``orbs_per_atom`` replaces the reference's ``atoms[].n_orbs``; occupied and
+    virtual indices are used as contiguous ranges, as the reference does.
+    """
+    ctx = ctx or _ctx()
+    ob, o, vb, v = _ranges(occ_indices, virt_indices)
+    opa = np.ascontiguousarray(orbs_per_atom, dtype=np.int32)
+    orbs = nat.f64(orbs)
+    s = nat.f64(s)
+    n_ao = orbs.shape[0]
+    q_ov = np.empty((n_atoms, o * v))
+    q_oo = np.empty((n_atoms, o * o))
+    q_vv = np.empty((n_atoms, v * v))
+    ctx.check(ctx.lib.tda_trans_charges(ctx.handle, n_atoms, n_ao, nat.iptr(opa), nat.dptr(orbs), nat.dptr(s),
+                                        ob, o, vb, v, nat.dptr(q_ov), nat.dptr(q_oo), nat.dptr(q_vv)))
+    return q_ov, q_oo, q_vv
+
+
+def orbe_differences(occ_quant, virt_quant):
+    occ = nat.f64(occ_quant)
+    virt = nat.f64(virt_quant)
+    out = np.empty(len(occ) * len(virt))
+    lib = nat.load()
+    rc = lib.tda_orbe_differences(nat.dptr(occ), len(occ), nat.dptr(virt), len(virt), nat.dptr(out))
+    if rc != nat.TDA_OK:
+        raise nat.TdaError(rc, "tda_orbe_differences failed")
+    return out
+
+
+class TdaSystem:
+    """DavidsonEngine + prepare_tda/run_tda of one System or Monomer."""
+
+    def __init__(self, n_atoms, orbs_per_atom, xyz, occ_indices, virt_indices, orbs, s, orbe, gamma,
+                 gamma_lr=None, precision="fp64", device=0, hbm_budget_bytes=0):
+        self.ctx = nat.Context(device, precision, hbm_budget_bytes)
+        self.n_atoms = n_atoms
+        self.occ_indices = list(occ_indices)
+        self.virt_indices = list(virt_indices)
+        self.ob, self.n_occ, self.vb, self.n_virt = _ranges(occ_indices, virt_indices)
+        self.opa = np.ascontiguousarray(orbs_per_atom, dtype=np.int32)
+        self.xyz = nat.f64(np.asarray(xyz).reshape(n_atoms, 3))
+        self._orbs = nat.f64(orbs)
+        self._s = nat.f64(s)
+        self._orbe = nat.f64(np.ravel(orbe))
+        self._gamma = nat.f64(gamma)
+        self._gamma_lr = nat.f64(gamma_lr)
+        self.gammafunction_lc = gamma_lr is not None
+        self.properties = {}
+        self.last_stats = None
+        n_ao = self._orbs.shape[0]
+        self.ctx.check(self.ctx.lib.tda_set_system(
+            self.ctx.handle, n_atoms, n_ao, nat.iptr(self.opa), nat.dptr(self._orbs), nat.dptr(self._s),
+            nat.dptr(self._orbe), self.ob, self.n_occ, self.vb, self.n_virt, nat.dptr(self._gamma),
+            nat.dptr(self._gamma_lr), nat.dptr(self.xyz)))
+
+    def close(self):
+        self.ctx.close()
+
+    # -- prepare_tda (tda/system.rs:10-52) ---------------------------------
+    def prepare_tda(self, fetch_charges=True):
+        self.ctx.check(self.ctx.lib.tda_prepare(self.ctx.handle))
+        N, o, v = self.n_atoms, self.n_occ, self.n_virt
+        omega = np.empty(o * v)
+        if fetch_charges:
+            q_ov = np.empty((N, o * v))
+            q_oo = np.empty((N, o * o)) if self.gammafunction_lc else None
+            q_vv = np.empty((N, v * v)) if self.gammafunction_lc else None
+            self.ctx.check(self.ctx.lib.tda_get_charges(self.ctx.handle, nat.dptr(q_ov), nat.dptr(q_oo),
+                                                        nat.dptr(q_vv), nat.dptr(omega)))
+            self.properties.update(q_ov=q_ov, q_oo=q_oo, q_vv=q_vv)
+        else:
+            self.ctx.check(self.ctx.lib.tda_get_charges(self.ctx.handle, None, None, None, nat.dptr(omega)))
+        self.properties.update(omega=omega, homo=self.occ_indices[-1], lumo=self.virt_indices[0])
+
+    # -- DavidsonEngine (solvers/mod.rs:20-34) --------------------------------
+    def compute_products(self, x):
+        """A x for every column of x [ov, k] (operator level, no cache)."""
+        x = nat.f64(x)
+        squeeze = x.ndim == 1
+        if squeeze:
+            x = x[:, None].copy()
+        k = x.shape[1]
+        ax = np.empty_like(x)
+        self.ctx.check(self.ctx.lib.tda_compute_products(self.ctx.handle, nat.dptr(x), k, nat.dptr(ax)))
+        return ax[:, 0] if squeeze else ax
+
+    def precondition(self, r_k, w_k):
+        r = nat.f64(r_k)
+        out = np.empty_like(r)
+        self.ctx.check(self.ctx.lib.tda_precondition(self.ctx.handle, nat.dptr(r), float(w_k), nat.dptr(out)))
+        return out
+
+    def get_size(self):
+        return self.n_occ * self.n_virt
+
+    # -- run_tda (tda/mod.rs:74-131) ----------------------------------------
+    def run_tda(self, n_roots, max_iter, tolerance):
+        N, ov = self.n_atoms, self.n_occ * self.n_virt
+        omega = np.empty(n_roots)
+        X = np.empty((ov, n_roots))
+        q_trans = np.empty((N, n_roots))
+        tr_dip = np.empty((3, n_roots))
+        f = np.empty(n_roots)
+        st = nat.TdaStats()
+        rc = self.ctx.lib.tda_run(self.ctx.handle, n_roots, max_iter, float(tolerance), nat.dptr(omega),
+                                  nat.dptr(X), nat.dptr(q_trans), nat.dptr(tr_dip), nat.dptr(f),
+                                  nat.ctypes.byref(st))
+        self.last_stats = st.as_dict()
+        if rc == nat.TDA_ERR_NOT_CONVERGED:
+            raise DavidsonError("Davidson Algorithm did not converge!")
+        self.ctx.check(rc)
+        self.properties.update(ci_eigenvalues=omega, ci_coefficients=X, q_trans=q_trans, tr_dipoles=tr_dip,
+                               oscillator_strengths=f)
+        return omega, X
+
+    def tdm(self, idx):
+        """Transition density of state idx as [n_occ, n_virt] (properties/getter.rs:130-148)."""
+        return self.properties["ci_coefficients"][:, idx].reshape(self.n_occ, self.n_virt)
