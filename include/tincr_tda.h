@@ -49,9 +49,13 @@ typedef struct tda_stats {
   double t_total_ms;        /* tda_run wall time */
   double t_qbuild_ms;       /* q build + gamma_lr.q_oo (inside tda_run if not prepared) */
   double t_products_ms;     /* sum of A.x product time (device events) */
-  double t_exchange_ms;     /* sum of exchange-stage GEMM time (device events) */
-  int32_t n_exchange_launches;
-  double exchange_flops;    /* algorithmic flops of those launches */
+  /* exchange contraction kernels, [stage][class]: stage 0 = X.q_vv (stage 1),
+   * 1 = G^T.T (stage 3); class 0 = FP64 products, 1 = low-precision products.
+   * Time of the main MFMA kernel only (device events on the context stream),
+   * its launch count and its algorithmic flops. */
+  double exch_ms[2][2];
+  int32_t exch_launches[2][2];
+  double exch_flops[2][2];
 } tda_stats;
 
 /* Create a context on HIP device `device`.  hbm_budget_bytes bounds the
@@ -88,6 +92,11 @@ int tda_set_system(tda_ctx* ctx, int n_atoms, int n_ao, const int32_t* orbs_per_
  * gamma_lr.q_oo, once per system instead of once per product call as
  * solvers/mod.rs:328-331 does). */
 int tda_prepare(tda_ctx* ctx);
+
+/* Drops the prepared charges, as Properties::reset does for q_ov/q_oo/q_vv/
+ * omega (src/properties/mod.rs:23-63): the next tda_prepare / tda_run
+ * rebuilds them from the loaded orbitals. */
+int tda_clear_charges(tda_ctx* ctx);
 
 /* Copies the prepared charges / omega back (reference layouts; NULL skips). */
 int tda_get_charges(tda_ctx* ctx, double* q_ov, double* q_oo, double* q_vv, double* omega);

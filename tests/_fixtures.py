@@ -6,6 +6,12 @@ import numpy as np
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 BOHR_TO_ANGS = 0.529177210903   # src/constants.rs:4
+# The tests/data fixtures were generated with the older factor used by the
+# legacy tests (not_refactored_tests/solver.rs:34) and these Hubbard values;
+# with them gamma_atomwise{,_lc,_grad,_grad_lc}.dat reproduce to 1e-16.
+FIXTURE_BOHR = 0.529177249
+FIXTURE_HUBBARD = {1: 0.47201583989641355, 6: 0.36742624125200113, 7: 0.5299999999999999,
+                   8: 0.4467609798860577}
 MOLECULES = ["h2o", "ammonia", "benzene", "uracil"]
 ELEMENT_Z = {"H": 1, "C": 6, "N": 7, "O": 8}
 
@@ -24,7 +30,7 @@ def load_xyz(mol):
         p = ln.split()
         z.append(ELEMENT_Z[p[0]])
         xyz.append([float(t) for t in p[1:4]])
-    return np.array(z), np.array(xyz) / BOHR_TO_ANGS
+    return np.array(z), np.array(xyz) / FIXTURE_BOHR
 
 
 def molecule(mol):

@@ -1,0 +1,218 @@
+"""End-to-end GPU parity of the LC-TDA path against the CPU oracle.
+
+Tolerances (FP64 path): transition charges 1e-14 absolute; A.x 1e-12
+relative to |A| |x|; excitation energies 1e-10 Ha against the dense closed form
+(the reference's tda_with_lc intent) and 1e-6 eV against the oracle Davidson;
+identical state ordering.  Mixed precision: max |omega - omega_FP64| <= 1 meV.
+"""
+import numpy as np
+import pytest
+
+from oracle import tda_oracle as orc
+from tests._fixtures import MOLECULES, legacy, molecule
+
+pytestmark = pytest.mark.gpu
+
+HARTREE_TO_EV = orc.HARTREE_TO_EV
+
+
+def _tda():
+    from tincr_amd import tda
+    return tda
+
+
+def _oracle_engine(m, lc=True):
+    q_ov, q_oo, q_vv = orc.trans_charges(m["n_atoms"], m["orbs_per_atom"], m["orbs"], m["s"],
+                                         m["occ_indices"], m["virt_indices"])
+    orbe = np.ravel(m["orbe"])
+    homo, lumo = m["occ_indices"][-1], m["virt_indices"][0]
+    omega = orc.orbe_differences(orbe[:homo + 1], orbe[lumo:])
+    return orc.SystemEngine(q_ov, q_oo, q_vv, m["gamma"], m["gamma_lr"] if lc else None, omega,
+                            len(m["occ_indices"]), len(m["virt_indices"]), faithful=False)
+
+
+def _system(m, precision="fp64", lc=True, **kw):
+    tda = _tda()
+    return tda.TdaSystem(m["n_atoms"], m["orbs_per_atom"], m["xyz"], m["occ_indices"], m["virt_indices"],
+                         m["orbs"], m["s"], m["orbe"], m["gamma"], m["gamma_lr"] if lc else None,
+                         precision=precision, **kw)
+
+
+def _synthetic(builder, n, seed=0):
+    from tincr_amd import systems as sy
+    mol = builder(n, seed=seed)
+    d = sy.tda_inputs(mol, seed=seed)
+    d["name"] = mol.name
+    d["orbs_per_atom"] = mol.orbs_per_atom
+    return d
+
+
+def _cases():
+    return MOLECULES
+
+
+@pytest.mark.parametrize("mol", MOLECULES)
+def test_trans_charges(mol):
+    print(f"trans_charges {mol}", flush=True)
+    m = molecule(mol)
+    tda = _tda()
+    got = tda.trans_charges(m["n_atoms"], m["orbs_per_atom"], m["orbs"], m["s"], m["occ_indices"],
+                            m["virt_indices"])
+    ref = orc.trans_charges(m["n_atoms"], m["orbs_per_atom"], m["orbs"], m["s"], m["occ_indices"],
+                            m["virt_indices"])
+    for g, r in zip(got, ref):
+        assert g.shape == r.shape
+        assert np.abs(g - r).max() < 1e-14
+
+
+def test_trans_charges_legacy_water_golden_path():
+    """Charges from the legacy water orbitals reproduce the 8 golden omegas
+    (not_refactored_tests/solver.rs:33-199) when fed to the oracle's dense A."""
+    g = legacy("legacy_water_tda.json")
+    w = molecule("h2o")
+    q = _tda().trans_charges(3, [4, 1, 1], g["orbs"], g["S"], [0, 1, 2, 3], [4, 5])
+    omega = orc.orbe_differences(g["orbe"][:4], g["orbe"][4:])
+    eng = orc.SystemEngine(*q, w["gamma"], w["gamma_lr"], omega, 4, 2)
+    assert np.abs(np.linalg.eigh(eng.dense())[0] - g["omega_ref_out"]).max() < 1e-10
+
+
+@pytest.mark.parametrize("mol", MOLECULES)
+@pytest.mark.parametrize("lc", [True, False])
+def test_products_fixtures(mol, lc):
+    print(f"products {mol} lc={lc}", flush=True)
+    m = molecule(mol)
+    sysg = _system(m, lc=lc)
+    sysg.prepare_tda()
+    eng = _oracle_engine(m, lc)
+    ov = eng.get_size()
+    x = np.random.default_rng(3).standard_normal((ov, 7))
+    ax = sysg.compute_products(x)
+    ref = eng.apply(x)
+    scale = np.abs(ref).max()
+    assert np.abs(ax - ref).max() < 1e-12 * max(scale, 1.0)
+    assert np.abs(sysg.properties["q_ov"] - eng.q_ov).max() < 1e-14
+    if lc:
+        assert np.abs(sysg.properties["q_oo"] - eng.q_oo).max() < 1e-14
+        assert np.abs(sysg.properties["q_vv"] - eng.q_vv).max() < 1e-14
+    assert np.array_equal(sysg.properties["omega"], eng.omega)
+    r = np.random.default_rng(4).standard_normal(ov)
+    for w in (eng.omega[3], 0.5):
+        assert np.abs(sysg.precondition(r, w) - eng.precondition(r, w)).max() == 0.0
+    sysg.close()
+
+
+def test_products_benzene_apbv_golden():
+    """A.x through the GPU charges of benzene's fixture orbitals equals the
+    oracle; the oracle itself is pinned by the (A+B).v golden."""
+    m = molecule("benzene")
+    sysg = _system(m)
+    eng = _oracle_engine(m)
+    x = np.random.default_rng(5).standard_normal((eng.get_size(), 3))
+    assert np.abs(sysg.compute_products(x) - eng.apply(x)).max() < 1e-13
+    sysg.close()
+
+
+@pytest.mark.parametrize("mol", MOLECULES)
+def test_run_tda_fixtures(mol):
+    print(f"run_tda {mol}", flush=True)
+    m = molecule(mol)
+    sysg = _system(m)
+    sysg.prepare_tda()
+    n = 4
+    w, x = sysg.run_tda(n, 200, 1e-8)
+    eng = _oracle_engine(m)
+    dense = orc.tda_full_matrix(eng, m["xyz"])
+    assert np.abs(w - dense["omega"][:n]).max() < 1e-10
+    for s in range(n):
+        assert abs(abs(x[:, s] @ dense["X"][:, s]) - 1.0) < 1e-6
+    ref = orc.run_tda(_oracle_engine(m), m["xyz"], n, 200, 1e-8)
+    assert np.abs(w - ref["omega"]).max() * HARTREE_TO_EV < 1e-6
+    f = sysg.properties["oscillator_strengths"]
+    assert np.abs(f - ref["f"]).max() < 1e-8
+    assert np.abs(sysg.properties["q_trans"] - eng.q_ov @ x).max() < 1e-13
+    sysg.close()
+
+
+def test_run_tda_not_converged_raises():
+    m = molecule("benzene")
+    sysg = _system(m)
+    tda = _tda()
+    with pytest.raises(tda.DavidsonError):
+        sysg.run_tda(4, 1, 1e-8)
+    sysg.close()
+
+
+def test_run_tda_bad_args():
+    from tincr_amd import _native as nat
+    m = molecule("h2o")
+    sysg = _system(m)
+    with pytest.raises(nat.TdaError) as e:
+        sysg.run_tda(100, 10, 1e-5)   # n_roots > ov
+    assert e.value.code == nat.TDA_ERR_BAD_ARG
+    sysg.close()
+
+
+@pytest.mark.parametrize("case", ["water16", "polyene20"])
+def test_run_tda_synthetic_vs_oracle(case):
+    """Config-0-sized (H2O)_16 and a C20H22 polyene with synthetic orbitals:
+    GPU Davidson vs the oracle Davidson (same guess, same rules)."""
+    from tincr_amd import systems as sy
+    print(f"synthetic {case}", flush=True)
+    m = _synthetic(sy.water_cluster, 16) if case == "water16" else _synthetic(sy.polyene, 20)
+    n = 10
+    sysg = _system(m)
+    w, x = sysg.run_tda(n, 100, 1e-5)
+    ref = orc.run_tda(_oracle_engine(m), m["xyz"], n, 100, 1e-5)
+    assert np.abs(w - ref["omega"]).max() * HARTREE_TO_EV < 1e-6
+    for s in range(n):
+        assert abs(abs(x[:, s] @ ref["X"][:, s]) - 1.0) < 1e-4
+    st = sysg.last_stats
+    assert st["converged"] == 1 and st["n_iter"] == ref["n_iter"]
+    sysg.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+def test_products_mixed_precision(prec):
+    from tincr_amd import systems as sy
+    m = _synthetic(sy.water_cluster, 16)
+    sysg = _system(m, precision=prec)
+    eng = _oracle_engine(m)
+    x = np.random.default_rng(9).standard_normal((eng.get_size(), 5))
+    x /= np.linalg.norm(x, axis=0)
+    err = np.abs(sysg.compute_products(x) - eng.apply(x)).max()
+    tol = {"fp32": 1e-6, "bf16": 2e-2, "fp16": 3e-3}[prec]
+    assert err < tol, err
+    sysg.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("mol", ["uracil", "benzene"])
+def test_run_tda_mixed_precision(prec, mol):
+    """Mixed-precision Davidson + FP64 continuation: omega within 1 meV of
+    FP64 (in practice the FP64 continuation gives ~1e-10 Ha)."""
+    m = molecule(mol)
+    s64 = _system(m)
+    w64, x64 = s64.run_tda(4, 200, 1e-5)
+    slp = _system(m, precision=prec)
+    w, x = slp.run_tda(4, 200, 1e-5)
+    assert np.abs(w - w64).max() * HARTREE_TO_EV < 1e-3
+    assert slp.last_stats["converged"] == 1
+    assert slp.last_stats["max_residual"] < 1e-5
+    s64.close()
+    slp.close()
+
+
+def test_exchange_atom_blocking():
+    """A tiny HBM budget forces several atom blocks in the exchange."""
+    from tincr_amd import systems as sy
+    m = _synthetic(sy.water_cluster, 16)
+    eng = _oracle_engine(m)
+    x = np.random.default_rng(11).standard_normal((eng.get_size(), 4))
+    full = _system(m)
+    blocked = _system(m, hbm_budget_bytes=4 * 64 * 64 * 8 * 5)
+    a1 = full.compute_products(x)
+    a2 = blocked.compute_products(x)
+    assert np.abs(a1 - eng.apply(x)).max() < 1e-12
+    assert np.abs(a2 - a1).max() < 1e-13
+    full.close()
+    blocked.close()

@@ -24,7 +24,7 @@ PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "fp16": 3}
 EXPORTS = [
     "tda_ctx_create", "tda_ctx_destroy", "tda_last_error", "tda_set_precision",
     "tda_trans_charges", "tda_orbe_differences", "tda_set_system", "tda_prepare",
-    "tda_get_charges", "tda_compute_products", "tda_precondition", "tda_run",
+    "tda_clear_charges", "tda_get_charges", "tda_compute_products", "tda_precondition", "tda_run",
 ]
 
 
@@ -40,13 +40,19 @@ class TdaStats(ctypes.Structure):
         ("t_total_ms", ctypes.c_double),
         ("t_qbuild_ms", ctypes.c_double),
         ("t_products_ms", ctypes.c_double),
-        ("t_exchange_ms", ctypes.c_double),
-        ("n_exchange_launches", ctypes.c_int32),
-        ("exchange_flops", ctypes.c_double),
+        ("exch_ms", (ctypes.c_double * 2) * 2),
+        ("exch_launches", (ctypes.c_int32 * 2) * 2),
+        ("exch_flops", (ctypes.c_double * 2) * 2),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            if name.startswith("exch_"):
+                v = [[v[i][j] for j in range(2)] for i in range(2)]
+            out[name] = v
+        return out
 
 
 class TdaError(RuntimeError):
@@ -81,6 +87,7 @@ def load():
     lib.tda_orbe_differences.argtypes = [D, i, D, i, D]
     lib.tda_set_system.argtypes = [P, i, i, I32, D, D, D, i, i, i, i, D, D, D]
     lib.tda_prepare.argtypes = [P]
+    lib.tda_clear_charges.argtypes = [P]
     lib.tda_get_charges.argtypes = [P, D, D, D, D]
     lib.tda_compute_products.argtypes = [P, D, i, D]
     lib.tda_precondition.argtypes = [P, D, ctypes.c_double, D]
@@ -138,3 +145,42 @@ class Context:
     def set_precision(self, precision):
         self.check(self.lib.tda_set_precision(self.handle, PREC[precision]))
         self.precision = precision
+
+
+def load_debug():
+    """Bind the kernel-level hooks of include/tincr_tda_debug.h."""
+    lib = load()
+    D = ctypes.POINTER(ctypes.c_double)
+    i, i64 = ctypes.c_int, ctypes.c_int64
+    lib.tda_debug_gemm.argtypes = [i, i, i, i64, i64, i64, D, D, D, ctypes.c_double, ctypes.c_double, i]
+    lib.tda_debug_gemm.restype = ctypes.c_int
+    lib.tda_debug_eigh.argtypes = [i, i, D, D, D]
+    lib.tda_debug_eigh.restype = ctypes.c_int
+    return lib
+
+
+def debug_gemm(a, b, precision="fp64", a_kc=True, b_kc=False, alpha=1.0, beta=0.0, c=None, ksplit=0):
+    """alpha * A @ B (+ beta * C) through the MFMA GEMM.  a: [M, K]; b: [K, N]."""
+    lib = load_debug()
+    m, k = a.shape
+    n = b.shape[1]
+    a_st = f64(a if a_kc else a.T)
+    b_st = f64(b.T if b_kc else b)
+    out = np.zeros((m, n)) if c is None else f64(c).copy()
+    rc = lib.tda_debug_gemm(PREC[precision], int(a_kc), int(b_kc), m, n, k, dptr(a_st), dptr(b_st), dptr(out),
+                            alpha, beta, ksplit)
+    if rc != TDA_OK:
+        raise TdaError(rc, "tda_debug_gemm failed")
+    return out
+
+
+def debug_eigh(h, n):
+    lib = load_debug()
+    h = f64(h)
+    m = h.shape[0]
+    ev = np.empty(n)
+    y = np.empty((n, m))
+    rc = lib.tda_debug_eigh(m, n, dptr(h), dptr(ev), dptr(y))
+    if rc != TDA_OK:
+        raise TdaError(rc, "tda_debug_eigh failed")
+    return ev, y

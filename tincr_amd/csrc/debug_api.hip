@@ -1,0 +1,73 @@
+// Kernel-level test hooks (include/tincr_tda_debug.h).
+#include <cstring>
+#include <vector>
+
+#include "../../include/tincr_tda.h"
+#include "../../include/tincr_tda_debug.h"
+#include "gemm.h"
+#include "kernels.h"
+
+using namespace tda;
+
+namespace {
+struct Buf {
+  void* p = nullptr;
+  explicit Buf(size_t n) { TDA_HIP(hipMalloc(&p, n ? n : 16)); }
+  ~Buf() { (void)hipFree(p); }
+};
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    return TDA_OK;
+  } catch (const Error& e) {
+    fprintf(stderr, "tincr_tda debug: %s\n", e.msg.c_str());
+    return e.code;
+  }
+}
+}  // namespace
+
+extern "C" int tda_debug_gemm(int in_prec, int a_kc, int b_kc, int64_t M, int64_t N, int64_t K, const double* A,
+                              const double* B, double* C, double alpha, double beta, int ksplit) {
+  return guard([&] {
+    TDA_CHECK(in_prec >= 0 && in_prec <= 3, TDA_ERR_BAD_ARG, "bad precision");
+    const size_t esz = prec_bytes(in_prec);
+    Buf a64(M * K * 8), b64(K * N * 8), a(M * K * esz), b(K * N * esz), c(M * N * 8);
+    TDA_HIP(hipMemcpy(a64.p, A, M * K * 8, hipMemcpyHostToDevice));
+    TDA_HIP(hipMemcpy(b64.p, B, K * N * 8, hipMemcpyHostToDevice));
+    TDA_HIP(hipMemcpy(c.p, C, M * N * 8, hipMemcpyHostToDevice));
+    launch_convert((const double*)a64.p, M * K, 1, nullptr, a.p, in_prec, 0);
+    launch_convert((const double*)b64.p, K * N, 1, nullptr, b.p, in_prec, 0);
+    GemmArgs g;
+    std::memset(&g, 0, sizeof(g));
+    g.A = a.p; g.B = b.p; g.C = c.p;
+    g.M = M; g.N = N; g.K = K;
+    g.am = a_kc ? plain(K) : plain(1);
+    g.ak = a_kc ? plain(1) : plain(M);
+    g.bk = b_kc ? plain(1) : plain(N);
+    g.bn = b_kc ? plain(K) : plain(1);
+    g.cm = plain(N); g.cn = plain(1);
+    g.batch = 1; g.alpha = alpha; g.beta = beta;
+    g.dmap = plain(1); g.xm = plain(0); g.xn = plain(0);
+    const DType dt = in_prec == 0 ? DT_F64 : in_prec == 1 ? DT_F32 : in_prec == 2 ? DT_BF16 : DT_F16;
+    GemmPlan plan = gemm_plan(dt, M, N, K, 1, a_kc, b_kc, ksplit);
+    size_t wsb = gemm_workspace_bytes(dt, plan, M, N, 1);
+    Buf ws(wsb);
+    g.ws = ws.p;
+    gemm_launch(dt, DT_F64, EPI_STORE, a_kc, b_kc, g, plan, 0);
+    TDA_HIP(hipDeviceSynchronize());
+    TDA_HIP(hipMemcpy(C, c.p, M * N * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+extern "C" int tda_debug_eigh(int m, int n, const double* H, double* evals, double* Y) {
+  return guard([&] {
+    Buf h((size_t)m * m * 8), w(eigh_work_doubles(m) * 8), e((size_t)n * 8), y((size_t)n * m * 8);
+    TDA_HIP(hipMemcpy(h.p, H, (size_t)m * m * 8, hipMemcpyHostToDevice));
+    launch_eigh((const double*)h.p, m, m, n, (double*)w.p, (double*)e.p, (double*)y.p, m, 0);
+    TDA_HIP(hipDeviceSynchronize());
+    TDA_HIP(hipMemcpy(evals, e.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    TDA_HIP(hipMemcpy(Y, y.p, (size_t)n * m * 8, hipMemcpyDeviceToHost));
+  });
+}

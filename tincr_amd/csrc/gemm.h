@@ -54,7 +54,9 @@ struct GemmPlan {
 GemmPlan gemm_plan(DType in, int64_t M, int64_t N, int64_t K, int batch, bool a_kc, bool b_kc,
                    int ksplit_hint = 0);
 size_t gemm_workspace_bytes(DType in, const GemmPlan& p, int64_t M, int64_t N, int batch);
+// `after_main` (optional) is recorded right after the main GEMM kernel, before
+// any split-K reduction, so callers can time the MFMA kernel alone.
 void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs args,
-                 const GemmPlan& plan, hipStream_t stream);
+                 const GemmPlan& plan, hipStream_t stream, hipEvent_t after_main = nullptr);
 
 }  // namespace tda

@@ -1,394 +1,7 @@
-// MFMA GEMM kernels (gfx950).  See gemm.h for the contract.
-//
-// Tiling: a 256-thread workgroup (4 waves of 64) owns a BM x BN tile of C;
-// each wave owns a (BM/WGM) x (BN/WGN) sub-tile made of 16x16 MFMA blocks.
-// K is walked in BK slices staged through LDS; the next slice is fetched into
-// registers (16-byte vector loads along the operand's contiguous dimension)
-// while the current one feeds the matrix cores.
-//
-// MFMA operand layouts used (lane l of 64, 16x16 output block):
-//   f64 16x16x4, f32 16x16x4 : A[l%16][l/16], B[l/16][l%16]
-//   f16/bf16 16x16x32        : A[l%16][8*(l/16)+e], B[8*(l/16)+e][l%16], e<8
-//   result (all)             : D[4*(l/16)+r][l%16], r<4
-// LDS holds A as [k][m] (E=1 types) or [m][k] (E=8 types) so that fragment
-// reads are single ds_read instructions.
-#include "gemm.h"
+// MFMA GEMM: planning, dispatch and the split-K reduction.
+#include "gemm_impl.h"
 
 namespace tda {
-
-template <typename TI>
-struct Mfma;
-template <>
-struct Mfma<double> {
-  using TA = double;
-  using Acc = v4d;
-  using Frag = double;
-  static constexpr int KSTEP = 4, E = 1;
-  static __device__ inline Acc op(Frag a, Frag b, Acc c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-};
-template <>
-struct Mfma<float> {
-  using TA = float;
-  using Acc = v4f;
-  using Frag = float;
-  static constexpr int KSTEP = 4, E = 1;
-  static __device__ inline Acc op(Frag a, Frag b, Acc c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-};
-template <>
-struct Mfma<_Float16> {
-  using TA = float;
-  using Acc = v4f;
-  using Frag = v8h;
-  static constexpr int KSTEP = 32, E = 8;
-  static __device__ inline Acc op(Frag a, Frag b, Acc c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-  }
-};
-template <>
-struct Mfma<__bf16> {
-  using TA = float;
-  using Acc = v4f;
-  using Frag = v8bf;
-  static constexpr int KSTEP = 32, E = 8;
-  static __device__ inline Acc op(Frag a, Frag b, Acc c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-  }
-};
-
-template <typename T, int N>
-struct alignas(16) VecN {
-  T v[N];
-};
-
-// Tile configurations.
-struct CfgBig {  // 128x128, waves 2x2 (64x64 each)
-  static constexpr int BM = 128, BN = 128, WGM = 2, WGN = 2;
-};
-struct CfgSkinnyN {  // 64x32, waves 2x2 (32x16 each): N <= 32-ish
-  static constexpr int BM = 64, BN = 32, WGM = 2, WGN = 2;
-};
-struct CfgSkinnyM {  // 32x128, waves 1x4 (32x32 each): M <= 32-ish
-  static constexpr int BM = 32, BN = 128, WGM = 1, WGN = 4;
-};
-struct CfgSmall {  // 64x64, waves 2x2 (32x32 each)
-  static constexpr int BM = 64, BN = 64, WGM = 2, WGN = 2;
-};
-
-template <typename TI>
-struct BKOf {
-  static constexpr int value = Mfma<TI>::E == 1 ? 16 : 64;
-};
-
-template <typename TO>
-__device__ inline void store_out(TO* p, double v) {
-  *p = to_type<TO>(v);
-}
-
-template <typename TI, typename TO, typename Cfg, bool A_KC, bool B_KC, int EPI>
-__global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
-  using MF = Mfma<TI>;
-  using TA = typename MF::TA;
-  constexpr int BM = Cfg::BM, BN = Cfg::BN, WGM = Cfg::WGM, WGN = Cfg::WGN;
-  constexpr int BK = BKOf<TI>::value;
-  constexpr int E = MF::E, KSTEP = MF::KSTEP;
-  constexpr int WTM = BM / WGM, WTN = BN / WGN;
-  constexpr int MI = WTM / 16, NI = WTN / 16;
-  constexpr int VEC = 16 / sizeof(TI);
-  constexpr int PAD = (E == 1) ? (sizeof(TI) == 8 ? 2 : 4) : 8;
-  // LDS geometry: E==1 -> [k][m] ; E==8 -> [m][k]
-  constexpr int A_OUT = (E == 1) ? BK : BM, A_IN = (E == 1) ? BM + PAD : BK + PAD;
-  constexpr int B_OUT = (E == 1) ? BK : BN, B_IN = (E == 1) ? BN + PAD : BK + PAD;
-  __shared__ TI As[A_OUT * A_IN];
-  __shared__ TI Bs[B_OUT * B_IN];
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wm0 = (wave / WGN) * WTM;
-  const int wn0 = (wave % WGN) * WTN;
-
-  const int zb = blockIdx.z / g.ksplit;
-  const int zs = blockIdx.z % g.ksplit;
-  const int64_t m0 = (int64_t)blockIdx.y * BM;
-  const int64_t n0 = (int64_t)blockIdx.x * BN;
-  const TI* A = (const TI*)g.A + zb * g.a_batch;
-  const TI* B = (const TI*)g.B + zb * g.b_batch;
-
-  const int64_t kbeg = (int64_t)zs * g.k_chunk;
-  const int64_t kend = min(g.K, kbeg + g.k_chunk);
-  const int nkt = kend > kbeg ? (int)((kend - kbeg + BK - 1) / BK) : 0;
-
-  // ---- loader geometry (compile-time counts) ----
-  constexpr int A_VECS = BM * BK / VEC;  // vectors per A tile
-  constexpr int B_VECS = BN * BK / VEC;
-  constexpr int A_PER = (A_VECS + 255) / 256;
-  constexpr int B_PER = (B_VECS + 255) / 256;
-  VecN<TI, VEC> ra[A_PER];
-  VecN<TI, VEC> rb[B_PER];
-
-  // Precompute the (tile-invariant) row/col parts of addresses.
-  // For A_KC: vector idx -> (m = idx / (BK/VEC), kv = idx % (BK/VEC))
-  // else      : vector idx -> (k = idx / (BM/VEC), mv = idx % (BM/VEC))
-  int64_t a_row_off[A_PER];
-  bool a_row_ok[A_PER];
-#pragma unroll
-  for (int i = 0; i < A_PER; ++i) {
-    int idx = tid + i * 256;
-    int64_t m;
-    if (A_KC) m = m0 + idx / (BK / VEC);
-    else m = m0 + (idx % (BM / VEC)) * VEC;
-    a_row_ok[i] = (idx < A_VECS) && (m < g.M);
-    a_row_off[i] = a_row_ok[i] ? g.am(m) : 0;
-  }
-  int64_t b_col_off[B_PER];
-  bool b_col_ok[B_PER];
-#pragma unroll
-  for (int i = 0; i < B_PER; ++i) {
-    int idx = tid + i * 256;
-    int64_t n;
-    if (B_KC) n = n0 + idx / (BK / VEC);
-    else n = n0 + (idx % (BN / VEC)) * VEC;
-    b_col_ok[i] = (idx < B_VECS) && (n < g.N);
-    b_col_off[i] = b_col_ok[i] ? g.bn(n) : 0;
-  }
-
-  auto load_tile = [&](int kt) {
-    const int64_t k0 = kbeg + (int64_t)kt * BK;
-    // K-map evaluated once per tile (tiles never straddle a K-map boundary).
-    const int64_t ak0 = g.ak(k0), bk0 = g.bk(k0);
-#pragma unroll
-    for (int i = 0; i < A_PER; ++i) {
-      int idx = tid + i * 256;
-      VecN<TI, VEC> v;
-#pragma unroll
-      for (int e = 0; e < VEC; ++e) v.v[e] = (TI)0;
-      if (A_KC) {
-        int kk = (idx % (BK / VEC)) * VEC;
-        int64_t k = k0 + kk;
-        if (a_row_ok[i]) {
-          const TI* p = A + a_row_off[i] + ak0 + kk * g.ak.s0;
-          if (k + VEC <= kend && (((uintptr_t)p & 15) == 0)) v = *(const VecN<TI, VEC>*)p;
-          else
-            for (int e = 0; e < VEC; ++e)
-              if (k + e < kend) v.v[e] = p[e];
-        }
-      } else {
-        int kk = idx / (BM / VEC);
-        int64_t k = k0 + kk;
-        int64_t m = m0 + (idx % (BM / VEC)) * VEC;
-        if (idx < A_VECS && k < kend && m < g.M) {
-          const TI* p = A + a_row_off[i] + ak0 + g.ak.s0 * kk;
-          if (m + VEC <= g.M && (((uintptr_t)p & 15) == 0)) v = *(const VecN<TI, VEC>*)p;
-          else
-            for (int e = 0; e < VEC; ++e)
-              if (m + e < g.M) v.v[e] = p[e];
-        }
-      }
-      ra[i] = v;
-    }
-#pragma unroll
-    for (int i = 0; i < B_PER; ++i) {
-      int idx = tid + i * 256;
-      VecN<TI, VEC> v;
-#pragma unroll
-      for (int e = 0; e < VEC; ++e) v.v[e] = (TI)0;
-      if (B_KC) {
-        int kk = (idx % (BK / VEC)) * VEC;
-        int64_t k = k0 + kk;
-        if (b_col_ok[i]) {
-          const TI* p = B + b_col_off[i] + bk0 + kk * g.bk.s0;
-          if (k + VEC <= kend && (((uintptr_t)p & 15) == 0)) v = *(const VecN<TI, VEC>*)p;
-          else
-            for (int e = 0; e < VEC; ++e)
-              if (k + e < kend) v.v[e] = p[e];
-        }
-      } else {
-        int kk = idx / (BN / VEC);
-        int64_t k = k0 + kk;
-        int64_t n = n0 + (idx % (BN / VEC)) * VEC;
-        if (idx < B_VECS && k < kend && n < g.N) {
-          const TI* p = B + b_col_off[i] + bk0 + g.bk.s0 * kk;
-          if (n + VEC <= g.N && (((uintptr_t)p & 15) == 0)) v = *(const VecN<TI, VEC>*)p;
-          else
-            for (int e = 0; e < VEC; ++e)
-              if (n + e < g.N) v.v[e] = p[e];
-        }
-      }
-      rb[i] = v;
-    }
-  };
-
-  auto store_tile = [&]() {
-#pragma unroll
-    for (int i = 0; i < A_PER; ++i) {
-      int idx = tid + i * 256;
-      if (idx >= A_VECS) continue;
-      int m, k;
-      if (A_KC) { m = idx / (BK / VEC); k = (idx % (BK / VEC)) * VEC; }
-      else { k = idx / (BM / VEC); m = (idx % (BM / VEC)) * VEC; }
-      if (E == 1) {  // LDS [k][m]
-        if (A_KC) {
-#pragma unroll
-          for (int e = 0; e < VEC; ++e) As[(k + e) * A_IN + m] = ra[i].v[e];
-        } else {
-          *(VecN<TI, VEC>*)&As[k * A_IN + m] = ra[i];
-        }
-      } else {  // LDS [m][k]
-        if (A_KC) {
-          *(VecN<TI, VEC>*)&As[m * A_IN + k] = ra[i];
-        } else {
-#pragma unroll
-          for (int e = 0; e < VEC; ++e) As[(m + e) * A_IN + k] = ra[i].v[e];
-        }
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < B_PER; ++i) {
-      int idx = tid + i * 256;
-      if (idx >= B_VECS) continue;
-      int n, k;
-      if (B_KC) { n = idx / (BK / VEC); k = (idx % (BK / VEC)) * VEC; }
-      else { k = idx / (BN / VEC); n = (idx % (BN / VEC)) * VEC; }
-      if (E == 1) {  // LDS [k][n]
-        if (B_KC) {
-#pragma unroll
-          for (int e = 0; e < VEC; ++e) Bs[(k + e) * B_IN + n] = rb[i].v[e];
-        } else {
-          *(VecN<TI, VEC>*)&Bs[k * B_IN + n] = rb[i];
-        }
-      } else {  // LDS [n][k]
-        if (B_KC) {
-          *(VecN<TI, VEC>*)&Bs[n * B_IN + k] = rb[i];
-        } else {
-#pragma unroll
-          for (int e = 0; e < VEC; ++e) Bs[(n + e) * B_IN + k] = rb[i].v[e];
-        }
-      }
-    }
-  };
-
-  typename MF::Acc acc[MI][NI];
-#pragma unroll
-  for (int i = 0; i < MI; ++i)
-#pragma unroll
-    for (int j = 0; j < NI; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[i][j][r] = 0;
-
-  if (nkt > 0) load_tile(0);
-  for (int kt = 0; kt < nkt; ++kt) {
-    __syncthreads();
-    store_tile();
-    __syncthreads();
-    if (kt + 1 < nkt) load_tile(kt + 1);
-#pragma unroll
-    for (int ks = 0; ks < BK / KSTEP; ++ks) {
-      typename MF::Frag fa[MI], fb[NI];
-#pragma unroll
-      for (int i = 0; i < MI; ++i) {
-        const int m = wm0 + i * 16 + (lane & 15);
-        if constexpr (E == 1) {
-          fa[i] = As[(ks * KSTEP + (lane >> 4)) * A_IN + m];
-        } else {
-          fa[i] = *(const typename MF::Frag*)&As[m * A_IN + ks * KSTEP + 8 * (lane >> 4)];
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < NI; ++j) {
-        const int n = wn0 + j * 16 + (lane & 15);
-        if constexpr (E == 1) {
-          fb[j] = Bs[(ks * KSTEP + (lane >> 4)) * B_IN + n];
-        } else {
-          fb[j] = *(const typename MF::Frag*)&Bs[n * B_IN + ks * KSTEP + 8 * (lane >> 4)];
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < MI; ++i)
-#pragma unroll
-        for (int j = 0; j < NI; ++j) acc[i][j] = MF::op(fa[i], fb[j], acc[i][j]);
-    }
-  }
-
-  // ---- epilogue ----
-  if (g.ksplit > 1) {
-    TA* ws = (TA*)g.ws + (int64_t)blockIdx.z * g.M * g.N;
-#pragma unroll
-    for (int i = 0; i < MI; ++i)
-#pragma unroll
-      for (int j = 0; j < NI; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int64_t m = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
-          const int64_t n = n0 + wn0 + j * 16 + (lane & 15);
-          if (m < g.M && n < g.N) ws[m * g.N + n] = acc[i][j][r];
-        }
-    return;
-  }
-  TO* C = (TO*)g.C + zb * g.c_batch;
-  double alpha = g.alpha;
-  if (g.batch_scale) alpha *= (double)g.batch_scale[zb];
-#pragma unroll
-  for (int i = 0; i < MI; ++i)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int64_t m = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
-      if (m >= g.M) continue;
-      const int64_t cmo = g.cm(m);
-#pragma unroll
-      for (int j = 0; j < NI; ++j) {
-        const int64_t n = n0 + wn0 + j * 16 + (lane & 15);
-        if (n >= g.N) continue;
-        TO* p = C + cmo + g.cn(n);
-        double v = alpha * (double)acc[i][j][r];
-        if constexpr (EPI == EPI_DIAG) {
-          v += g.diag[g.dmap(n)] * g.xsrc[g.xm(m) + g.xn(n)];
-        } else {
-          if (g.beta != 0.0) v += g.beta * to_double<TO>(*p);
-        }
-        store_out<TO>(p, v);
-      }
-    }
-}
-
-// Sum split-K partials and apply the epilogue.
-template <typename TA, typename TO, int EPI>
-__global__ void __launch_bounds__(256) splitk_reduce_kernel(GemmArgs g) {
-  const int64_t MN = g.M * g.N;
-  const int64_t total = MN * g.batch;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int zb = (int)(t / MN);
-    const int64_t mn = t % MN;
-    const int64_t m = mn / g.N, n = mn % g.N;
-    const TA* ws = (const TA*)g.ws + (int64_t)zb * g.ksplit * MN + mn;
-    double s = 0.0;
-    for (int z = 0; z < g.ksplit; ++z) s += (double)ws[z * MN];
-    double alpha = g.alpha;
-    if (g.batch_scale) alpha *= (double)g.batch_scale[zb];
-    TO* p = (TO*)g.C + zb * g.c_batch + g.cm(m) + g.cn(n);
-    double v = alpha * s;
-    if constexpr (EPI == EPI_DIAG) {
-      v += g.diag[g.dmap(n)] * g.xsrc[g.xm(m) + g.xn(n)];
-    } else {
-      if (g.beta != 0.0) v += g.beta * to_double<TO>(*p);
-    }
-    *p = to_type<TO>(v);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// host side
-// ---------------------------------------------------------------------------
-enum CfgId : int { CFG_BIG = 0, CFG_SKINNY_N = 1, CFG_SKINNY_M = 2, CFG_SMALL = 3 };
-
-static int cfg_bm(int c) { return c == CFG_BIG ? 128 : c == CFG_SKINNY_N ? 64 : c == CFG_SKINNY_M ? 32 : 64; }
-static int cfg_bn(int c) { return c == CFG_BIG ? 128 : c == CFG_SKINNY_N ? 32 : c == CFG_SKINNY_M ? 128 : 64; }
-static int bk_of(DType in) { return (in == DT_F64 || in == DT_F32) ? 16 : 64; }
-static size_t acc_bytes(DType in) { return in == DT_F64 ? 8 : 4; }
 
 GemmPlan gemm_plan(DType in, int64_t M, int64_t N, int64_t K, int batch, bool, bool, int ksplit_hint) {
   GemmPlan p;
@@ -418,47 +31,6 @@ size_t gemm_workspace_bytes(DType in, const GemmPlan& p, int64_t M, int64_t N, i
   return (size_t)p.ksplit * batch * M * N * acc_bytes(in);
 }
 
-template <typename TI, typename TO, typename Cfg, int EPI>
-static void launch_cfg(bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s) {
-  dim3 grid((unsigned)ceil_div(g.N, Cfg::BN), (unsigned)ceil_div(g.M, Cfg::BM),
-            (unsigned)(g.batch * g.ksplit));
-  if (a_kc && b_kc) gemm_kernel<TI, TO, Cfg, true, true, EPI><<<grid, 256, 0, s>>>(g);
-  else if (a_kc && !b_kc) gemm_kernel<TI, TO, Cfg, true, false, EPI><<<grid, 256, 0, s>>>(g);
-  else if (!a_kc && b_kc) gemm_kernel<TI, TO, Cfg, false, true, EPI><<<grid, 256, 0, s>>>(g);
-  else gemm_kernel<TI, TO, Cfg, false, false, EPI><<<grid, 256, 0, s>>>(g);
-  TDA_LAUNCH_CHECK();
-}
-
-template <typename TI, typename TO, int EPI>
-static void launch_t(int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s) {
-  switch (cfg) {
-    case CFG_BIG: launch_cfg<TI, TO, CfgBig, EPI>(a_kc, b_kc, g, s); break;
-    case CFG_SKINNY_N: launch_cfg<TI, TO, CfgSkinnyN, EPI>(a_kc, b_kc, g, s); break;
-    case CFG_SKINNY_M: launch_cfg<TI, TO, CfgSkinnyM, EPI>(a_kc, b_kc, g, s); break;
-    default: launch_cfg<TI, TO, CfgSmall, EPI>(a_kc, b_kc, g, s); break;
-  }
-}
-
-template <typename TA, typename TO, int EPI>
-static void launch_reduce(GemmArgs& g, hipStream_t s) {
-  const int64_t total = g.M * g.N * g.batch;
-  const int blocks = (int)std::min<int64_t>(ceil_div(total, 256), 8192);
-  splitk_reduce_kernel<TA, TO, EPI><<<blocks, 256, 0, s>>>(g);
-  TDA_LAUNCH_CHECK();
-}
-
-// The instantiated (in, out) pairs of the TDA path.
-template <int EPI>
-static void dispatch(DType in, DType out, int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s) {
-  if (in == DT_F64 && out == DT_F64) launch_t<double, double, EPI>(cfg, a_kc, b_kc, g, s);
-  else if (in == DT_F32 && out == DT_F64) launch_t<float, double, EPI>(cfg, a_kc, b_kc, g, s);
-  else if (in == DT_F32 && out == DT_F32) launch_t<float, float, EPI>(cfg, a_kc, b_kc, g, s);
-  else if (EPI == EPI_STORE && in == DT_F16 && out == DT_F16) launch_t<_Float16, _Float16, EPI_STORE>(cfg, a_kc, b_kc, g, s);
-  else if (EPI == EPI_STORE && in == DT_F16 && out == DT_F64) launch_t<_Float16, double, EPI_STORE>(cfg, a_kc, b_kc, g, s);
-  else if (EPI == EPI_STORE && in == DT_BF16 && out == DT_BF16) launch_t<__bf16, __bf16, EPI_STORE>(cfg, a_kc, b_kc, g, s);
-  else if (EPI == EPI_STORE && in == DT_BF16 && out == DT_F64) launch_t<__bf16, double, EPI_STORE>(cfg, a_kc, b_kc, g, s);
-  else throw Error{2, "gemm: unsupported type combination"};
-}
 
 template <int EPI>
 static void dispatch_reduce(DType in, DType out, GemmArgs& g, hipStream_t s) {
@@ -472,7 +44,7 @@ static void dispatch_reduce(DType in, DType out, GemmArgs& g, hipStream_t s) {
 }
 
 void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g, const GemmPlan& plan,
-                 hipStream_t stream) {
+                 hipStream_t stream, hipEvent_t after_main) {
   TDA_CHECK(g.M >= 0 && g.N >= 0 && g.K >= 0, 2, "gemm: negative extent");
   if (g.M == 0 || g.N == 0) return;
   if (g.batch < 1) g.batch = 1;
@@ -489,8 +61,13 @@ void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g,
   if (b_kc) TDA_CHECK(g.bk.s0 == 1, 2, "gemm: B K-contiguous needs bk.s0 == 1");
   else TDA_CHECK(g.bn.s0 == 1 && g.bn.div >= g.N, 2, "gemm: B N-contiguous needs plain unit bn");
   if (plan.ksplit > 1) TDA_CHECK(g.ws != nullptr, 2, "gemm: split-K needs a workspace");
-  if (epi == EPI_DIAG) dispatch<EPI_DIAG>(in, out, plan.cfg, a_kc, b_kc, g, stream);
-  else dispatch<EPI_STORE>(in, out, plan.cfg, a_kc, b_kc, g, stream);
+  switch (in) {
+    case DT_F64: gemm_dispatch_f64(epi, out, plan.cfg, a_kc, b_kc, g, stream); break;
+    case DT_F32: gemm_dispatch_f32(epi, out, plan.cfg, a_kc, b_kc, g, stream); break;
+    case DT_F16: gemm_dispatch_f16(epi, out, plan.cfg, a_kc, b_kc, g, stream); break;
+    default: gemm_dispatch_bf16(epi, out, plan.cfg, a_kc, b_kc, g, stream); break;
+  }
+  if (after_main) TDA_HIP(hipEventRecord(after_main, stream));
   if (plan.ksplit > 1) {
     if (epi == EPI_DIAG) dispatch_reduce<EPI_DIAG>(in, out, g, stream);
     else dispatch_reduce<EPI_STORE>(in, out, g, stream);

@@ -97,10 +97,18 @@ struct tda_ctx {
   DevBuf V, AV, H, Y, evals, RZ, R, P, W, norms, npart, idx, count, gs_scratch, eigw;
   // instrumentation
   bool timing = true;
-  double t_products_ms = 0, t_exchange_ms = 0, exchange_flops = 0;
-  int n_exchange_launches = 0, n_products = 0, n_products_fp64 = 0, n_blocks = 1;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_exch;
-  std::vector<double> ev_exch_flops;
+  double t_products_ms = 0;
+  int n_products = 0, n_products_fp64 = 0, n_blocks = 1;
+  // per exchange stage s (0: stage 1, 1: stage 3), per precision class p
+  // (0: FP64, 1: low precision): main-kernel time, launches, flops
+  double st_ms[2][2] = {{0, 0}, {0, 0}}, st_flops[2][2] = {{0, 0}, {0, 0}};
+  int st_n[2][2] = {{0, 0}, {0, 0}};
+  struct Rec {
+    hipEvent_t a, b;
+    int stage, pclass;
+    double flops;
+  };
+  std::vector<Rec> recs;
   std::vector<hipEvent_t> ev_pool;
 
   ~tda_ctx() {
@@ -125,17 +133,22 @@ struct tda_ctx {
   }
   // Fold recorded exchange events into the counters (after a sync).
   void harvest_events() {
-    for (size_t i = 0; i < ev_exch.size(); ++i) {
+    for (const Rec& r : recs) {
       float ms = 0;
-      TDA_HIP(hipEventElapsedTime(&ms, ev_exch[i].first, ev_exch[i].second));
-      t_exchange_ms += ms;
-      exchange_flops += ev_exch_flops[i];
-      ++n_exchange_launches;
-      ev_pool.push_back(ev_exch[i].first);
-      ev_pool.push_back(ev_exch[i].second);
+      TDA_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      st_ms[r.stage][r.pclass] += ms;
+      st_flops[r.stage][r.pclass] += r.flops;
+      st_n[r.stage][r.pclass] += 1;
+      ev_pool.push_back(r.a);
+      ev_pool.push_back(r.b);
     }
-    ev_exch.clear();
-    ev_exch_flops.clear();
+    recs.clear();
+  }
+  void reset_counters() {
+    t_products_ms = 0;
+    n_products = n_products_fp64 = 0;
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) st_ms[i][j] = st_flops[i][j] = 0, st_n[i][j] = 0;
   }
 };
 
@@ -161,14 +174,15 @@ GemmArgs gargs(const void* A, const void* B, void* C, int64_t M, int64_t N, int6
   return g;
 }
 
-void run_gemm(tda_ctx* c, DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g, int ksplit_hint = 0) {
+void run_gemm(tda_ctx* c, DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g, int ksplit_hint = 0,
+              hipEvent_t after_main = nullptr) {
   GemmPlan plan = gemm_plan(in, g.M, g.N, g.K, g.batch < 1 ? 1 : g.batch, a_kc, b_kc, ksplit_hint);
   size_t wsb = gemm_workspace_bytes(in, plan, g.M, g.N, g.batch < 1 ? 1 : g.batch);
   if (wsb) {
     c->ws_gemm.ensure(wsb);
     g.ws = c->ws_gemm.p;
   }
-  gemm_launch(in, out, epi, a_kc, b_kc, g, plan, c->stream);
+  gemm_launch(in, out, epi, a_kc, b_kc, g, plan, c->stream, after_main);
 }
 
 // ---------------------------------------------------------------------------
@@ -329,9 +343,13 @@ void apply_A(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   for (int B0 = 0; B0 < N; B0 += NB) {
     const int nb = std::min(NB, N - B0);
     const int64_t NBop = (int64_t)nb * op;
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    const int pclass = lp ? 1 : 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
     if (c->timing) {
       e0 = c->get_event();
+      e1 = c->get_event();
+      e2 = c->get_event();
+      e3 = c->get_event();
       TDA_HIP(hipEventRecord(e0, c->stream));
     }
     // stage 1: T[c][a][(B,j)] = sum_b q_vv[B][a][b] X[c][j][b]
@@ -343,12 +361,9 @@ void apply_A(tda_ctx* c, const double* X, int k, double* AX, int prec) {
       g.b_batch = 0;
       g.c_batch = op;
       if (s1scale) g.batch_scale = s1scale + B0;
-      run_gemm(c, dt, dt, EPI_STORE, true, true, g);
+      run_gemm(c, dt, dt, EPI_STORE, true, true, g, 0, e1);
     }
-    if (c->timing) {
-      e1 = c->get_event();
-      TDA_HIP(hipEventRecord(e1, c->stream));
-    }
+    if (c->timing) TDA_HIP(hipEventRecord(e2, c->stream));
     // stage 3: AX[c][i][a] -= sum_(B,j) G[B][i][j] T[c][a][(B,j)]
     {
       GemmArgs g = gargs((const char*)G + (size_t)B0 * op * op * esz, c->T.p, AX, o, (int64_t)k * v, NBop,
@@ -356,15 +371,12 @@ void apply_A(tda_ctx* c, const double* X, int k, double* AX, int prec) {
                          plain(vp), split(v, ovp, 1));
       g.alpha = -1.0;
       g.beta = 1.0;
-      run_gemm(c, dt, DT_F64, EPI_STORE, true, true, g);
+      run_gemm(c, dt, DT_F64, EPI_STORE, true, true, g, 0, e3);
     }
     if (c->timing) {
-      e2 = c->get_event();
-      TDA_HIP(hipEventRecord(e2, c->stream));
-      c->ev_exch.push_back({e0, e1});
-      c->ev_exch_flops.push_back(2.0 * v * (double)k * o * v * nb);
-      c->ev_exch.push_back({e1, e2});
-      c->ev_exch_flops.push_back(2.0 * o * (double)k * v * o * nb);
+      // algorithmic flops: stage 1 2*k*o*v*v*nb, stage 3 2*k*o*o*v*nb
+      c->recs.push_back({e0, e1, 0, pclass, 2.0 * k * (double)o * v * v * nb});
+      c->recs.push_back({e2, e3, 1, pclass, 2.0 * k * (double)o * o * v * nb});
     }
   }
 }
@@ -719,6 +731,13 @@ int tda_prepare(tda_ctx* c) {
   return guarded(c, [&] { prepare(c); });
 }
 
+int tda_clear_charges(tda_ctx* c) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  c->prepared = false;
+  c->lp_prec = -1;
+  return TDA_OK;
+}
+
 int tda_get_charges(tda_ctx* c, double* q_ov, double* q_oo, double* q_vv, double* omega) {
   if (!c) return TDA_ERR_BAD_ARG;
   return guarded(c, [&] {
@@ -789,8 +808,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     TDA_CHECK(n_roots >= 1 && n_roots <= 64 && n_roots <= ov, TDA_ERR_BAD_ARG,
               "n_roots must be in 1..min(64, n_occ*n_virt)");
     TDA_CHECK(max_iter >= 1 && tol > 0.0, TDA_ERR_BAD_ARG, "bad max_iter / tolerance");
-    c->t_products_ms = c->t_exchange_ms = c->exchange_flops = 0;
-    c->n_exchange_launches = c->n_products = c->n_products_fp64 = 0;
+    c->reset_counters();
     const auto tq0 = std::chrono::steady_clock::now();
     prepare(c);
     const double tq = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tq0).count();
@@ -863,9 +881,12 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       stats->max_residual = res64.max_err;
       stats->t_qbuild_ms = tq;
       stats->t_products_ms = c->t_products_ms;
-      stats->t_exchange_ms = c->t_exchange_ms;
-      stats->n_exchange_launches = c->n_exchange_launches;
-      stats->exchange_flops = c->exchange_flops;
+      for (int st = 0; st < 2; ++st)
+        for (int p = 0; p < 2; ++p) {
+          stats->exch_ms[st][p] = c->st_ms[st][p];
+          stats->exch_launches[st][p] = c->st_n[st][p];
+          stats->exch_flops[st][p] = c->st_flops[st][p];
+        }
     }
     if (!res64.converged) throw Error{TDA_ERR_NOT_CONVERGED, "Davidson Algorithm did not converge!"};
   });

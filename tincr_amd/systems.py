@@ -1,0 +1,221 @@
+"""Seeded molecular systems and TDA inputs for tests and benchmarks.
+
+* Geometries: water clusters on a jittered cubic lattice and planar all-trans
+  polyenes C_nH_{n+2} (the workloads of the excited-state benchmarks).
+* Minimal valence basis bookkeeping: H 1 AO / 1 e, C 4 AO / 4 e, N 4 / 5,
+  O 4 / 6 (tincr's confined pseudo-atom data), closed-shell occupation
+  occ = n_elec / 2 (src/initialization/system.rs:131-140).
+* Gaussian gamma functions gamma / gamma_lr (src/scc/gamma_approximation.rs:
+  29-35, 52-124, 175-193) with tincr's Hubbard parameters and
+  r_lr = 3.03 bohr (src/defaults.rs:39).
+* Synthetic orbitals for throughput runs: Haar-random orthogonal C with S = I
+  and sorted orbital energies with a HOMO-LUMO gap.
+"""
+import math
+
+import numpy as np
+
+BOHR_TO_ANGS = 0.529177210903          # src/constants.rs:4
+LONG_RANGE_RADIUS = 3.03               # src/defaults.rs:39 (bohr)
+# Hubbard U (Ha) of the confined pseudo atoms (src/param/slaterkoster/confined_pseudo_atom/*.ron:8)
+HUBBARD_U = {1: 0.39292032, 6: 0.34937503, 7: 0.42231745, 8: 0.49118359}
+VALENCE_ORBS = {1: 1, 6: 4, 7: 4, 8: 4}
+VALENCE_ELEC = {1: 1, 6: 4, 7: 5, 8: 6}
+SYMBOL = {1: "H", 6: "C", 7: "N", 8: "O"}
+
+
+class Molecule:
+    """Atomic numbers + coordinates (bohr)."""
+
+    def __init__(self, z, xyz_bohr, name=""):
+        self.z = np.asarray(z, dtype=np.int64)
+        self.xyz = np.asarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
+        self.name = name
+
+    @property
+    def n_atoms(self):
+        return len(self.z)
+
+    @property
+    def orbs_per_atom(self):
+        return np.array([VALENCE_ORBS[int(a)] for a in self.z], dtype=np.int32)
+
+    @property
+    def n_ao(self):
+        return int(self.orbs_per_atom.sum())
+
+    @property
+    def n_elec(self):
+        return int(sum(VALENCE_ELEC[int(a)] for a in self.z))
+
+    @property
+    def n_occ(self):
+        return self.n_elec // 2
+
+    @property
+    def n_virt(self):
+        return self.n_ao - self.n_occ
+
+    def to_xyz(self):
+        lines = [str(self.n_atoms), self.name]
+        for a, r in zip(self.z, self.xyz * BOHR_TO_ANGS):
+            lines.append(f"{SYMBOL[int(a)]} {r[0]:.8f} {r[1]:.8f} {r[2]:.8f}")
+        return "\n".join(lines) + "\n"
+
+
+def _random_rotation(rng):
+    """Uniform random rotation (Shoemake 1992, 'Uniform random rotations')."""
+    u1, u2, u3 = rng.random(3)
+    q = np.array([math.sqrt(1 - u1) * math.sin(2 * math.pi * u2),
+                  math.sqrt(1 - u1) * math.cos(2 * math.pi * u2),
+                  math.sqrt(u1) * math.sin(2 * math.pi * u3),
+                  math.sqrt(u1) * math.cos(2 * math.pi * u3)])
+    x, y, z, w = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)],
+    ])
+
+
+def water_cluster(n, seed=0, spacing=3.1, jitter=0.3):
+    """(H2O)_n: O-H 0.9572 A, HOH 104.52 deg, centres on a cubic lattice of
+    side ceil(n^(1/3)) filled lexicographically (first n sites), +-jitter A
+    uniform per centre coordinate, uniform random orientation."""
+    rng = np.random.default_rng(seed)
+    side = 1
+    while side ** 3 < n:
+        side += 1
+    r_oh, ang = 0.9572, math.radians(104.52)
+    local = np.array([[0.0, 0.0, 0.0],
+                      [r_oh * math.sin(ang / 2), r_oh * math.cos(ang / 2), 0.0],
+                      [-r_oh * math.sin(ang / 2), r_oh * math.cos(ang / 2), 0.0]])
+    z, xyz = [], []
+    count = 0
+    for ix in range(side):
+        for iy in range(side):
+            for iz in range(side):
+                if count == n:
+                    break
+                centre = spacing * np.array([ix, iy, iz], dtype=float)
+                centre = centre + rng.uniform(-jitter, jitter, 3)
+                rot = _random_rotation(rng)
+                for a, p in zip((8, 1, 1), local @ rot.T + centre):
+                    z.append(a)
+                    xyz.append(p)
+                count += 1
+    return Molecule(z, np.array(xyz) / BOHR_TO_ANGS, f"(H2O)_{n} seed {seed}")
+
+
+def polyene(n_c, seed=0, jitter=0.02):
+    """Planar all-trans C_nH_{n+2}: alternating C-C 1.34/1.46 A starting with a
+    double bond, C-H 1.09 A, 120 deg angles, CH2 termini, +-jitter A uniform
+    Cartesian noise."""
+    rng = np.random.default_rng(seed)
+    carbons = [np.zeros(2)]
+    # zig-zag chain along x: bond directions alternate +-30 deg
+    for k in range(1, n_c):
+        length = 1.34 if (k - 1) % 2 == 0 else 1.46
+        ang = math.radians(30.0 if k % 2 == 1 else -30.0)
+        carbons.append(carbons[-1] + length * np.array([math.cos(ang), math.sin(ang)]))
+    carbons = np.array(carbons)
+    hyd = []
+    r_ch = 1.09
+    for k in range(n_c):
+        nbrs = [carbons[j] for j in (k - 1, k + 1) if 0 <= j < n_c]
+        if len(nbrs) == 2:
+            d = (carbons[k] - nbrs[0]) / np.linalg.norm(carbons[k] - nbrs[0])
+            d = d + (carbons[k] - nbrs[1]) / np.linalg.norm(carbons[k] - nbrs[1])
+            hyd.append(carbons[k] + r_ch * d / np.linalg.norm(d))
+        else:
+            # terminal CH2: two hydrogens at +-120 deg from the C-C bond
+            b = (carbons[k] - nbrs[0]) / np.linalg.norm(carbons[k] - nbrs[0])
+            for sgn in (1.0, -1.0):
+                a = math.radians(60.0) * sgn
+                rot = np.array([[math.cos(a), -math.sin(a)], [math.sin(a), math.cos(a)]])
+                hyd.append(carbons[k] + r_ch * rot @ b)
+    pos = np.vstack([carbons, np.array(hyd)])
+    pos3 = np.zeros((len(pos), 3))
+    pos3[:, :2] = pos
+    pos3 += rng.uniform(-jitter, jitter, pos3.shape)
+    z = [6] * n_c + [1] * len(hyd)
+    return Molecule(z, pos3 / BOHR_TO_ANGS, f"C{n_c}H{n_c + 2} seed {seed}")
+
+
+# ---------------------------------------------------------------------------
+# gamma functions (src/scc/gamma_approximation.rs)
+# ---------------------------------------------------------------------------
+def _erf(x):
+    from scipy.special import erf
+    return erf(x)
+
+
+def _sigmas(z, hubbard):
+    hub = HUBBARD_U if hubbard is None else hubbard
+    return np.array([1.0 / (hub[int(a)] * math.sqrt(math.pi)) for a in z])
+
+
+def gamma_atomwise(z, xyz, r_lr=0.0, hubbard=None):
+    """gamma_AB = erf(c_AB r)/r, c_AB = 1/sqrt(2(s_A^2 + s_B^2 + r_lr^2/2)),
+    gamma_AA = 1/sqrt(pi (s_A^2 + r_lr^2/4)), s = 1/(U sqrt(pi)).
+    ``hubbard`` maps atomic number -> U (default: HUBBARD_U)."""
+    z = np.asarray(z)
+    sig = _sigmas(z, hubbard)
+    c = 1.0 / np.sqrt(2.0 * (sig[:, None] ** 2 + sig[None, :] ** 2 + 0.5 * r_lr ** 2))
+    d = np.linalg.norm(xyz[:, None, :] - xyz[None, :, :], axis=-1)
+    n = len(z)
+    g = np.empty((n, n))
+    off = ~np.eye(n, dtype=bool)
+    g[off] = _erf(c[off] * d[off]) / d[off]
+    g[np.diag_indices(n)] = 1.0 / np.sqrt(math.pi * (sig ** 2 + 0.25 * r_lr ** 2))
+    return g
+
+
+def gamma_gradients_atomwise(z, xyz, r_lr=0.0, hubbard=None):
+    """dgamma/dR as [3N, N, N] (gamma_approximation.rs:126-171, 256-282)."""
+    z = np.asarray(z)
+    n = len(z)
+    sig = _sigmas(z, hubbard)
+    c = 1.0 / np.sqrt(2.0 * (sig[:, None] ** 2 + sig[None, :] ** 2 + 0.5 * r_lr ** 2))
+    diff = xyz[:, None, :] - xyz[None, :, :]
+    d = np.linalg.norm(diff, axis=-1)
+    g1 = np.zeros((3 * n, n, n))
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                continue
+            r = d[i, j]
+            cv = c[i, j]
+            dv = 2.0 * cv / math.sqrt(math.pi) * math.exp(-(cv * r) ** 2) / r - math.erf(cv * r) / r ** 2
+            g1[3 * i:3 * i + 3, i, j] = diff[i, j] / r * dv
+    return g1
+
+
+# ---------------------------------------------------------------------------
+# synthetic orbitals
+# ---------------------------------------------------------------------------
+def synthetic_orbitals(mol, seed=0, gap=0.1):
+    """Haar-random orthogonal C (S = I) and sorted orbital energies with a
+    HOMO-LUMO gap >= `gap` Ha.  For throughput runs: A.x cost does not depend
+    on the data."""
+    rng = np.random.default_rng(seed + 1000)
+    n = mol.n_ao
+    q, r = np.linalg.qr(rng.standard_normal((n, n)))
+    q = q * np.sign(np.diag(r))[None, :]
+    o = mol.n_occ
+    occ = np.sort(rng.uniform(-1.0, -0.3, o))
+    virt = np.sort(rng.uniform(-0.3 + gap, 0.6, n - o))
+    return q, np.eye(n), np.concatenate([occ, virt])
+
+
+def tda_inputs(mol, seed=0, lc=True):
+    """Dict of TDA inputs for a Molecule (synthetic orbitals, real gammas)."""
+    c, s, orbe = synthetic_orbitals(mol, seed)
+    o = mol.n_occ
+    return dict(
+        n_atoms=mol.n_atoms, orbs_per_atom=mol.orbs_per_atom, xyz=mol.xyz,
+        occ_indices=list(range(o)), virt_indices=list(range(o, mol.n_ao)),
+        orbs=c, s=s, orbe=orbe,
+        gamma=gamma_atomwise(mol.z, mol.xyz, 0.0),
+        gamma_lr=gamma_atomwise(mol.z, mol.xyz, LONG_RANGE_RADIUS) if lc else None,
+    )

@@ -119,6 +119,12 @@ class TdaSystem:
             self.ctx.check(self.ctx.lib.tda_get_charges(self.ctx.handle, None, None, None, nat.dptr(omega)))
         self.properties.update(omega=omega, homo=self.occ_indices[-1], lumo=self.virt_indices[0])
 
+    def clear_charges(self):
+        """Forget the prepared charges (Properties::reset, properties/mod.rs:23-63)."""
+        self.ctx.check(self.ctx.lib.tda_clear_charges(self.ctx.handle))
+        for key in ("q_ov", "q_oo", "q_vv", "omega"):
+            self.properties.pop(key, None)
+
     # -- DavidsonEngine (solvers/mod.rs:20-34) --------------------------------
     def compute_products(self, x):
         """A x for every column of x [ov, k] (operator level, no cache)."""
