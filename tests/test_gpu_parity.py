@@ -216,3 +216,20 @@ def test_exchange_atom_blocking():
     assert np.abs(a2 - a1).max() < 1e-13
     full.close()
     blocked.close()
+
+
+def test_repeated_solves_switch_precision():
+    """One context, several solves: FP16 -> FP16 (charges cleared) -> FP64 ->
+    FP16.  The exchange intermediate is reused across precisions; its pad rows
+    (uracil: n_occ = 21, o_p = 64) must never leak stale bytes into a sum."""
+    m = molecule("uracil")
+    ref = _system(m)
+    w64, _ = ref.run_tda(4, 200, 1e-5)
+    s = _system(m, precision="fp16")
+    for prec in ["fp16", "fp16", "fp64", "fp16", "bf16", "fp32"]:
+        s.ctx.set_precision(prec)
+        s.clear_charges()
+        w, _ = s.run_tda(4, 200, 1e-5)
+        assert np.abs(w - w64).max() * HARTREE_TO_EV < 1e-3, prec
+    ref.close()
+    s.close()

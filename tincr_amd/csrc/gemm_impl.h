@@ -9,7 +9,8 @@
 // MFMA operand layouts used (lane l of 64, 16x16 output block):
 //   f64 16x16x4, f32 16x16x4 : A[l%16][l/16], B[l/16][l%16]
 //   f16/bf16 16x16x32        : A[l%16][8*(l/16)+e], B[8*(l/16)+e][l%16], e<8
-//   result (all)             : D[4*(l/16)+r][l%16], r<4
+//   result f32 accumulators   : D[4*(l/16)+r][l%16], r<4
+//   result f64 16x16x4        : D[(l/16)+4*r][l%16], r<4   (checked on gfx950)
 // LDS holds A as [k][m] (E=1 types) or [m][k] (E=8 types) so that fragment
 // reads are single ds_read instructions.
 #pragma once
@@ -28,6 +29,8 @@ struct Mfma<double> {
   static __device__ inline Acc op(Frag a, Frag b, Acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // result row of register r in lane l: rows interleave with stride 4
+  static __device__ inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <>
 struct Mfma<float> {
@@ -38,6 +41,7 @@ struct Mfma<float> {
   static __device__ inline Acc op(Frag a, Frag b, Acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 template <>
 struct Mfma<_Float16> {
@@ -48,6 +52,7 @@ struct Mfma<_Float16> {
   static __device__ inline Acc op(Frag a, Frag b, Acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 template <>
 struct Mfma<__bf16> {
@@ -58,6 +63,7 @@ struct Mfma<__bf16> {
   static __device__ inline Acc op(Frag a, Frag b, Acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 template <typename T, int N>
@@ -323,7 +329,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
       for (int j = 0; j < NI; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int64_t m = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
+          const int64_t m = m0 + wm0 + i * 16 + MF::row(lane, r);
           const int64_t n = n0 + wn0 + j * 16 + (lane & 15);
           if (m < g.M && n < g.N) ws[m * g.N + n] = acc[i][j][r];
         }
@@ -336,7 +342,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
   for (int i = 0; i < MI; ++i)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int64_t m = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
+      const int64_t m = m0 + wm0 + i * 16 + MF::row(lane, r);
       if (m >= g.M) continue;
       const int64_t cmo = g.cm(m);
 #pragma unroll

@@ -50,6 +50,9 @@ void launch_gs_step(const double* Pt, double* Wt, double* Q, int64_t L, int* cou
                     int max_new, hipStream_t s);
 void launch_copy(const double* src, double* dst, int64_t n, hipStream_t s);
 void launch_fill(double* dst, double v, int64_t n, hipStream_t s);
+// zero columns [c0, c0 + cw) of `rows` rows of length ld (elements of `esz`
+// bytes, esz even): the never-written pad rows of the exchange intermediate
+void launch_zero_runs(void* base, int64_t rows, int64_t ld, int64_t c0, int64_t cw, int esz, hipStream_t s);
 // byte-wise zero fill by a kernel (graph-capture safe, unlike hipMemsetAsync)
 void launch_zero_bytes(void* dst, size_t bytes, hipStream_t s);
 // tiny dense: C[M][N] = A[M][K] . B[K][N]   (FP64, row-major, small sizes)

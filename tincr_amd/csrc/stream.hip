@@ -469,6 +469,19 @@ void launch_zero_bytes(void* dst, size_t bytes, hipStream_t s) {
   }
 }
 
+__global__ void zero_runs_kernel(uint16_t* __restrict__ base, int64_t rows, int64_t ld, int64_t c0, int64_t cw) {
+  const int64_t total = rows * cw;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256)
+    base[(t / cw) * ld + c0 + t % cw] = 0;
+}
+
+void launch_zero_runs(void* base, int64_t rows, int64_t ld, int64_t c0, int64_t cw, int esz, hipStream_t s) {
+  if (rows == 0 || cw == 0) return;
+  const int64_t u = esz / 2;  // in 16-bit units
+  zero_runs_kernel<<<grid_for(rows * cw * u), 256, 0, s>>>((uint16_t*)base, rows, ld * u, c0 * u, cw * u);
+  TDA_LAUNCH_CHECK();
+}
+
 __global__ void small_gemm_kernel(const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C,
                                   int M, int N, int K) {
   const int64_t total = (int64_t)M * N;

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <numeric>
@@ -26,9 +27,12 @@ namespace {
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
-  // returns true when (re)allocated
-  bool ensure(size_t n) {
+  // Grows the buffer; returns true when (re)allocated.  Work already queued on
+  // `stream` may still read the old allocation, so the stream is drained
+  // before it is freed.
+  bool ensure(size_t n, hipStream_t stream = nullptr) {
     if (n <= bytes) return false;
+    if (p) TDA_HIP(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize());
     release();
     if (n == 0) return false;
     hipError_t e = hipMalloc(&p, n);
@@ -97,6 +101,7 @@ struct tda_ctx {
   DevBuf V, AV, H, Y, evals, RZ, R, P, W, norms, npart, idx, count, gs_scratch, eigw;
   // instrumentation
   bool timing = true;
+  bool verbose = false;
   double t_products_ms = 0;
   int n_products = 0, n_products_fp64 = 0, n_blocks = 1;
   // per exchange stage s (0: stage 1, 1: stage 3), per precision class p
@@ -337,12 +342,14 @@ void apply_A(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   size_t tb = std::max<size_t>(per_atom, std::min<size_t>(c->budget, (size_t)8 << 30));
   int NB = (int)std::max<size_t>(1, std::min<size_t>((size_t)N, tb / per_atom));
   c->n_blocks = (int)ceil_div(N, NB);
-  // T must hold finite values everywhere: its pad rows/columns are read
-  // (against zero G / q_vv pads) but never written.
-  if (c->T.ensure((size_t)NB * per_atom)) launch_zero_bytes(c->T.p, c->T.bytes, c->stream);
+  c->T.ensure((size_t)NB * per_atom);
   for (int B0 = 0; B0 < N; B0 += NB) {
     const int nb = std::min(NB, N - B0);
     const int64_t NBop = (int64_t)nb * op;
+    // Stage 3 reads T's pad rows j in [o, o_p) of every atom against the zero
+    // pads of G; they are never written by stage 1 and the buffer is shared by
+    // all precisions, so clear them (0 * NaN-pattern would poison the sum).
+    if (op > o) launch_zero_runs(c->T.p, (int64_t)k * vp * nb, op, o, op - o, (int)esz, c->stream);
     const int pclass = lp ? 1 : 0;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
     if (c->timing) {
@@ -458,8 +465,19 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       c->t_products_ms += ms;
       c->harvest_events();
     }
-    for (double e : h_err)
-      TDA_CHECK(std::isfinite(e), TDA_ERR_HIP, "Davidson: non-finite residual norm");
+    bool finite = true;
+    for (int r = 0; r < n; ++r) finite = finite && std::isfinite(h_err[r]) && std::isfinite(h_evals[r]);
+    if (c->verbose || !finite) {
+      double mx = 0, sum = 0;
+      int cvd = 0;
+      for (double e : h_err) { mx = std::max(mx, e); sum += e; cvd += (e < tol); }
+      // columns as solvers/utils.rs:82-121 prints them, plus precision and basis size
+      fprintf(stderr, "tincr_tda davidson prec=%d iter=%4d conv=%3d left=%3d dim_sub=%4d m=%4d total=%.3e max=%.3e w0=%.10f\n",
+              prec, it + 1, cvd, n - cvd, dim_sub, m, sum, mx, h_evals[0]);
+    }
+    if (!finite)
+      throw Error{TDA_ERR_HIP, "Davidson: non-finite residual norm or Ritz value (prec " + std::to_string(prec) +
+                                   ", iteration " + std::to_string(it + 1) + ", basis " + std::to_string(m) + ")"};
     int roots_cvd = 0;
     double max_err = 0;
     for (double e : h_err) {
@@ -602,6 +620,10 @@ int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx**
   }
   c->device = device;
   c->prec = precision;
+  {
+    const char* e = getenv("TINCR_TDA_VERBOSE");
+    c->verbose = e && e[0] && e[0] != '0';
+  }
   if (hbm_budget_bytes) c->budget = hbm_budget_bytes;
   int rc = guarded(c, [&] {
     int ndev = 0;
