@@ -2,11 +2,12 @@
 
 Workload: all-trans polyene C200H202 (402 atoms, 1002 AO, 501 occ x 501 virt,
 ov = 251,001), lowest 20 singlets, residual tolerance 1e-5.  One step = one
-full solve from device-resident orbitals: transition-charge build,
-gamma_lr.q_oo, Davidson (low-precision products, then the FP64 continuation)
-and the outputs (omega, X, q_trans, dipoles, f) back on the host.
+full solve from device-resident orbitals: transition-charge build, operator
+set-up, Davidson (low-precision products, then the FP64 continuation) and the
+outputs (omega, X, q_trans, dipoles, f) back on the host.
 
     python bench.py [--steps K] [--warmup W] [--prec fp16|bf16|fp32|fp64]
+                    [--operator ao|charges]
 
 Prints one JSON line (see the end of main()).
 """
@@ -24,6 +25,11 @@ sys.path.insert(0, ROOT)
 METRIC = "LC-TDA states/sec: mixed-prec batched vs plain FP64, same GPU, ≤1 meV ω err"
 HARTREE_TO_EV = 27.211386245988
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}
+KERNEL_NAMES = {
+    "ao": ["AO transition-density GEMM L X R^T with gamma_lr Hadamard epilogue",
+           "AO back-contraction GEMM R' (gamma_lr o D)^T"],
+    "charges": ["exchange stage 1 GEMM X q_vv^T", "exchange stage 3 GEMM G^T T"],
+}
 
 
 def build_inputs(system, seed):
@@ -37,10 +43,11 @@ def build_inputs(system, seed):
     return mol, sy.tda_inputs(mol, seed=seed)
 
 
-def make_system(inp, precision):
+def make_system(inp, precision, operator):
     from tincr_amd.tda import TdaSystem
     return TdaSystem(inp["n_atoms"], inp["orbs_per_atom"], inp["xyz"], inp["occ_indices"], inp["virt_indices"],
-                     inp["orbs"], inp["s"], inp["orbe"], inp["gamma"], inp["gamma_lr"], precision=precision)
+                     inp["orbs"], inp["s"], inp["orbe"], inp["gamma"], inp["gamma_lr"], precision=precision,
+                     operator=operator)
 
 
 def sync():
@@ -104,6 +111,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--prec", default="fp16", choices=["fp64", "fp32", "bf16", "fp16"])
     ap.add_argument("--system", default="polyene200")
+    ap.add_argument("--operator", default="ao", choices=["ao", "charges"])
     ap.add_argument("--states", type=int, default=20)
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--max-iter", type=int, default=450)
@@ -117,8 +125,8 @@ def main():
 
     mol, inp = build_inputs(args.system, args.seed)
     n = args.states
-    s64 = make_system(inp, "fp64")
-    slp = make_system(inp, args.prec) if args.prec != "fp64" else s64
+    s64 = make_system(inp, "fp64", args.operator)
+    slp = make_system(inp, args.prec, args.operator) if args.prec != "fp64" else s64
 
     # warm-up (kernel loading, allocations) -- untimed
     for _ in range(max(args.warmup, 0)):
@@ -159,7 +167,7 @@ def main():
     roofline = {
         "bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_TFLOPS[kprec], "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_TFLOPS[kprec], 4), "traffic": None,
-        "kernel": f"exchange stage {'1 (X.q_vv)' if si == 0 else '3 (G^T.T)'} {kprec} MFMA GEMM",
+        "kernel": KERNEL_NAMES[args.operator][si] + f" ({kprec} MFMA)",
         "launches": int(tot_n[si, pi]),
         "avg_launch_ms": round(tot_ms[si, pi] / max(tot_n[si, pi], 1), 4),
         "flops_per_launch": tot_fl[si, pi] / max(tot_n[si, pi], 1),
@@ -175,10 +183,11 @@ def main():
         "config": {
             "workload": f"LC-TDA {mol.name}: {mol.n_atoms} atoms, {mol.n_ao} AO, {mol.n_occ} occ x "
                         f"{mol.n_virt} virt (ov={mol.n_occ * mol.n_virt}), lowest {n} states, tol {args.tol}",
-            "precision": args.prec, "n_states": n, "tol": args.tol,
+            "precision": args.prec, "operator": args.operator, "n_states": n, "tol": args.tol,
         },
         "fp64": {"states_per_s": round(fp64_value, 4), "ms_per_solve": round(1e3 * t64, 3),
-                 "n_iter": st64["n_iter"], "n_products": st64["n_products"]},
+                 "n_iter": st64["n_iter"], "n_products": st64["n_products"],
+                 "t_products_ms": round(st64["t_products_ms"], 2)},
         "speedup_vs_fp64": round(value / fp64_value, 3),
         "max_abs_domega_meV": dw_mev, "max_abs_df": df,
         "mixed": {"n_iter": last["n_iter"], "n_iter_fp64": last["n_iter_fp64"],

@@ -36,6 +36,10 @@ extern "C" {
 #define TDA_PREC_BF16 2 /* BF16 exchange (per-atom scaled, FP32 accumulate) */
 #define TDA_PREC_FP16 3 /* FP16 exchange (per-atom scaled, FP32 accumulate) */
 
+/* form of the A.x operator */
+#define TDA_OP_AO 0      /* exchange contracted in the AO basis from C, SC and gamma_lr (default) */
+#define TDA_OP_CHARGES 1 /* exchange from the per-atom charges q_oo / q_vv (three-stage, as solvers/mod.rs:328-360) */
+
 typedef struct tda_ctx tda_ctx;
 
 typedef struct tda_stats {
@@ -49,8 +53,10 @@ typedef struct tda_stats {
   double t_total_ms;        /* tda_run wall time */
   double t_qbuild_ms;       /* q build + gamma_lr.q_oo (inside tda_run if not prepared) */
   double t_products_ms;     /* sum of A.x product time (device events) */
-  /* exchange contraction kernels, [stage][class]: stage 0 = X.q_vv (stage 1),
-   * 1 = G^T.T (stage 3); class 0 = FP64 products, 1 = low-precision products.
+  /* exchange contraction kernels, [stage][class].  TDA_OP_CHARGES: stage 0 =
+   * X.q_vv, 1 = G^T.T; TDA_OP_AO: stage 0 = transition densities L X R^T
+   * (gamma_lr Hadamard epilogue), 1 = their back-contraction with R'.
+   * class 0 = FP64 products, 1 = low-precision products.
    * Time of the main MFMA kernel only (device events on the context stream),
    * its launch count and its algorithmic flops. */
   double exch_ms[2][2];
@@ -64,6 +70,8 @@ int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx**
 void tda_ctx_destroy(tda_ctx* ctx);
 const char* tda_last_error(const tda_ctx* ctx);
 int tda_set_precision(tda_ctx* ctx, int precision);
+/* Selects the operator form (TDA_OP_*); both give the reference's A.x. */
+int tda_set_operator(tda_ctx* ctx, int op);
 
 /* Replaces trans_charges (src/excited_states/transition_charges.rs:8-15).
  * orbs, s: [n_ao, n_ao]; orbs_per_atom: [n_atoms] (AOs of each atom are

@@ -78,10 +78,11 @@ def test_trans_charges_legacy_water_golden_path():
 
 @pytest.mark.parametrize("mol", MOLECULES)
 @pytest.mark.parametrize("lc", [True, False])
-def test_products_fixtures(mol, lc):
-    print(f"products {mol} lc={lc}", flush=True)
+@pytest.mark.parametrize("op", ["ao", "charges"])
+def test_products_fixtures(mol, lc, op):
+    print(f"products {mol} lc={lc} op={op}", flush=True)
     m = molecule(mol)
-    sysg = _system(m, lc=lc)
+    sysg = _system(m, lc=lc, operator=op)
     sysg.prepare_tda()
     eng = _oracle_engine(m, lc)
     ov = eng.get_size()
@@ -113,10 +114,11 @@ def test_products_benzene_apbv_golden():
 
 
 @pytest.mark.parametrize("mol", MOLECULES)
-def test_run_tda_fixtures(mol):
-    print(f"run_tda {mol}", flush=True)
+@pytest.mark.parametrize("op", ["ao", "charges"])
+def test_run_tda_fixtures(mol, op):
+    print(f"run_tda {mol} op={op}", flush=True)
     m = molecule(mol)
-    sysg = _system(m)
+    sysg = _system(m, operator=op)
     sysg.prepare_tda()
     n = 4
     w, x = sysg.run_tda(n, 200, 1e-8)
@@ -153,14 +155,15 @@ def test_run_tda_bad_args():
 
 
 @pytest.mark.parametrize("case", ["water16", "polyene20"])
-def test_run_tda_synthetic_vs_oracle(case):
+@pytest.mark.parametrize("op", ["ao", "charges"])
+def test_run_tda_synthetic_vs_oracle(case, op):
     """Config-0-sized (H2O)_16 and a C20H22 polyene with synthetic orbitals:
     GPU Davidson vs the oracle Davidson (same guess, same rules)."""
     from tincr_amd import systems as sy
-    print(f"synthetic {case}", flush=True)
+    print(f"synthetic {case} op={op}", flush=True)
     m = _synthetic(sy.water_cluster, 16) if case == "water16" else _synthetic(sy.polyene, 20)
     n = 10
-    sysg = _system(m)
+    sysg = _system(m, operator=op)
     w, x = sysg.run_tda(n, 100, 1e-5)
     ref = orc.run_tda(_oracle_engine(m), m["xyz"], n, 100, 1e-5)
     assert np.abs(w - ref["omega"]).max() * HARTREE_TO_EV < 1e-6
@@ -172,10 +175,11 @@ def test_run_tda_synthetic_vs_oracle(case):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
-def test_products_mixed_precision(prec):
+@pytest.mark.parametrize("op", ["ao", "charges"])
+def test_products_mixed_precision(prec, op):
     from tincr_amd import systems as sy
     m = _synthetic(sy.water_cluster, 16)
-    sysg = _system(m, precision=prec)
+    sysg = _system(m, precision=prec, operator=op)
     eng = _oracle_engine(m)
     x = np.random.default_rng(9).standard_normal((eng.get_size(), 5))
     x /= np.linalg.norm(x, axis=0)
@@ -187,13 +191,14 @@ def test_products_mixed_precision(prec):
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
 @pytest.mark.parametrize("mol", ["uracil", "benzene"])
-def test_run_tda_mixed_precision(prec, mol):
+@pytest.mark.parametrize("op", ["ao", "charges"])
+def test_run_tda_mixed_precision(prec, mol, op):
     """Mixed-precision Davidson + FP64 continuation: omega within 1 meV of
     FP64 (in practice the FP64 continuation gives ~1e-10 Ha)."""
     m = molecule(mol)
     s64 = _system(m)
     w64, x64 = s64.run_tda(4, 200, 1e-5)
-    slp = _system(m, precision=prec)
+    slp = _system(m, precision=prec, operator=op)
     w, x = slp.run_tda(4, 200, 1e-5)
     assert np.abs(w - w64).max() * HARTREE_TO_EV < 1e-3
     assert slp.last_stats["converged"] == 1
@@ -208,8 +213,8 @@ def test_exchange_atom_blocking():
     m = _synthetic(sy.water_cluster, 16)
     eng = _oracle_engine(m)
     x = np.random.default_rng(11).standard_normal((eng.get_size(), 4))
-    full = _system(m)
-    blocked = _system(m, hbm_budget_bytes=4 * 64 * 64 * 8 * 5)
+    full = _system(m, operator="charges")
+    blocked = _system(m, hbm_budget_bytes=4 * 64 * 64 * 8 * 5, operator="charges")
     a1 = full.compute_products(x)
     a2 = blocked.compute_products(x)
     assert np.abs(a1 - eng.apply(x)).max() < 1e-12
@@ -225,11 +230,25 @@ def test_repeated_solves_switch_precision():
     m = molecule("uracil")
     ref = _system(m)
     w64, _ = ref.run_tda(4, 200, 1e-5)
-    s = _system(m, precision="fp16")
-    for prec in ["fp16", "fp16", "fp64", "fp16", "bf16", "fp32"]:
+    s = _system(m, precision="fp16", operator="charges")
+    for prec, op in [("fp16", "charges"), ("fp16", "charges"), ("fp64", "charges"), ("fp16", "charges"),
+                     ("bf16", "ao"), ("fp32", "ao"), ("fp16", "ao"), ("fp64", "ao"), ("fp16", "charges")]:
         s.ctx.set_precision(prec)
+        s.ctx.set_operator(op)
         s.clear_charges()
         w, _ = s.run_tda(4, 200, 1e-5)
-        assert np.abs(w - w64).max() * HARTREE_TO_EV < 1e-3, prec
+        assert np.abs(w - w64).max() * HARTREE_TO_EV < 1e-3, (prec, op)
     ref.close()
+    s.close()
+
+
+def test_ao_small_workspace_chunks():
+    """A tiny HBM budget forces the AO operator to split the vectors into
+    several workspace chunks."""
+    from tincr_amd import systems as sy
+    m = _synthetic(sy.water_cluster, 16)
+    eng = _oracle_engine(m)
+    x = np.random.default_rng(12).standard_normal((eng.get_size(), 7))
+    s = _system(m, hbm_budget_bytes=3 * 1024 * 1024)
+    assert np.abs(s.compute_products(x) - eng.apply(x)).max() < 1e-12
     s.close()

@@ -19,10 +19,11 @@ TDA_ERR_HIP = 4
 TDA_ERR_STATE = 5
 
 PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "fp16": 3}
+OPERATOR = {"ao": 0, "charges": 1}
 
 # every symbol include/tincr_tda.h declares
 EXPORTS = [
-    "tda_ctx_create", "tda_ctx_destroy", "tda_last_error", "tda_set_precision",
+    "tda_ctx_create", "tda_ctx_destroy", "tda_last_error", "tda_set_precision", "tda_set_operator",
     "tda_trans_charges", "tda_orbe_differences", "tda_set_system", "tda_prepare",
     "tda_clear_charges", "tda_get_charges", "tda_compute_products", "tda_precondition", "tda_run",
 ]
@@ -83,6 +84,7 @@ def load():
     lib.tda_last_error.argtypes = [P]
     lib.tda_last_error.restype = ctypes.c_char_p
     lib.tda_set_precision.argtypes = [P, i]
+    lib.tda_set_operator.argtypes = [P, i]
     lib.tda_trans_charges.argtypes = [P, i, i, I32, D, D, i, i, i, i, D, D, D]
     lib.tda_orbe_differences.argtypes = [D, i, D, i, D]
     lib.tda_set_system.argtypes = [P, i, i, I32, D, D, D, i, i, i, i, D, D, D]
@@ -141,6 +143,9 @@ class Context:
         if rc != TDA_OK and rc not in allow:
             raise TdaError(rc, self.lib.tda_last_error(self.handle).decode())
         return rc
+
+    def set_operator(self, op):
+        self.check(self.lib.tda_set_operator(self.handle, OPERATOR[op]))
 
     def set_precision(self, precision):
         self.check(self.lib.tda_set_precision(self.handle, PREC[precision]))

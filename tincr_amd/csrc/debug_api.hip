@@ -49,7 +49,8 @@ extern "C" int tda_debug_gemm(int in_prec, int a_kc, int b_kc, int64_t M, int64_
     g.bn = b_kc ? plain(K) : plain(1);
     g.cm = plain(N); g.cn = plain(1);
     g.batch = 1; g.alpha = alpha; g.beta = beta;
-    g.dmap = plain(1); g.xm = plain(0); g.xn = plain(0);
+    g.dm = plain(0); g.dmap = plain(1); g.xm = plain(0); g.xn = plain(0);
+    g.hm = plain(0); g.hn = plain(0);
     const DType dt = in_prec == 0 ? DT_F64 : in_prec == 1 ? DT_F32 : in_prec == 2 ? DT_BF16 : DT_F16;
     GemmPlan plan = gemm_plan(dt, M, N, K, 1, a_kc, b_kc, ksplit);
     size_t wsb = gemm_workspace_bytes(dt, plan, M, N, 1);

@@ -11,8 +11,9 @@
 namespace tda {
 
 enum Epi : int {
-  EPI_STORE = 0,  // C = alpha*acc*scale[z] + beta*C
-  EPI_DIAG = 1,   // C = alpha*acc + diag[dmap(n)] * X[xm(m) + xn(n)]
+  EPI_STORE = 0,     // C = alpha*acc*scale[z] + beta*C
+  EPI_DIAG = 1,      // C = alpha*acc + diag[dm(m) + dmap(n)] * X[z*x_batch + xm(m) + xn(n)]
+  EPI_HADAMARD = 2,  // C = alpha*acc * H[hm(m) + hn(n)]
 };
 
 struct GemmArgs {
@@ -32,9 +33,13 @@ struct GemmArgs {
   const float* batch_scale;  // optional: multiplies alpha per batch index
   // EPI_DIAG
   const double* diag;
-  IdxMap dmap;
+  IdxMap dm, dmap;
   const double* xsrc;
   IdxMap xm, xn;
+  int64_t x_batch;
+  // EPI_HADAMARD
+  const double* hsrc;
+  IdxMap hm, hn;
 };
 
 // Element type of an operand / result buffer.

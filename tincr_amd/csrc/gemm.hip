@@ -35,12 +35,23 @@ size_t gemm_workspace_bytes(DType in, const GemmPlan& p, int64_t M, int64_t N, i
 template <int EPI>
 static void dispatch_reduce(DType in, DType out, GemmArgs& g, hipStream_t s) {
   const bool acc64 = (in == DT_F64);
-  if (acc64 && out == DT_F64) launch_reduce<double, double, EPI>(g, s);
-  else if (!acc64 && out == DT_F64) launch_reduce<float, double, EPI>(g, s);
-  else if (!acc64 && out == DT_F32) launch_reduce<float, float, EPI>(g, s);
-  else if (EPI == EPI_STORE && !acc64 && out == DT_F16) launch_reduce<float, _Float16, EPI_STORE>(g, s);
-  else if (EPI == EPI_STORE && !acc64 && out == DT_BF16) launch_reduce<float, __bf16, EPI_STORE>(g, s);
-  else throw Error{2, "gemm reduce: unsupported type combination"};
+  if (acc64) {
+    if (out != DT_F64) throw Error{2, "gemm reduce: f64 input needs f64 output"};
+    launch_reduce<double, double, EPI>(g, s);
+    return;
+  }
+  if (out == DT_F64) {
+    if constexpr (EPI == EPI_HADAMARD) throw Error{2, "gemm reduce: unsupported epilogue"};
+    else launch_reduce<float, double, EPI>(g, s);
+    return;
+  }
+  if constexpr (EPI == EPI_DIAG) {
+    throw Error{2, "gemm reduce: unsupported epilogue"};
+  } else {
+    if (out == DT_F32) launch_reduce<float, float, EPI>(g, s);
+    else if (out == DT_F16) launch_reduce<float, _Float16, EPI>(g, s);
+    else launch_reduce<float, __bf16, EPI>(g, s);
+  }
 }
 
 void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g, const GemmPlan& plan,
@@ -70,6 +81,7 @@ void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g,
   if (after_main) TDA_HIP(hipEventRecord(after_main, stream));
   if (plan.ksplit > 1) {
     if (epi == EPI_DIAG) dispatch_reduce<EPI_DIAG>(in, out, g, stream);
+    else if (epi == EPI_HADAMARD) dispatch_reduce<EPI_HADAMARD>(in, out, g, stream);
     else dispatch_reduce<EPI_STORE>(in, out, g, stream);
   }
 }

@@ -4,11 +4,14 @@
 namespace tda {
 
 void gemm_dispatch_f32(int epi, DType out, int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s) {
-  if (out == DT_F64) {
+  if (out == DT_F32) {
+    if (epi == EPI_HADAMARD) launch_t<float, float, EPI_HADAMARD>(cfg, a_kc, b_kc, g, s);
+    else if (epi == EPI_STORE) launch_t<float, float, EPI_STORE>(cfg, a_kc, b_kc, g, s);
+    else throw Error{2, "gemm f32: unsupported epilogue"};
+  } else if (out == DT_F64) {
     if (epi == EPI_DIAG) launch_t<float, double, EPI_DIAG>(cfg, a_kc, b_kc, g, s);
-    else launch_t<float, double, EPI_STORE>(cfg, a_kc, b_kc, g, s);
-  } else if (out == DT_F32 && epi == EPI_STORE) {
-    launch_t<float, float, EPI_STORE>(cfg, a_kc, b_kc, g, s);
+    else if (epi == EPI_STORE) launch_t<float, double, EPI_STORE>(cfg, a_kc, b_kc, g, s);
+    else throw Error{2, "gemm f32: unsupported epilogue"};
   } else {
     throw Error{2, "gemm f32: unsupported output"};
   }

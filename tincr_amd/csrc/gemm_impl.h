@@ -95,6 +95,21 @@ __device__ inline void store_out(TO* p, double v) {
   *p = to_type<TO>(v);
 }
 
+// Final value of C(m, n) in batch zb from the accumulated sum.
+template <int EPI, typename TO>
+__device__ inline double epi_value(const GemmArgs& g, int zb, int64_t m, int64_t n, double acc, double alpha,
+                                   const TO* p) {
+  double v = alpha * acc;
+  if constexpr (EPI == EPI_DIAG) {
+    v += g.diag[g.dm(m) + g.dmap(n)] * g.xsrc[zb * g.x_batch + g.xm(m) + g.xn(n)];
+  } else if constexpr (EPI == EPI_HADAMARD) {
+    v *= g.hsrc[g.hm(m) + g.hn(n)];
+  } else {
+    if (g.beta != 0.0) v += g.beta * to_double<TO>(*p);
+  }
+  return v;
+}
+
 template <typename TI, typename TO, typename Cfg, bool A_KC, bool B_KC, int EPI>
 __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
   using MF = Mfma<TI>;
@@ -350,12 +365,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
         const int64_t n = n0 + wn0 + j * 16 + (lane & 15);
         if (n >= g.N) continue;
         TO* p = C + cmo + g.cn(n);
-        double v = alpha * (double)acc[i][j][r];
-        if constexpr (EPI == EPI_DIAG) {
-          v += g.diag[g.dmap(n)] * g.xsrc[g.xm(m) + g.xn(n)];
-        } else {
-          if (g.beta != 0.0) v += g.beta * to_double<TO>(*p);
-        }
+        const double v = epi_value<EPI, TO>(g, zb, m, n, (double)acc[i][j][r], alpha, p);
         store_out<TO>(p, v);
       }
     }
@@ -377,12 +387,7 @@ __global__ void __launch_bounds__(256) splitk_reduce_kernel(GemmArgs g) {
     double alpha = g.alpha;
     if (g.batch_scale) alpha *= (double)g.batch_scale[zb];
     TO* p = (TO*)g.C + zb * g.c_batch + g.cm(m) + g.cn(n);
-    double v = alpha * s;
-    if constexpr (EPI == EPI_DIAG) {
-      v += g.diag[g.dmap(n)] * g.xsrc[g.xm(m) + g.xn(n)];
-    } else {
-      if (g.beta != 0.0) v += g.beta * to_double<TO>(*p);
-    }
+    const double v = epi_value<EPI, TO>(g, zb, m, n, s, alpha, p);
     *p = to_type<TO>(v);
   }
 }

@@ -482,6 +482,107 @@ void launch_zero_runs(void* base, int64_t rows, int64_t ld, int64_t c0, int64_t 
   TDA_LAUNCH_CHECK();
 }
 
+// ---------------------------------------------------------------------------
+// AO-factored exchange / Coulomb operands
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) ao_operand_kernel(const double* __restrict__ C, const double* __restrict__ SC,
+                                                         int n_ao, int base, int P, int P_pad, int swap, int transpose,
+                                                         double* __restrict__ out) {
+  const int64_t n2 = 2 * (int64_t)n_ao;
+  const int64_t total = n2 * P_pad;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    const int64_t r = t / P_pad;
+    const int col = (int)(t % P_pad);
+    const int sidx = (int)(r / n_ao);
+    const int64_t mu = r % n_ao;
+    const double* src = (sidx == swap) ? SC : C;
+    const double v = (col < P) ? src[mu * n_ao + base + col] : 0.0;
+    if (transpose) out[(int64_t)col * n2 + r] = v;
+    else out[t] = v;
+  }
+}
+
+void launch_ao_operand(const double* C, const double* SC, int n_ao, int base, int P, int P_pad, int swap,
+                       int transpose, double* out, hipStream_t s) {
+  const int64_t total = 2 * (int64_t)n_ao * P_pad;
+  if (total == 0) return;
+  ao_operand_kernel<<<grid_for(total), 256, 0, s>>>(C, SC, n_ao, base, P, P_pad, swap, transpose, out);
+  TDA_LAUNCH_CHECK();
+}
+
+__global__ void __launch_bounds__(256) ao_gamma_kernel(const double* __restrict__ g, const int* __restrict__ atom,
+                                                       int n_ao, int n_atoms, double* __restrict__ Gt) {
+  const int64_t total = (int64_t)n_ao * n_ao;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256)
+    Gt[t] = g[(int64_t)atom[t / n_ao] * n_atoms + atom[t % n_ao]];
+}
+
+void launch_ao_gamma(const double* gamma_lr, const int* atom, int n_ao, int n_atoms, double* Gt, hipStream_t s) {
+  ao_gamma_kernel<<<grid_for((int64_t)n_ao * n_ao), 256, 0, s>>>(gamma_lr, atom, n_ao, n_atoms, Gt);
+  TDA_LAUNCH_CHECK();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) ao_charges_kernel(const T* __restrict__ M, int k, int n_ao,
+                                                        const int* __restrict__ ao_off, const double* __restrict__ Gt,
+                                                        double* __restrict__ u) {
+  const int A = blockIdx.x;
+  const int c = blockIdx.y;
+  const int64_t n2 = 2 * (int64_t)n_ao;
+  const T* Mc = M + (int64_t)c * n2 * n2;
+  double acc = 0.0;
+  for (int mu = ao_off[A] + threadIdx.x; mu < ao_off[A + 1]; mu += 64) {
+    // M = -1/4 gamma~ o D  =>  D = M / (-1/4 gamma~_mu,mu)
+    const double inv = -4.0 / Gt[(int64_t)mu * n_ao + mu];
+    const double d2 = to_double<T>(Mc[(int64_t)mu * n2 + n_ao + mu]);
+    const double d3 = to_double<T>(Mc[(int64_t)(n_ao + mu) * n2 + mu]);
+    acc += 0.5 * (d2 + d3) * inv;
+  }
+  acc = wave_sum(acc);
+  if (threadIdx.x == 0) u[(int64_t)A * k + c] = acc;
+}
+
+void launch_ao_charges(const void* M, int dtype, int k, int n_ao, const int* ao_off, int n_atoms, const double* Gt,
+                       double* u, hipStream_t s) {
+  dim3 grid(n_atoms, k);
+  switch (dtype) {
+    case PREC_FP64: ao_charges_kernel<double><<<grid, 64, 0, s>>>((const double*)M, k, n_ao, ao_off, Gt, u); break;
+    case PREC_FP32: ao_charges_kernel<float><<<grid, 64, 0, s>>>((const float*)M, k, n_ao, ao_off, Gt, u); break;
+    case PREC_BF16: ao_charges_kernel<__bf16><<<grid, 64, 0, s>>>((const __bf16*)M, k, n_ao, ao_off, Gt, u); break;
+    default: ao_charges_kernel<_Float16><<<grid, 64, 0, s>>>((const _Float16*)M, k, n_ao, ao_off, Gt, u); break;
+  }
+  TDA_LAUNCH_CHECK();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) ao_add_coulomb_kernel(T* __restrict__ M, int k, int n_ao,
+                                                             const int* __restrict__ atom,
+                                                             const double* __restrict__ w) {
+  const int64_t n2 = 2 * (int64_t)n_ao;
+  const int64_t total = (int64_t)k * n_ao;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    const int c = (int)(t / n_ao);
+    const int64_t mu = t % n_ao;
+    const double add = w[(int64_t)atom[mu] * k + c];
+    T* Mc = M + (int64_t)c * n2 * n2;
+    T* p1 = Mc + mu * n2 + n_ao + mu;
+    T* p2 = Mc + (n_ao + mu) * n2 + mu;
+    *p1 = to_type<T>(to_double<T>(*p1) + add);
+    *p2 = to_type<T>(to_double<T>(*p2) + add);
+  }
+}
+
+void launch_ao_add_coulomb(void* M, int dtype, int k, int n_ao, const int* atom, const double* w, hipStream_t s) {
+  const int g = grid_for((int64_t)k * n_ao);
+  switch (dtype) {
+    case PREC_FP64: ao_add_coulomb_kernel<double><<<g, 256, 0, s>>>((double*)M, k, n_ao, atom, w); break;
+    case PREC_FP32: ao_add_coulomb_kernel<float><<<g, 256, 0, s>>>((float*)M, k, n_ao, atom, w); break;
+    case PREC_BF16: ao_add_coulomb_kernel<__bf16><<<g, 256, 0, s>>>((__bf16*)M, k, n_ao, atom, w); break;
+    default: ao_add_coulomb_kernel<_Float16><<<g, 256, 0, s>>>((_Float16*)M, k, n_ao, atom, w); break;
+  }
+  TDA_LAUNCH_CHECK();
+}
+
 __global__ void small_gemm_kernel(const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C,
                                   int M, int N, int K) {
   const int64_t total = (int64_t)M * N;

@@ -92,6 +92,12 @@ struct tda_ctx {
   DevBuf C, S, SC, orbe, aooff, gamma, gamma_lr;
   // prepared operators (FP64)
   DevBuf qov, qoo, qvv, G, de;
+  // operator form: 0 = AO-factored exchange, 1 = per-atom charges (three-stage)
+  int op_kind = 0;
+  bool mo_built = false;  // q_oo / q_vv present
+  // AO-factored operands: Rt [2n_ao][v_p], Lm [2n_ao][o_p], Rpt [v_p][2n_ao],
+  // Lpt [o_p][2n_ao], Gt = AO-expanded gamma_lr [n_ao][n_ao]
+  DevBuf ao_atom, Gt, aoRt, aoLm, aoRpt, aoLpt, aoRt_lp, aoLm_lp, aoRpt_lp, aoLpt_lp, aoXR, aoM, aoF;
   // low precision copies
   int lp_prec = -1;
   DevBuf qov32, qvv_lp, G_lp, qvv_amax, G_amax, stage1_scale;
@@ -121,7 +127,8 @@ struct tda_ctx {
     for (DevBuf* b : {&C, &S, &SC, &orbe, &aooff, &gamma, &gamma_lr, &qov, &qoo, &qvv, &G, &de, &qov32, &qvv_lp,
                       &G_lp, &qvv_amax, &G_amax, &stage1_scale, &ws_gemm, &T, &xlp, &u, &uu, &vv32, &ax_tmp,
                       &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &P, &W, &norms, &npart, &idx, &count,
-                      &gs_scratch, &eigw})
+                      &gs_scratch, &eigw, &ao_atom, &Gt, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoRt_lp,
+                      &aoLm_lp, &aoRpt_lp, &aoLpt_lp, &aoXR, &aoM, &aoF})
       b->release();
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -175,7 +182,8 @@ GemmArgs gargs(const void* A, const void* B, void* C, int64_t M, int64_t N, int6
   g.ksplit = 1;
   g.alpha = 1.0;
   g.beta = 0.0;
-  g.dmap = plain(1); g.xm = plain(0); g.xn = plain(0);
+  g.dm = plain(0); g.dmap = plain(1); g.xm = plain(0); g.xn = plain(0);
+  g.hm = plain(0); g.hn = plain(0);
   return g;
 }
 
@@ -210,6 +218,19 @@ void prepare_lowprec(tda_ctx* c) {
   if (c->prec == PREC_FP64 || c->lp_prec == c->prec) return;
   const int N = c->N;
   const size_t esz = prec_bytes(c->prec);
+  if (c->has_lc && c->op_kind == 0) {
+    const int64_t n2 = 2 * (int64_t)c->nao;
+    struct Pair { DevBuf* src; DevBuf* dst; int64_t len; };
+    Pair pairs[4] = {{&c->aoRt, &c->aoRt_lp, n2 * c->vp}, {&c->aoLm, &c->aoLm_lp, n2 * c->op},
+                     {&c->aoRpt, &c->aoRpt_lp, n2 * c->vp}, {&c->aoLpt, &c->aoLpt_lp, n2 * c->op}};
+    for (Pair& pr : pairs) {
+      pr.dst->ensure((size_t)pr.len * esz);
+      launch_convert(pr.src->as<double>(), pr.len, 1, nullptr, pr.dst->p, c->prec, c->stream);
+    }
+    TDA_HIP(hipStreamSynchronize(c->stream));
+    c->lp_prec = c->prec;
+    return;
+  }
   // Coulomb operand in FP32 for every mixed precision
   c->qov32.ensure((size_t)N * c->ovp * 4);
   launch_convert(c->qov.as<double>(), c->ovp, N, nullptr, c->qov32.p, PREC_FP32, c->stream);
@@ -250,6 +271,44 @@ void prepare_lowprec(tda_ctx* c) {
   c->lp_prec = c->prec;
 }
 
+// q_oo, q_vv and G = gamma_lr q_oo (solvers/mod.rs:328-331, once per system)
+void build_mo_charges(tda_ctx* c) {
+  if (c->mo_built) return;
+  const int N = c->N;
+  c->qoo.ensure((size_t)N * c->op * c->op * 8);
+  c->qvv.ensure((size_t)N * c->vp * c->vp * 8);
+  build_charges(c, c->C.as<double>(), c->SC.as<double>(), c->aooff.as<int>(), N, c->nao, c->ob, c->o, c->vb, c->v,
+                c->op, c->vp, nullptr, c->qoo.as<double>(), c->qvv.as<double>());
+  if (c->has_lc) {
+    const int64_t loo = (int64_t)c->op * c->op;
+    c->G.ensure((size_t)N * loo * 8);
+    GemmArgs g = gargs(c->gamma_lr.p, c->qoo.p, c->G.p, N, loo, N, plain(N), plain(1), plain(loo), plain(1),
+                       plain(loo), plain(1));
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, g);
+  }
+  TDA_HIP(hipStreamSynchronize(c->stream));
+  c->mo_built = true;
+}
+
+void build_ao_operands(tda_ctx* c) {
+  const int nao = c->nao;
+  const int64_t n2 = 2 * (int64_t)nao;
+  const double* C = c->C.as<double>();
+  const double* SC = c->SC.as<double>();
+  c->aoRt.ensure((size_t)n2 * c->vp * 8);
+  c->aoLm.ensure((size_t)n2 * c->op * 8);
+  c->aoRpt.ensure((size_t)n2 * c->vp * 8);
+  c->aoLpt.ensure((size_t)n2 * c->op * 8);
+  c->Gt.ensure((size_t)nao * nao * 8);
+  // Rt rows (SC_v; C_v), Lm rows (SC_o; C_o): they build D = L X R^T
+  launch_ao_operand(C, SC, nao, c->vb, c->v, c->vp, 0, 0, c->aoRt.as<double>(), c->stream);
+  launch_ao_operand(C, SC, nao, c->ob, c->o, c->op, 0, 0, c->aoLm.as<double>(), c->stream);
+  // the back-contraction pairs each D block with the other operator: (C; SC), stored transposed
+  launch_ao_operand(C, SC, nao, c->vb, c->v, c->vp, 1, 1, c->aoRpt.as<double>(), c->stream);
+  launch_ao_operand(C, SC, nao, c->ob, c->o, c->op, 1, 1, c->aoLpt.as<double>(), c->stream);
+  launch_ao_gamma(c->gamma_lr.as<double>(), c->ao_atom.as<int>(), nao, c->N, c->Gt.as<double>(), c->stream);
+}
+
 void prepare(tda_ctx* c) {
   TDA_CHECK(c->has_system, TDA_ERR_STATE, "tda_prepare: no system loaded");
   if (c->prepared) {
@@ -260,25 +319,14 @@ void prepare(tda_ctx* c) {
   c->SC.ensure((size_t)nao * nao * 8);
   gemm_sc(c, c->S.as<double>(), c->C.as<double>(), c->SC.as<double>(), nao);
   c->qov.ensure((size_t)N * c->ovp * 8);
-  double* qoo = nullptr;
-  double* qvv = nullptr;
-  if (c->has_lc) {
-    c->qoo.ensure((size_t)N * c->op * c->op * 8);
-    c->qvv.ensure((size_t)N * c->vp * c->vp * 8);
-    qoo = c->qoo.as<double>();
-    qvv = c->qvv.as<double>();
-  }
   build_charges(c, c->C.as<double>(), c->SC.as<double>(), c->aooff.as<int>(), N, nao, c->ob, c->o, c->vb, c->v,
-                c->op, c->vp, c->qov.as<double>(), qoo, qvv);
+                c->op, c->vp, c->qov.as<double>(), nullptr, nullptr);
   c->de.ensure((size_t)c->ovp * 8);
   launch_delta_eps(c->orbe.as<double>(), c->ob, c->o, c->vb, c->v, c->vp, c->de.as<double>(), c->stream);
+  c->mo_built = false;
   if (c->has_lc) {
-    // G[B] = sum_C gamma_lr[B][C] q_oo[C]  (solvers/mod.rs:328-331, once per system)
-    const int64_t loo = (int64_t)c->op * c->op;
-    c->G.ensure((size_t)N * loo * 8);
-    GemmArgs g = gargs(c->gamma_lr.p, c->qoo.p, c->G.p, N, loo, N, plain(N), plain(1), plain(loo), plain(1),
-                       plain(loo), plain(1));
-    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, g);
+    if (c->op_kind == 1) build_mo_charges(c);
+    else build_ao_operands(c);
   }
   TDA_HIP(hipStreamSynchronize(c->stream));
   c->prepared = true;
@@ -388,11 +436,116 @@ void apply_A(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// A.x with the AO-factored exchange.  With the Mulliken charges
+// q^A_pq = 1/2 sum_{mu in A}(C_mu,p SC_mu,q + C_mu,q SC_mu,p) the exchange
+// sum_AB q_oo^A gamma_lr,AB q_vv^B x regroups as
+//   K x = 1/4 L'^T (gamma~ o (L x R^T)) R',  L = (SC_o; C_o), R = (SC_v; C_v),
+//   L' = (C_o; SC_o), R' = (C_v; SC_v), gamma~_{nu,mu} = gamma_lr[A(nu)][B(mu)],
+// and the Coulomb charges are q_ov x = 1/2 sum_{mu in A} (D_{(0,mu),(1,mu)} +
+// D_{(1,mu),(0,mu)}) with D = L x R^T; 2 q_ov^T gamma (q_ov x) adds gamma u on
+// the diagonals of the off-diagonal blocks of the middle matrix.
+// Four GEMMs per batch of k vectors:
+//   G1  XR_c = R x_c^T           [2n_ao][o]      (stored per vector)
+//   G2  M_c  = -1/4 gamma~ o (L XR_c^T)          [2n_ao][2n_ao]  (Hadamard epilogue)
+//   G3  F_c  = R' M_c^T ...      [v_p][2n_ao]
+//   G4  AX_c = de o x_c + L'^T F_c^T             [o][v_p]
+// ---------------------------------------------------------------------------
+void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
+  const int o = c->o, v = c->v, op = c->op, vp = c->vp, nao = c->nao, N = c->N;
+  const int64_t ovp = c->ovp, n2 = 2 * (int64_t)nao;
+  const bool lp = prec != PREC_FP64;
+  const DType dt = lp ? dtype_of(prec) : DT_F64;
+  const size_t esz = lp ? prec_bytes(prec) : 8;
+  const void* Xin = X;
+  if (lp) {
+    c->xlp.ensure((size_t)k * ovp * esz);
+    launch_cast_vectors(X, (int64_t)k * ovp, c->xlp.p, prec, c->stream);
+    Xin = c->xlp.p;
+  }
+  const void* Rt = lp ? c->aoRt_lp.p : c->aoRt.p;
+  const void* Lm = lp ? c->aoLm_lp.p : c->aoLm.p;
+  const void* Rpt = lp ? c->aoRpt_lp.p : c->aoRpt.p;
+  const void* Lpt = lp ? c->aoLpt_lp.p : c->aoLpt.p;
+  c->aoXR.ensure((size_t)k * n2 * op * esz);
+  c->aoM.ensure((size_t)k * n2 * n2 * esz);
+  c->aoF.ensure((size_t)k * vp * n2 * esz);
+  const int pclass = lp ? 1 : 0;
+  hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+  if (c->timing)
+    for (auto& ev : e) ev = c->get_event();
+  // G1: XR[c][(s,mu)][j] = sum_b Rt[(s,mu)][b] X[c][j][b]
+  {
+    GemmArgs g = gargs(Rt, Xin, c->aoXR.p, n2, (int64_t)k * o, vp, plain(vp), plain(1), plain(1), plain(vp),
+                       plain(op), split(o, n2 * op, 1));
+    run_gemm(c, dt, dt, EPI_STORE, true, true, g);
+  }
+  // G2: M[c] = -1/4 Gt o (Lm XR[c]^T)
+  if (c->timing) TDA_HIP(hipEventRecord(e[0], c->stream));
+  {
+    GemmArgs g = gargs(Lm, c->aoXR.p, c->aoM.p, n2, n2, o, plain(op), plain(1), plain(1), plain(op), plain(n2),
+                       plain(1));
+    g.batch = k;
+    g.b_batch = n2 * op;
+    g.c_batch = n2 * n2;
+    g.alpha = -0.25;
+    g.hsrc = c->Gt.as<double>();
+    g.hm = split(nao, 0, nao);
+    g.hn = split(nao, 0, 1);
+    run_gemm(c, dt, dt, EPI_HADAMARD, true, true, g, 0, e[1]);
+  }
+  // Coulomb: u = q_ov x from the D diagonals, w = gamma u, added to M
+  c->u.ensure((size_t)N * k * 8);
+  c->uu.ensure((size_t)N * k * 8);
+  launch_ao_charges(c->aoM.p, lp ? prec : PREC_FP64, k, nao, c->aooff.as<int>(), N, c->Gt.as<double>(),
+                    c->u.as<double>(), c->stream);
+  launch_small_gemm(c->gamma.as<double>(), c->u.as<double>(), c->uu.as<double>(), N, k, N, c->stream);
+  launch_ao_add_coulomb(c->aoM.p, lp ? prec : PREC_FP64, k, nao, c->ao_atom.as<int>(), c->uu.as<double>(),
+                        c->stream);
+  // G3: F[c][a][(t,nu)] = sum_(s,mu) Rpt[a][(s,mu)] M[c][(t,nu)][(s,mu)]
+  if (c->timing) TDA_HIP(hipEventRecord(e[2], c->stream));
+  {
+    GemmArgs g = gargs(Rpt, c->aoM.p, c->aoF.p, vp, n2, n2, plain(n2), plain(1), plain(1), plain(n2), plain(n2),
+                       plain(1));
+    g.batch = k;
+    g.b_batch = n2 * n2;
+    g.c_batch = vp * n2;
+    run_gemm(c, dt, dt, EPI_STORE, true, true, g, 0, e[3]);
+  }
+  // G4: AX[c][i][a] = de[i][a] X[c][i][a] + sum_(t,nu) Lpt[i][(t,nu)] F[c][a][(t,nu)]
+  {
+    GemmArgs g = gargs(Lpt, c->aoF.p, AX, o, vp, n2, plain(n2), plain(1), plain(1), plain(n2), plain(vp), plain(1));
+    g.batch = k;
+    g.b_batch = vp * n2;
+    g.c_batch = ovp;
+    g.diag = c->de.as<double>();
+    g.dm = plain(vp);
+    g.dmap = plain(1);
+    g.xsrc = X;
+    g.xm = plain(vp);
+    g.xn = plain(1);
+    g.x_batch = ovp;
+    run_gemm(c, dt, DT_F64, EPI_DIAG, true, true, g);
+  }
+  if (c->timing) {
+    c->recs.push_back({e[0], e[1], 0, pclass, 2.0 * k * (double)n2 * n2 * o});
+    c->recs.push_back({e[2], e[3], 1, pclass, 2.0 * k * (double)n2 * n2 * vp});
+  }
+}
+
 void products(tda_ctx* c, const double* X, int k, double* AX, int prec) {
-  // chunk so that one call never holds more than 64 vectors of workspace
-  for (int c0 = 0; c0 < k; c0 += 64) {
-    const int kc = std::min(64, k - c0);
-    apply_A(c, X + (size_t)c0 * c->ovp, kc, AX + (size_t)c0 * c->ovp, prec);
+  const bool ao = c->has_lc && c->op_kind == 0;
+  int chunk = 64;
+  if (ao) {
+    // per-vector AO workspace: XR + M + F
+    const double n2 = 2.0 * c->nao;
+    const double per = (n2 * c->op + n2 * n2 + c->vp * n2) * (prec == PREC_FP64 ? 8.0 : prec_bytes(prec));
+    chunk = (int)std::max(1.0, std::min(64.0, std::min((double)c->budget, 8.0 * (1ull << 30)) / per));
+  }
+  for (int c0 = 0; c0 < k; c0 += chunk) {
+    const int kc = std::min(chunk, k - c0);
+    if (ao) apply_A_ao(c, X + (size_t)c0 * c->ovp, kc, AX + (size_t)c0 * c->ovp, prec);
+    else apply_A(c, X + (size_t)c0 * c->ovp, kc, AX + (size_t)c0 * c->ovp, prec);
   }
   c->n_products += k;
   if (prec == PREC_FP64) c->n_products_fp64 += k;
@@ -659,6 +812,20 @@ int tda_set_precision(tda_ctx* c, int precision) {
   return TDA_OK;
 }
 
+int tda_set_operator(tda_ctx* c, int op) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  if (op != TDA_OP_AO && op != TDA_OP_CHARGES) {
+    c->err = "bad operator form";
+    return TDA_ERR_BAD_ARG;
+  }
+  if (op != c->op_kind) {
+    c->op_kind = op;
+    c->prepared = false;
+    c->lp_prec = -1;
+  }
+  return TDA_OK;
+}
+
 int tda_orbe_differences(const double* occ, int n_occ, const double* virt, int n_virt, double* omega) {
   if (!occ || !virt || !omega || n_occ < 0 || n_virt < 0) return TDA_ERR_BAD_ARG;
   for (int i = 0; i < n_occ; ++i)
@@ -739,6 +906,16 @@ int tda_set_system(tda_ctx* c, int n_atoms, int n_ao, const int32_t* opa, const 
       TDA_HIP(hipMemcpyAsync(c->gamma_lr.p, gamma_lr, (size_t)n_atoms * n_atoms * 8, hipMemcpyHostToDevice,
                              c->stream));
     }
+    {
+      std::vector<int> atom_of((size_t)n_ao);
+      for (int A = 0; A < n_atoms; ++A)
+        for (int mu = off[A]; mu < off[A + 1]; ++mu) atom_of[mu] = A;
+      c->ao_atom.ensure((size_t)n_ao * sizeof(int));
+      TDA_HIP(hipMemcpyAsync(c->ao_atom.p, atom_of.data(), (size_t)n_ao * sizeof(int), hipMemcpyHostToDevice,
+                             c->stream));
+      TDA_HIP(hipStreamSynchronize(c->stream));
+    }
+    c->mo_built = false;
     c->xyz.assign(xyz, xyz + (size_t)n_atoms * 3);
     c->h_de.resize((size_t)n_occ * n_virt);
     for (int i = 0; i < n_occ; ++i)
@@ -771,6 +948,7 @@ int tda_get_charges(tda_ctx* c, double* q_ov, double* q_oo, double* q_vv, double
       copy_unpad_rows(h.data(), N * o, v, vp, q_ov);
     }
     if ((q_oo || q_vv) && !c->has_lc) throw Error{TDA_ERR_STATE, "q_oo/q_vv are built only with gamma_lr"};
+    if (q_oo || q_vv) build_mo_charges(c);
     if (q_oo) {
       std::vector<double> h((size_t)N * op * op);
       TDA_HIP(hipMemcpy(h.data(), c->qoo.p, h.size() * 8, hipMemcpyDeviceToHost));

@@ -78,8 +78,9 @@ class TdaSystem:
     """DavidsonEngine + prepare_tda/run_tda of one System or Monomer."""
 
     def __init__(self, n_atoms, orbs_per_atom, xyz, occ_indices, virt_indices, orbs, s, orbe, gamma,
-                 gamma_lr=None, precision="fp64", device=0, hbm_budget_bytes=0):
+                 gamma_lr=None, precision="fp64", device=0, hbm_budget_bytes=0, operator="ao"):
         self.ctx = nat.Context(device, precision, hbm_budget_bytes)
+        self.ctx.set_operator(operator)
         self.n_atoms = n_atoms
         self.occ_indices = list(occ_indices)
         self.virt_indices = list(virt_indices)
