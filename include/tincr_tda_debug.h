@@ -20,6 +20,10 @@ int tda_debug_gemm(int in_prec, int a_kc, int b_kc, int64_t M, int64_t N, int64_
 /* Lowest n eigenpairs of the symmetric m x m matrix H (upper triangle read):
  * evals [n] ascending, Y [n][m] (row r = eigenvector r). */
 int tda_debug_eigh(int m, int n, const double* H, double* evals, double* Y);
+/* As tda_debug_eigh, and phase_us[5] receives the device time (us) of the
+ * phases: copy, tridiagonalisation, bisection, inverse iteration,
+ * back-transformation. */
+int tda_debug_eigh_timed(int m, int n, const double* H, double* evals, double* Y, double* phase_us);
 
 #ifdef __cplusplus
 }

@@ -161,6 +161,8 @@ def load_debug():
     lib.tda_debug_gemm.restype = ctypes.c_int
     lib.tda_debug_eigh.argtypes = [i, i, D, D, D]
     lib.tda_debug_eigh.restype = ctypes.c_int
+    lib.tda_debug_eigh_timed.argtypes = [i, i, D, D, D, D]
+    lib.tda_debug_eigh_timed.restype = ctypes.c_int
     return lib
 
 
@@ -179,13 +181,14 @@ def debug_gemm(a, b, precision="fp64", a_kc=True, b_kc=False, alpha=1.0, beta=0.
     return out
 
 
-def debug_eigh(h, n):
+def debug_eigh(h, n, timed=False):
     lib = load_debug()
     h = f64(h)
     m = h.shape[0]
     ev = np.empty(n)
     y = np.empty((n, m))
-    rc = lib.tda_debug_eigh(m, n, dptr(h), dptr(ev), dptr(y))
+    ph = np.zeros(5)
+    rc = lib.tda_debug_eigh_timed(m, n, dptr(h), dptr(ev), dptr(y), dptr(ph) if timed else None)
     if rc != TDA_OK:
         raise TdaError(rc, "tda_debug_eigh failed")
-    return ev, y
+    return (ev, y, ph) if timed else (ev, y)

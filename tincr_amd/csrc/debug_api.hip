@@ -63,11 +63,23 @@ extern "C" int tda_debug_gemm(int in_prec, int a_kc, int b_kc, int64_t M, int64_
 }
 
 extern "C" int tda_debug_eigh(int m, int n, const double* H, double* evals, double* Y) {
+  return tda_debug_eigh_timed(m, n, H, evals, Y, nullptr);
+}
+
+extern "C" int tda_debug_eigh_timed(int m, int n, const double* H, double* evals, double* Y, double* phase_us) {
   return guard([&] {
-    Buf h((size_t)m * m * 8), w(eigh_work_doubles(m) * 8), e((size_t)n * 8), y((size_t)n * m * 8);
+    Buf h((size_t)m * m * 8), w(eigh_work_doubles(m) * 8), e((size_t)n * 8), y((size_t)n * m * 8), clk(64);
     TDA_HIP(hipMemcpy(h.p, H, (size_t)m * m * 8, hipMemcpyHostToDevice));
-    launch_eigh((const double*)h.p, m, m, n, (double*)w.p, (double*)e.p, (double*)y.p, m, 0);
+    launch_eigh((const double*)h.p, m, m, n, (double*)w.p, (double*)e.p, (double*)y.p, m, 0,
+                phase_us ? (unsigned long long*)clk.p : nullptr);
     TDA_HIP(hipDeviceSynchronize());
+    if (phase_us) {
+      unsigned long long st[6];
+      TDA_HIP(hipMemcpy(st, clk.p, sizeof(st), hipMemcpyDeviceToHost));
+      int rate_khz = 0;
+      TDA_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0));
+      for (int i = 0; i < 5; ++i) phase_us[i] = (double)(st[i + 1] - st[i]) * 1e3 / (double)rate_khz;
+    }
     TDA_HIP(hipMemcpy(evals, e.p, (size_t)n * 8, hipMemcpyDeviceToHost));
     TDA_HIP(hipMemcpy(Y, y.p, (size_t)n * m * 8, hipMemcpyDeviceToHost));
   });

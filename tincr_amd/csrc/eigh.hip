@@ -1,7 +1,7 @@
 // FP64 symmetric eigensolver for the Davidson subspace (davidson.rs:93,
 // `a_proj.eigh(UPLO::Upper)` followed by taking the n lowest pairs, :96-97).
 //
-// One workgroup of 512 threads per problem (m <= 1024):
+// One workgroup of 1024 threads per problem (m <= 1024):
 //   1. Householder reduction to tridiagonal form T = Q^T H Q.  Each step does
 //      a symmetric mat-vec and a rank-2 update of the trailing block, one wave
 //      per row (coalesced along the row).  The trailing block lives in global
@@ -19,10 +19,10 @@
 
 namespace tda {
 
-constexpr int EIGH_THREADS = 512;
+constexpr int EIGH_THREADS = 1024;
 constexpr int EIGH_NW = EIGH_THREADS / 64;
 constexpr int EIGH_MAXM = 1024;
-constexpr int EIGH_POOL = 14336;  // doubles of LDS pool (112 KiB)
+constexpr int EIGH_POOL = 12288;  // doubles of LDS pool (96 KiB)
 
 size_t eigh_work_doubles(int m) {
   // A copy (m*m) + tau (m) + spill area for vectors (64*m) + slots (6*m*64)
@@ -89,15 +89,19 @@ __device__ void tri_lu(const double* d, const double* e, int m, double lam, doub
 
 __device__ void tri_solve(const double* ri, const double* u1, const double* u2, const double* l, const double* piv,
                           int m, double* x) {
+  // forward (row swaps and L); x[i] is carried in a register
+  double xi = x[0];
   for (int i = 0; i < m - 1; ++i) {
+    const double xn = x[i + 1];
     if (piv[i] < 0.0) {
-      const double t = x[i];
-      x[i] = x[i + 1];
-      x[i + 1] = t - l[i] * x[i];
+      x[i] = xn;
+      xi = xi - l[i] * xn;
     } else {
-      x[i + 1] -= l[i] * x[i];
+      x[i] = xi;
+      xi = xn - l[i] * xi;
     }
   }
+  x[m - 1] = xi;
   double x1 = 0.0, x2 = 0.0;  // x[i+1], x[i+2]
   for (int i = m - 1; i >= 0; --i) {
     const double v = (x[i] - u1[i] * x1 - u2[i] * x2) * ri[i];
@@ -109,10 +113,15 @@ __device__ void tri_solve(const double* ri, const double* u1, const double* u2, 
 
 __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __restrict__ H, int ldh, int m, int n,
                                                             double* __restrict__ work, double* __restrict__ evals,
-                                                            double* __restrict__ Y, int ldy) {
+                                                            double* __restrict__ Y, int ldy,
+                                                            unsigned long long* __restrict__ clk) {
+#define EIGH_STAMP(i) \
+  if (clk && threadIdx.x == 0) clk[i] = wall_clock64();
   __shared__ double pool[EIGH_POOL];
-  __shared__ double sv[EIGH_MAXM];   // Householder vector
-  __shared__ double sw[EIGH_MAXM];   // p, then w
+  __shared__ double sv[EIGH_MAXM];   // pending Householder vector v_{k-1}
+  __shared__ double sw[EIGH_MAXM];   // pending w_{k-1}
+  __shared__ double snv[EIGH_MAXM];  // current Householder vector
+  __shared__ double snp[EIGH_MAXM];  // current p
   __shared__ double sd[EIGH_MAXM];   // diagonal of T
   __shared__ double se[EIGH_MAXM];   // off-diagonal of T
   __shared__ double se2[EIGH_MAXM];  // squares of off-diagonal
@@ -138,6 +147,7 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
     return s;
   };
 
+  EIGH_STAMP(0)
   // 1. symmetric working copy from the upper triangle
   for (int64_t t = tid; t < (int64_t)m * m; t += EIGH_THREADS) {
     const int i = (int)(t / m), j = (int)(t % m);
@@ -145,14 +155,22 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
   }
   __syncthreads();
 
-  // 2. Householder tridiagonalisation; element (i, j) of the trailing matrix
-  //    is Am[(i - k0) * lda + (j - k0)].
+  EIGH_STAMP(1)
+  // 2. Householder tridiagonalisation (lower; A(k, j) == A(j, k)).  Element
+  //    (i, j) of the working matrix is Am[(i - k0) * lda + (j - k0)].  The
+  //    rank-2 update of step k-1 is kept pending and applied inside step k's
+  //    mat-vec pass, so each step reads and writes the trailing block once.
+  //    Mat-vec in column form: thread (i, seg) sums A(j, i) v_j over its
+  //    segment of j (coalesced over i), partial sums combined through LDS.
   double* Am = A;
   int k0 = 0, lda = m;
   bool in_lds = false;
+  bool pending = false;  // (pv_v, pv_w) = (v_{k-1}, w_{k-1}) not yet applied to rows/cols >= k
+  double* part = pool + EIGH_POOL - 2048;  // segment partial sums (global phase only)
   for (int k = 0; k < m - 2; ++k) {
-    if (!in_lds && (m - k) * (m - k) <= EIGH_POOL) {
-      // move the trailing block rows/cols k..m-1 into LDS
+    const int nt = m - k - 1;  // trailing size (rows/cols k+1..m-1)
+    if (!in_lds && (m - k) * (m - k) <= EIGH_POOL - 2048) {
+      // move rows/cols k..m-1 into LDS (the pending update travels along)
       const int nb = m - k;
       for (int t = tid; t < nb * nb; t += EIGH_THREADS) {
         const int i = t / nb, j = t % nb;
@@ -164,59 +182,82 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
       lda = nb;
       in_lds = true;
     }
-    const double* rowk = Am + (int64_t)(k - k0) * lda - k0;  // rowk[j] = A(k, j)
-    double part = 0.0;
-    for (int j = k + 2 + tid; j < m; j += EIGH_THREADS) part += rowk[j] * rowk[j];
-    const double sigma = block_sum(part);
+    double* rowk = Am + (int64_t)(k - k0) * lda - k0;  // rowk[j] = A(k, j)
+    // bring row k up to date
+    if (pending)
+      for (int j = k + tid; j < m; j += EIGH_THREADS) rowk[j] -= sv[k] * sw[j] + sw[k] * sv[j];
+    __syncthreads();
+    double part_s = 0.0;
+    for (int j = k + 2 + tid; j < m; j += EIGH_THREADS) part_s += rowk[j] * rowk[j];
+    const double sigma = block_sum(part_s);
     const double alpha = rowk[k + 1];
     if (tid == 0) sd[k] = rowk[k];
-    if (sigma == 0.0) {
-      if (tid == 0) { se[k] = alpha; tau[k] = 0.0; }
-      __syncthreads();
-      continue;
+    double tk = 0.0, beta = alpha, sc = 0.0;
+    if (sigma != 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+      tk = (beta - alpha) / beta;
+      sc = 1.0 / (alpha - beta);
     }
-    const double beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
-    const double tk = (beta - alpha) / beta;
-    const double sc = 1.0 / (alpha - beta);
+    // new Householder vector into snv (kept in global row k for the back-transformation)
     for (int j = k + 1 + tid; j < m; j += EIGH_THREADS) {
       const double vj = (j == k + 1) ? 1.0 : rowk[j] * sc;
-      sv[j] = vj;
-      if (j >= k + 2) A[(int64_t)k * m + j] = vj;  // kept for the back-transformation
+      snv[j] = vj;
+      if (j >= k + 2) A[(int64_t)k * m + j] = vj;
     }
     __syncthreads();
-    // p = tk * A22 v by columns: p_i = sum_j A(j, i) v_j, thread per i, the
-    // j loop unrolled so that loads (coalesced over i) overlap
-    double pv = 0.0;
-    for (int i = k + 1 + tid; i < m; i += EIGH_THREADS) {
-      const double* ci = Am + (i - k0) - (int64_t)k0 * lda;  // ci[j * lda] = A(j, i)
-      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-      int j = k + 1;
-      for (; j + 3 < m; j += 4) {
-        a0 += ci[(int64_t)j * lda] * sv[j];
-        a1 += ci[(int64_t)(j + 1) * lda] * sv[j + 1];
-        a2 += ci[(int64_t)(j + 2) * lda] * sv[j + 2];
-        a3 += ci[(int64_t)(j + 3) * lda] * sv[j + 3];
+    // fused pass over the trailing block: apply the pending update, then p = A v
+    const int segs = max(1, min(8, EIGH_THREADS / nt));
+    {
+      const int i = k + 1 + tid % nt;
+      const int seg = tid / nt;
+      double acc = 0.0;
+      if (seg < segs) {
+        double* ci = Am + (i - k0) - (int64_t)k0 * lda;  // ci[j * lda] = A(j, i)
+        const double vpi = pending ? sv[i] : 0.0, wpi = pending ? sw[i] : 0.0;
+        const int len = (nt + segs - 1) / segs;
+        const int j0 = k + 1 + seg * len, j1 = min(m, j0 + len);
+#pragma unroll 4
+        for (int j = j0; j < j1; ++j) {
+          double a = ci[(int64_t)j * lda];
+          if (pending) {
+            a -= sv[j] * wpi + sw[j] * vpi;
+            ci[(int64_t)j * lda] = a;
+          }
+          acc += a * snv[j];
+        }
+        if (segs > 1) part[seg * nt + (i - k - 1)] = acc;
       }
-      for (; j < m; ++j) a0 += ci[(int64_t)j * lda] * sv[j];
-      const double pi = ((a0 + a1) + (a2 + a3)) * tk;
-      sw[i] = pi;
-      pv += pi * sv[i];
+      if (segs > 1) {
+        __syncthreads();
+        if (tid < nt) {
+          double s = 0.0;
+          for (int q = 0; q < segs; ++q) s += part[q * nt + tid];
+          acc = s;
+        }
+      }
+      __syncthreads();  // all reads of sv / sw (pending update) done
+      if (tid < nt) snp[k + 1 + tid] = acc * tk;
     }
-    const double ptv = block_sum(pv);
-    const double K = 0.5 * tk * ptv;
-    for (int j = k + 1 + tid; j < m; j += EIGH_THREADS) sw[j] -= K * sv[j];
     __syncthreads();
-    // A22 -= v w^T + w v^T, one wave per row
-    for (int i = k + 1 + wave; i < m; i += EIGH_NW) {
-      double* ri = Am + (int64_t)(i - k0) * lda - k0;
-      const double vi = sv[i], wi = sw[i];
-      for (int j = k + 1 + lane; j < m; j += 64) ri[j] -= vi * sw[j] + wi * sv[j];
+    double pvp = 0.0;
+    for (int j = k + 1 + tid; j < m; j += EIGH_THREADS) pvp += snp[j] * snv[j];
+    const double ptv = block_sum(pvp);
+    const double K = 0.5 * tk * ptv;
+    // the new (v, w) become the pending update
+    for (int j = k + 1 + tid; j < m; j += EIGH_THREADS) {
+      sv[j] = snv[j];
+      sw[j] = snp[j] - K * snv[j];
     }
+    pending = (sigma != 0.0);
     if (tid == 0) { se[k] = beta; tau[k] = tk; }
     __syncthreads();
   }
   if (tid == 0) {
-    auto at = [&](int i, int j) { return Am[(int64_t)(i - k0) * lda + (j - k0)]; };
+    auto at = [&](int i, int j) {
+      double a = Am[(int64_t)(i - k0) * lda + (j - k0)];
+      if (pending) a -= sv[i] * sw[j] + sw[i] * sv[j];
+      return a;
+    };
     if (m >= 2) {
       sd[m - 2] = at(m - 2, m - 2);
       se[m - 2] = at(m - 1, m - 2);
@@ -247,6 +288,7 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
   __syncthreads();
   const double glo = sval[0], ghi = sval[1], tnorm = sval[2], pivmin = sval[3];
 
+  EIGH_STAMP(2)
   // 3. eigenvalues: one wave per root, 64-way multisection
   for (int r = wave; r < n; r += EIGH_NW) {
     double a = glo, b = ghi;
@@ -269,6 +311,7 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
   }
   __syncthreads();
 
+  EIGH_STAMP(3)
   // 4. eigenvectors of T by inverse iteration.  With eigenvalues bisected to
   //    full precision, vectors of roots separated by gap come out orthogonal
   //    to ~eps*||T||/gap; roots closer than 1e-9*||T|| form clusters that one
@@ -282,107 +325,158 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
   }
   __syncthreads();
   const int nclus = snclus;
-  // LDS slots of 6*m doubles (u0, u1, u2, l, piv, z); spill to global if needed
+  // LDS slots of 6*m doubles (u0, u1, u2, l, piv, z); clusters in rounds
   const int slot_sz = 6 * m;
-  const int nslots_lds = EIGH_POOL / slot_sz;
+  const int nslots = max(1, min(64, EIGH_POOL / slot_sz));
   double* gslots = tau + m + (size_t)64 * m;
-  for (int c = tid; c < nclus; c += EIGH_THREADS) {
-    double* sl = (c < nslots_lds) ? pool + (size_t)c * slot_sz : gslots + (size_t)c * slot_sz;
-    double *u0 = sl, *u1 = sl + m, *u2 = sl + 2 * m, *l = sl + 3 * m, *piv = sl + 4 * m, *z = sl + 5 * m;
-    for (int r = cl_start[c]; r < cl_start[c + 1]; ++r) {
-      double lam = evals[r];
-      if (r > cl_start[c] && lam - evals[r - 1] < 10.0 * 2.2e-16 * tnorm) lam = evals[r - 1] + 10.0 * 2.2e-16 * tnorm;
-      tri_lu(sd, se, m, lam, pivmin, u0, u1, u2, l, piv);
-      double nrm = 0.0;
-      for (int i = 0; i < m; ++i) { z[i] = start_entry(i, r); nrm += z[i] * z[i]; }
-      nrm = 1.0 / sqrt(nrm);
-      for (int i = 0; i < m; ++i) z[i] *= nrm;
-      for (int it = 0; it < 2; ++it) {
-        tri_solve(u0, u1, u2, l, piv, m, z);
-        for (int s = cl_start[c]; s < r; ++s) {
-          const double* zs = Y + (size_t)s * ldy;
-          double dt = 0.0;
-          for (int i = 0; i < m; ++i) dt += zs[i] * z[i];
-          for (int i = 0; i < m; ++i) z[i] -= dt * zs[i];
-        }
-        double s2 = 0.0;
-        for (int i = 0; i < m; ++i) s2 += z[i] * z[i];
-        if (!(s2 > 0.0) || !isfinite(s2)) {  // rescale by the max entry first
-          double mx = 0.0;
-          for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(z[i]));
-          if (!(mx > 0.0) || !isfinite(mx)) {
-            for (int i = 0; i < m; ++i) z[i] = 0.0;
-            z[r % m] = 1.0;
-            mx = 1.0;
+  for (int base = 0; base < nclus; base += nslots) {
+    const int c = base + tid;
+    if (tid < nslots && c < nclus) {
+      double* sl = (slot_sz <= EIGH_POOL) ? pool + (size_t)tid * slot_sz : gslots + (size_t)c * slot_sz;
+      double *u0 = sl, *u1 = sl + m, *u2 = sl + 2 * m, *l = sl + 3 * m, *piv = sl + 4 * m, *z = sl + 5 * m;
+      for (int r = cl_start[c]; r < cl_start[c + 1]; ++r) {
+        double lam = evals[r];
+        if (r > cl_start[c] && lam - evals[r - 1] < 10.0 * 2.2e-16 * tnorm)
+          lam = evals[r - 1] + 10.0 * 2.2e-16 * tnorm;
+        tri_lu(sd, se, m, lam, pivmin, u0, u1, u2, l, piv);
+        double nrm = 0.0;
+        for (int i = 0; i < m; ++i) { z[i] = start_entry(i, r); nrm += z[i] * z[i]; }
+        nrm = 1.0 / sqrt(nrm);
+        for (int i = 0; i < m; ++i) z[i] *= nrm;
+        for (int it = 0; it < 1; ++it) {
+          tri_solve(u0, u1, u2, l, piv, m, z);
+          for (int s2 = cl_start[c]; s2 < r; ++s2) {
+            const double* zs = Y + (size_t)s2 * ldy;
+            double dt = 0.0;
+            for (int i = 0; i < m; ++i) dt += zs[i] * z[i];
+            for (int i = 0; i < m; ++i) z[i] -= dt * zs[i];
           }
-          const double im = 1.0 / mx;
-          s2 = 0.0;
-          for (int i = 0; i < m; ++i) { z[i] *= im; s2 += z[i] * z[i]; }
+          double s2 = 0.0;
+          for (int i = 0; i < m; ++i) s2 += z[i] * z[i];
+          if (!(s2 > 0.0) || !isfinite(s2)) {  // rescale by the max entry first
+            double mx = 0.0;
+            for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(z[i]));
+            if (!(mx > 0.0) || !isfinite(mx)) {
+              for (int i = 0; i < m; ++i) z[i] = 0.0;
+              z[r % m] = 1.0;
+              mx = 1.0;
+            }
+            const double im = 1.0 / mx;
+            s2 = 0.0;
+            for (int i = 0; i < m; ++i) { z[i] *= im; s2 += z[i] * z[i]; }
+          }
+          const double inv = 1.0 / sqrt(s2);
+          for (int i = 0; i < m; ++i) z[i] *= inv;
         }
-        const double inv = 1.0 / sqrt(s2);
-        for (int i = 0; i < m; ++i) z[i] *= inv;
+        double* y = Y + (size_t)r * ldy;
+        for (int i = 0; i < m; ++i) y[i] = z[i];
       }
+    }
+    __syncthreads();
+  }
+
+  EIGH_STAMP(4)
+  // 5. back-transformation Y = H_0 H_1 ... H_{m-3} Z in blocks of BW
+  //    reflectors: H_s ... H_{s+BW-1} = I - V T V^T (T upper triangular,
+  //    forward accumulation as LAPACK dlarft), applied last block first:
+  //    Z -= V (T (V^T Z)).  Z and the V panel live in LDS when they fit.
+  constexpr int BW = 16;
+  const int nref = max(0, m - 2);  // reflectors 0..m-3
+  const bool fits = (size_t)n * m + (size_t)BW * m + 2 * BW * BW + (size_t)BW * n <= (size_t)EIGH_POOL;
+  if (fits && nref > 0) {
+    double* Zs = pool;                       // [n][m]
+    double* Vp = pool + (size_t)n * m;       // [BW][m]  (row q = reflector s+q)
+    double* Tm = Vp + (size_t)BW * m;        // [BW][BW]
+    double* Wm = Tm + BW * BW;               // [BW][n], then Gv [BW][BW]
+    for (int t = tid; t < n * m; t += EIGH_THREADS) Zs[t] = Y[(size_t)(t / m) * ldy + (t % m)];
+    const int nblk = (nref + BW - 1) / BW;
+    for (int blk = nblk - 1; blk >= 0; --blk) {
+      const int s0 = blk * BW;
+      const int bw = min(BW, nref - s0);
+      // panel: Vp[q][i] = v_{s0+q}[i]  (0 for i <= s0+q, 1 at s0+q+1)
+      for (int t = tid; t < bw * m; t += EIGH_THREADS) {
+        const int q = t / m, i = t % m, kk = s0 + q;
+        Vp[t] = (i <= kk) ? 0.0 : (i == kk + 1) ? 1.0 : A[(int64_t)kk * m + i];
+      }
+      __syncthreads();
+      // Gram matrix Gv[p][q] = v_p . v_q (p < q), one wave per pair
+      double* Gv = Wm + BW * n;  // [BW][BW]
+      for (int pq = wave; pq < bw * bw; pq += EIGH_NW) {
+        const int p = pq / bw, q = pq % bw;
+        if (p >= q) continue;
+        double g = 0.0;
+        for (int i = s0 + q + 1 + lane; i < m; i += 64) g += Vp[p * m + i] * Vp[q * m + i];
+        g = wave_sum(g);
+        if (lane == 0) Gv[p * BW + q] = g;
+      }
+      __syncthreads();
+      // T (dlarft, forward): T[0:q, q] = -tau_q T[0:q, 0:q] Gv[0:q, q]; T[q][q] = tau_q
+      if (wave == 0) {
+        for (int q = 0; q < bw; ++q) {
+          double tq = 0.0;
+          if (lane < q)
+            for (int r2 = lane; r2 < q; ++r2) tq += Tm[lane * BW + r2] * Gv[r2 * BW + q];
+          if (lane < q) Tm[lane * BW + q] = -tau[s0 + q] * tq;
+          if (lane == q) Tm[q * BW + q] = tau[s0 + q];
+          if (lane > q && lane < BW) Tm[lane * BW + q] = 0.0;
+        }
+      }
+      __syncthreads();
+      // W = V^T Z  (bw x n)
+      for (int t = tid; t < bw * n; t += EIGH_THREADS) {
+        const int q = t / n, r = t % n;
+        double acc = 0.0;
+        for (int i = s0 + q + 1; i < m; ++i) acc += Vp[q * m + i] * Zs[(size_t)r * m + i];
+        Wm[q * n + r] = acc;
+      }
+      __syncthreads();
+      // W = T W  (upper triangular, in place via registers)
+      double tw = 0.0;
+      int tq2 = -1, tr2 = -1;
+      if (tid < bw * n) {
+        tq2 = tid / n;
+        tr2 = tid % n;
+        for (int p = tq2; p < bw; ++p) tw += Tm[tq2 * BW + p] * Wm[p * n + tr2];
+      }
+      __syncthreads();
+      if (tid < bw * n) Wm[tq2 * n + tr2] = tw;
+      __syncthreads();
+      // Z -= V W
+      for (int t = tid; t < n * m; t += EIGH_THREADS) {
+        const int r = t / m, i = t % m;
+        double acc = 0.0;
+        for (int q = 0; q < bw; ++q) acc += Vp[q * m + i] * Wm[q * n + r];
+        Zs[t] -= acc;
+      }
+      __syncthreads();
+    }
+    for (int t = tid; t < n * m; t += EIGH_THREADS) Y[(size_t)(t / m) * ldy + (t % m)] = Zs[t];
+  } else if (nref > 0) {
+    // large problems: one wave per root, reflectors one at a time (global memory)
+    for (int r = wave; r < n; r += EIGH_NW) {
       double* y = Y + (size_t)r * ldy;
-      for (int i = 0; i < m; ++i) y[i] = z[i];
+      for (int k = m - 3; k >= 0; --k) {
+        const double tk = tau[k];
+        if (tk == 0.0) continue;
+        const double* vk = A + (int64_t)k * m;
+        double part2 = 0.0;
+        for (int i = k + 1 + lane; i < m; i += 64) part2 += ((i == k + 1) ? 1.0 : vk[i]) * y[i];
+        const double dt = wave_sum(part2) * tk;
+        for (int i = k + 1 + lane; i < m; i += 64) y[i] -= dt * ((i == k + 1) ? 1.0 : vk[i]);
+      }
     }
   }
   __syncthreads();
-
-  // 5. back-transformation y = H_0 H_1 ... H_{m-3} z.  Each wave applies the
-  //    reflectors to its roots (r = wave + 8t) held in LDS when they fit.
-  const bool y_lds = (size_t)n * m <= (size_t)EIGH_POOL;
-  double* yb = y_lds ? pool : Y;
-  const int ldb = y_lds ? m : ldy;
-  if (y_lds) {
-    for (int t = tid; t < n * m; t += EIGH_THREADS) pool[t] = Y[(size_t)(t / m) * ldy + (t % m)];
-    __syncthreads();
-  }
-  constexpr int RPW = 8;  // roots per wave per pass
-  for (int r0 = wave; r0 < n; r0 += EIGH_NW * RPW) {
-    int nr = 0;
-    int rs[RPW];
-#pragma unroll
-    for (int q = 0; q < RPW; ++q) {
-      const int r = r0 + q * EIGH_NW;
-      rs[q] = r;
-      if (r < n) nr = q + 1;
-    }
-    for (int k = m - 3; k >= 0; --k) {
-      const double tk = tau[k];
-      if (tk == 0.0) continue;
-      const double* vk = A + (int64_t)k * m;  // v[k+1] = 1 implicit, v[i>k+1] stored
-      double dts[RPW];
-#pragma unroll
-      for (int q = 0; q < RPW; ++q) dts[q] = 0.0;
-      for (int i = k + 1 + lane; i < m; i += 64) {
-        const double vi = (i == k + 1) ? 1.0 : vk[i];
-#pragma unroll
-        for (int q = 0; q < RPW; ++q)
-          if (q < nr) dts[q] += vi * yb[(size_t)rs[q] * ldb + i];
-      }
-#pragma unroll
-      for (int q = 0; q < RPW; ++q)
-        if (q < nr) dts[q] = wave_sum(dts[q]) * tk;
-      for (int i = k + 1 + lane; i < m; i += 64) {
-        const double vi = (i == k + 1) ? 1.0 : vk[i];
-#pragma unroll
-        for (int q = 0; q < RPW; ++q)
-          if (q < nr) yb[(size_t)rs[q] * ldb + i] -= dts[q] * vi;
-      }
-    }
-  }
-  if (y_lds) {
-    __syncthreads();
-    for (int t = tid; t < n * m; t += EIGH_THREADS) Y[(size_t)(t / m) * ldy + (t % m)] = pool[t];
-  }
+  EIGH_STAMP(5)
+#undef EIGH_STAMP
 }
 
 void launch_eigh(const double* H, int ldh, int m, int n, double* work, double* evals, double* Y, int ldy,
-                 hipStream_t s) {
+                 hipStream_t s, unsigned long long* phase_clk) {
   TDA_CHECK(m >= 1 && m <= EIGH_MAXM, 2, "eigh: subspace dimension out of range (1..1024)");
   TDA_CHECK(n >= 1 && n <= m && n <= 64, 2, "eigh: number of roots out of range");
   TDA_CHECK(ldy >= m, 2, "eigh: ldy < m");
-  eigh_kernel<<<1, EIGH_THREADS, 0, s>>>(H, ldh, m, n, work, evals, Y, ldy);
+  eigh_kernel<<<1, EIGH_THREADS, 0, s>>>(H, ldh, m, n, work, evals, Y, ldy, phase_clk);
   TDA_LAUNCH_CHECK();
 }
 

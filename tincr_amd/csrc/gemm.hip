@@ -15,9 +15,11 @@ GemmPlan gemm_plan(DType in, int64_t M, int64_t N, int64_t K, int batch, bool, b
   if (ksplit_hint > 0) {
     ks = ksplit_hint;
   } else {
-    // aim for >= ~2048 workgroups while keeping >= 4 K-tiles per split
+    // split K only when the tiles alone cannot fill the 256 CUs; then aim
+    // for >= ~1024 workgroups while keeping >= 8 K-tiles per split
     const int64_t kt = ceil_div(K, bk);
-    while (tiles * ks < 2048 && kt / (ks * 2) >= 4) ks *= 2;
+    if (tiles < 512)
+      while (tiles * ks < 1024 && kt / (ks * 2) >= 8) ks *= 2;
   }
   const int64_t kt = ceil_div(K, bk);
   p.k_chunk = ceil_div(kt, ks) * bk;

@@ -76,7 +76,9 @@ void launch_ao_add_coulomb(void* M, int dtype, int k, int n_ao, const int* atom,
 // H: m x m symmetric (upper triangle read), leading dim ldh; work >= eigh_work_doubles(m)
 // Outputs the n lowest eigenvalues (ascending) and eigenvectors Y[r][0..m) (row per root).
 size_t eigh_work_doubles(int m);
+// phase_clk (optional, device): wall-clock stamps at the phase boundaries
+// (copy, tridiagonalisation, bisection, inverse iteration, back-transform).
 void launch_eigh(const double* H, int ldh, int m, int n, double* work, double* evals, double* Y, int ldy,
-                 hipStream_t s);
+                 hipStream_t s, unsigned long long* phase_clk = nullptr);
 
 }  // namespace tda

@@ -97,10 +97,14 @@ struct tda_ctx {
   bool mo_built = false;  // q_oo / q_vv present
   // AO-factored operands: Rt [2n_ao][v_p], Lm [2n_ao][o_p], Rpt [v_p][2n_ao],
   // Lpt [o_p][2n_ao], Gt = AO-expanded gamma_lr [n_ao][n_ao]
-  DevBuf ao_atom, Gt, aoRt, aoLm, aoRpt, aoLpt, aoRt_lp, aoLm_lp, aoRpt_lp, aoLpt_lp, aoXR, aoM, aoF;
-  // low precision copies
-  int lp_prec = -1;
-  DevBuf qov32, qvv_lp, G_lp, qvv_amax, G_amax, stage1_scale;
+  DevBuf ao_atom, Gt, aoRt, aoLm, aoRpt, aoLpt, aoXR, aoM, aoF;
+  // Low-precision operand copies, one set per precision (index PREC_*): the
+  // precision ladder of tda_run uses the context's precision and FP32.
+  struct LpSet {
+    bool ready = false;
+    DevBuf qvv, G, Rt, Lm, Rpt, Lpt, scale;  // scale: [s_B*g_B | 1/s_B | 1/g_B] per atom
+  } lps[4];
+  DevBuf qov32, qvv_amax, G_amax;
   // workspaces
   DevBuf ws_gemm, T, xlp, u, uu, vv32, ax_tmp, stage, stage2;
   // Davidson state
@@ -124,11 +128,12 @@ struct tda_ctx {
 
   ~tda_ctx() {
     for (auto e : ev_pool) (void)hipEventDestroy(e);
-    for (DevBuf* b : {&C, &S, &SC, &orbe, &aooff, &gamma, &gamma_lr, &qov, &qoo, &qvv, &G, &de, &qov32, &qvv_lp,
-                      &G_lp, &qvv_amax, &G_amax, &stage1_scale, &ws_gemm, &T, &xlp, &u, &uu, &vv32, &ax_tmp,
+    for (auto& ls : lps)
+      for (DevBuf* b : {&ls.qvv, &ls.G, &ls.Rt, &ls.Lm, &ls.Rpt, &ls.Lpt, &ls.scale}) b->release();
+    for (DevBuf* b : {&C, &S, &SC, &orbe, &aooff, &gamma, &gamma_lr, &qov, &qoo, &qvv, &G, &de, &qov32,
+                      &qvv_amax, &G_amax, &ws_gemm, &T, &xlp, &u, &uu, &vv32, &ax_tmp,
                       &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &P, &W, &norms, &npart, &idx, &count,
-                      &gs_scratch, &eigw, &ao_atom, &Gt, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoRt_lp,
-                      &aoLm_lp, &aoRpt_lp, &aoLpt_lp, &aoXR, &aoM, &aoF})
+                      &gs_scratch, &eigw, &ao_atom, &Gt, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF})
       b->release();
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -214,41 +219,46 @@ void gemm_sc(tda_ctx* c, const double* dS, const double* dC, double* dSC, int na
   run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, g);
 }
 
-void prepare_lowprec(tda_ctx* c) {
-  if (c->prec == PREC_FP64 || c->lp_prec == c->prec) return;
+// Low-precision copies of the operator operands for precision `prec`.
+void prepare_lowprec(tda_ctx* c, int prec) {
+  if (prec == PREC_FP64) return;
+  tda_ctx::LpSet& ls = c->lps[prec];
+  if (ls.ready) return;
   const int N = c->N;
-  const size_t esz = prec_bytes(c->prec);
+  const size_t esz = prec_bytes(prec);
   if (c->has_lc && c->op_kind == 0) {
     const int64_t n2 = 2 * (int64_t)c->nao;
     struct Pair { DevBuf* src; DevBuf* dst; int64_t len; };
-    Pair pairs[4] = {{&c->aoRt, &c->aoRt_lp, n2 * c->vp}, {&c->aoLm, &c->aoLm_lp, n2 * c->op},
-                     {&c->aoRpt, &c->aoRpt_lp, n2 * c->vp}, {&c->aoLpt, &c->aoLpt_lp, n2 * c->op}};
+    Pair pairs[4] = {{&c->aoRt, &ls.Rt, n2 * c->vp}, {&c->aoLm, &ls.Lm, n2 * c->op},
+                     {&c->aoRpt, &ls.Rpt, n2 * c->vp}, {&c->aoLpt, &ls.Lpt, n2 * c->op}};
     for (Pair& pr : pairs) {
       pr.dst->ensure((size_t)pr.len * esz);
-      launch_convert(pr.src->as<double>(), pr.len, 1, nullptr, pr.dst->p, c->prec, c->stream);
+      launch_convert(pr.src->as<double>(), pr.len, 1, nullptr, pr.dst->p, prec, c->stream);
     }
     TDA_HIP(hipStreamSynchronize(c->stream));
-    c->lp_prec = c->prec;
+    ls.ready = true;
     return;
   }
   // Coulomb operand in FP32 for every mixed precision
-  c->qov32.ensure((size_t)N * c->ovp * 4);
+  if (!c->qov32.p || c->qov32.bytes < (size_t)N * c->ovp * 4) {
+    c->qov32.ensure((size_t)N * c->ovp * 4);
+  }
   launch_convert(c->qov.as<double>(), c->ovp, N, nullptr, c->qov32.p, PREC_FP32, c->stream);
   if (c->has_lc) {
     const int64_t lvv = (int64_t)c->vp * c->vp, loo = (int64_t)c->op * c->op;
-    c->qvv_lp.ensure((size_t)N * lvv * esz);
-    c->G_lp.ensure((size_t)N * loo * esz);
-    c->qvv_amax.ensure((size_t)N * 4);
-    c->G_amax.ensure((size_t)N * 4);
-    c->stage1_scale.ensure((size_t)N * 4 * 3);
-    float* inv_vv = c->stage1_scale.as<float>() + N;
-    float* inv_g = c->stage1_scale.as<float>() + 2 * N;
-    if (c->prec == PREC_FP32) {
-      launch_convert(c->qvv.as<double>(), lvv, N, nullptr, c->qvv_lp.p, PREC_FP32, c->stream);
-      launch_convert(c->G.as<double>(), loo, N, nullptr, c->G_lp.p, PREC_FP32, c->stream);
+    ls.qvv.ensure((size_t)N * lvv * esz);
+    ls.G.ensure((size_t)N * loo * esz);
+    ls.scale.ensure((size_t)N * 4 * 3);
+    float* inv_vv = ls.scale.as<float>() + N;
+    float* inv_g = ls.scale.as<float>() + 2 * N;
+    if (prec == PREC_FP32) {
+      launch_convert(c->qvv.as<double>(), lvv, N, nullptr, ls.qvv.p, PREC_FP32, c->stream);
+      launch_convert(c->G.as<double>(), loo, N, nullptr, ls.G.p, PREC_FP32, c->stream);
     } else {
       // per-atom scaling: stored q_vv^B / s_B and G^B / g_B; stage 1 output
       // is multiplied by s_B * g_B so that stage 3 needs no rescaling.
+      c->qvv_amax.ensure((size_t)N * 4);
+      c->G_amax.ensure((size_t)N * 4);
       launch_block_amax(c->qvv.as<double>(), lvv, N, c->qvv_amax.as<float>(), c->stream);
       launch_block_amax(c->G.as<double>(), loo, N, c->G_amax.as<float>(), c->stream);
       std::vector<float> svv(N), sg(N), sc(3 * (size_t)N);
@@ -262,13 +272,17 @@ void prepare_lowprec(tda_ctx* c) {
         sc[N + b] = 1.f / s1;
         sc[2 * N + b] = 1.f / s2;
       }
-      TDA_HIP(hipMemcpyAsync(c->stage1_scale.p, sc.data(), sc.size() * 4, hipMemcpyHostToDevice, c->stream));
-      launch_convert(c->qvv.as<double>(), lvv, N, inv_vv, c->qvv_lp.p, c->prec, c->stream);
-      launch_convert(c->G.as<double>(), loo, N, inv_g, c->G_lp.p, c->prec, c->stream);
-      TDA_HIP(hipStreamSynchronize(c->stream));
+      TDA_HIP(hipMemcpyAsync(ls.scale.p, sc.data(), sc.size() * 4, hipMemcpyHostToDevice, c->stream));
+      launch_convert(c->qvv.as<double>(), lvv, N, inv_vv, ls.qvv.p, prec, c->stream);
+      launch_convert(c->G.as<double>(), loo, N, inv_g, ls.G.p, prec, c->stream);
     }
   }
-  c->lp_prec = c->prec;
+  TDA_HIP(hipStreamSynchronize(c->stream));
+  ls.ready = true;
+}
+
+void reset_lowprec(tda_ctx* c) {
+  for (auto& ls : c->lps) ls.ready = false;
 }
 
 // q_oo, q_vv and G = gamma_lr q_oo (solvers/mod.rs:328-331, once per system)
@@ -312,7 +326,7 @@ void build_ao_operands(tda_ctx* c) {
 void prepare(tda_ctx* c) {
   TDA_CHECK(c->has_system, TDA_ERR_STATE, "tda_prepare: no system loaded");
   if (c->prepared) {
-    prepare_lowprec(c);
+    prepare_lowprec(c, c->prec);
     return;
   }
   const int N = c->N, nao = c->nao;
@@ -330,8 +344,8 @@ void prepare(tda_ctx* c) {
   }
   TDA_HIP(hipStreamSynchronize(c->stream));
   c->prepared = true;
-  c->lp_prec = -1;
-  prepare_lowprec(c);
+  reset_lowprec(c);
+  prepare_lowprec(c, c->prec);
 }
 
 // ---------------------------------------------------------------------------
@@ -383,9 +397,10 @@ void apply_A(tda_ctx* c, const double* X, int k, double* AX, int prec) {
     launch_cast_vectors(X, (int64_t)k * ovp, c->xlp.p, prec, c->stream);
   }
   if (lp) Xin = c->xlp.p;  // FP32 cast above serves PREC_FP32
-  const void* qvv = lp ? c->qvv_lp.p : c->qvv.p;
-  const void* G = lp ? c->G_lp.p : c->G.p;
-  const float* s1scale = (lp && prec != PREC_FP32) ? c->stage1_scale.as<float>() : nullptr;
+  if (lp) prepare_lowprec(c, prec);
+  const void* qvv = lp ? c->lps[prec].qvv.p : c->qvv.p;
+  const void* G = lp ? c->lps[prec].G.p : c->G.p;
+  const float* s1scale = (lp && prec != PREC_FP32) ? c->lps[prec].scale.as<float>() : nullptr;
   size_t per_atom = (size_t)k * vp * op * esz;
   size_t tb = std::max<size_t>(per_atom, std::min<size_t>(c->budget, (size_t)8 << 30));
   int NB = (int)std::max<size_t>(1, std::min<size_t>((size_t)N, tb / per_atom));
@@ -463,10 +478,11 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
     launch_cast_vectors(X, (int64_t)k * ovp, c->xlp.p, prec, c->stream);
     Xin = c->xlp.p;
   }
-  const void* Rt = lp ? c->aoRt_lp.p : c->aoRt.p;
-  const void* Lm = lp ? c->aoLm_lp.p : c->aoLm.p;
-  const void* Rpt = lp ? c->aoRpt_lp.p : c->aoRpt.p;
-  const void* Lpt = lp ? c->aoLpt_lp.p : c->aoLpt.p;
+  if (lp) prepare_lowprec(c, prec);
+  const void* Rt = lp ? c->lps[prec].Rt.p : c->aoRt.p;
+  const void* Lm = lp ? c->lps[prec].Lm.p : c->aoLm.p;
+  const void* Rpt = lp ? c->lps[prec].Rpt.p : c->aoRpt.p;
+  const void* Lpt = lp ? c->lps[prec].Lpt.p : c->aoLpt.p;
   c->aoXR.ensure((size_t)k * n2 * op * esz);
   c->aoM.ensure((size_t)k * n2 * n2 * esz);
   c->aoF.ensure((size_t)k * vp * n2 * esz);
@@ -600,13 +616,13 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       n_done = m;
       // 2. subspace eigenproblem (davidson.rs:93-97)
       launch_eigh(H, cap, m, n, c->eigw.as<double>(), c->evals.as<double>(), Y, cap, c->stream);
-      // 3.1 Ritz vectors RZ = Y V, 3.2 residuals R = Y AV - u RZ (davidson.rs:101-104)
+      // 3.1 Ritz vectors RZ = Y V, 3.2 residuals R = Y AV - u RZ, 3.3 norms
+      // (davidson.rs:101-107)
       GemmArgs gr = gargs(Y, V, RZ, n, L, m, plain(cap), plain(1), plain(L), plain(1), plain(L), plain(1));
       run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, gr);
       GemmArgs ga = gargs(Y, AV, R, n, L, m, plain(cap), plain(1), plain(L), plain(1), plain(L), plain(1));
       run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, ga);
       launch_residual(R, RZ, c->evals.as<double>(), n, L, c->stream);
-      // 3.3 norms (davidson.rs:107)
       launch_norms(R, n, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
       h_evals.resize(n);
       h_err.resize(n);
@@ -821,7 +837,7 @@ int tda_set_operator(tda_ctx* c, int op) {
   if (op != c->op_kind) {
     c->op_kind = op;
     c->prepared = false;
-    c->lp_prec = -1;
+    reset_lowprec(c);
   }
   return TDA_OK;
 }
@@ -878,7 +894,7 @@ int tda_set_system(tda_ctx* c, int n_atoms, int n_ao, const int32_t* opa, const 
     TDA_CHECK(orbs && s && orbe && gamma && xyz, TDA_ERR_BAD_ARG, "NULL input array");
     c->has_system = false;
     c->prepared = false;
-    c->lp_prec = -1;
+    reset_lowprec(c);
     c->N = n_atoms;
     c->nao = n_ao;
     c->ob = occ_begin;
@@ -933,7 +949,7 @@ int tda_prepare(tda_ctx* c) {
 int tda_clear_charges(tda_ctx* c) {
   if (!c) return TDA_ERR_BAD_ARG;
   c->prepared = false;
-  c->lp_prec = -1;
+  reset_lowprec(c);
   return TDA_OK;
 }
 
@@ -1035,11 +1051,26 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       res = davidson(c, n, n, max_iter, tol, PREC_FP64, false, 0, h_evals, h_err);
       res64 = res;
     } else {
-      // low-precision phase, then FP64 continuation from the Ritz vectors
-      res = davidson(c, n, n, max_iter, tol, c->prec, false, 3, h_evals, h_err);
-      launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
-      const int left = std::max(1, max_iter - res.iters);
-      res64 = davidson(c, n, n, left, tol, PREC_FP64, true, 0, h_evals, h_err);
+      // Precision ladder.  FP16/BF16 products stop at their accuracy floor
+      // (stagnation); FP32 products then drive the residuals to `tol`; FP64
+      // products of the n Ritz vectors verify (and, if needed, continue).
+      // Each phase restarts from the previous phase's Ritz vectors.
+      int used = 0;
+      bool done = false;
+      if (c->prec == PREC_FP16 || c->prec == PREC_BF16) {
+        res = davidson(c, n, n, max_iter, tol, c->prec, false, 3, h_evals, h_err);
+        used += res.iters;
+        done = res.converged;
+        launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
+      }
+      if (!done) {
+        DavResult r32 = davidson(c, n, n, std::max(1, max_iter - used), tol, PREC_FP32, used > 0, 8, h_evals, h_err);
+        used += r32.iters;
+        res.iters = used;
+        launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
+      }
+      res64 = davidson(c, n, n, std::max(1, max_iter - used), tol, PREC_FP64, true, 0, h_evals, h_err);
+      res.iters = used;
     }
     // outputs (tda/mod.rs:88-126)
     const int N = c->N;
