@@ -96,6 +96,30 @@ int tda_set_system(tda_ctx* ctx, int n_atoms, int n_ao, const int32_t* orbs_per_
                    const double* s, const double* orbe, int occ_begin, int n_occ, int virt_begin, int n_virt,
                    const double* gamma, const double* gamma_lr, const double* xyz);
 
+/* Loads a charge-transfer pair H -> L, replacing
+ * PairChargeTransfer::prepare_ct_tda (src/excited_states/tda/ct_pair.rs:12-76)
+ * and selecting its DavidsonEngine (solvers/mod.rs:165-274): holes in the
+ * occupied MOs of monomer H, particles in the virtual MOs of monomer L.
+ *   H: n_atoms_h, n_ao_h, orbs_per_atom_h, orbs_h, s_h [n_ao_h^2], orbe_h,
+ *      occupied range occ_begin_h..+n_occ_h;
+ *   L: n_atoms_l, n_ao_l, orbs_per_atom_l, orbs_l, s_l [n_ao_l^2], orbe_l,
+ *      virtual range virt_begin_l..+n_virt_l;
+ *   s_hl [n_ao_h, n_ao_l]: AO overlap block (rows H, columns L);
+ *   gamma_h [n_atoms_h^2], gamma_l [n_atoms_l^2], gamma_hl [n_atoms_h,
+ *      n_atoms_l]: blocks of gamma the pair gamma is assembled from (:20-27);
+ *   gamma_lr_hl [n_atoms_h, n_atoms_l] (NULL => no exchange term);
+ *   xyz [(n_atoms_h + n_atoms_l), 3], H atoms first (pair atom order, :148).
+ * q_ov spans the H and L atoms (:78-123), q_oo is H's, q_vv is L's, and the
+ * exchange contracts gamma_lr_hl^T with q_oo (solvers/mod.rs:219-222).  The
+ * pair always uses the per-atom-charges operator (TDA_OP_CHARGES); the
+ * context's operator setting applies again after the next tda_set_system.
+ * tda_get_charges then returns q_oo over the H atoms and q_vv over the L atoms. */
+int tda_set_ct_pair(tda_ctx* ctx, int n_atoms_h, int n_ao_h, const int32_t* orbs_per_atom_h, const double* orbs_h,
+                    const double* s_h, const double* orbe_h, int occ_begin_h, int n_occ_h, int n_atoms_l, int n_ao_l,
+                    const int32_t* orbs_per_atom_l, const double* orbs_l, const double* s_l, const double* orbe_l,
+                    int virt_begin_l, int n_virt_l, const double* s_hl, const double* gamma_h,
+                    const double* gamma_l, const double* gamma_hl, const double* gamma_lr_hl, const double* xyz);
+
 /* prepare_tda: builds q_ov/q_oo/q_vv and omega on the GPU (and
  * gamma_lr.q_oo, once per system instead of once per product call as
  * solvers/mod.rs:328-331 does). */

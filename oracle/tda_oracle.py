@@ -19,6 +19,8 @@ It restates, in FP64 numpy, the reference's algorithm (tincr, Rust):
   ``mulliken_dipoles`` / ``oscillator_strength`` tda/moments.rs:7-24
 * ``tda_full_matrix``      src/excited_states/tda/mod.rs:133-215 (dense A)
 * ``ct_q_ov``              src/excited_states/tda/ct_pair.rs:78-123
+* ``ct_pair_engine``       src/excited_states/tda/ct_pair.rs:12-76 +
+  solvers/mod.rs:165-274
 
 Pinning (each checked in tests/test_oracle.py):
 
@@ -33,6 +35,12 @@ Pinning (each checked in tests/test_oracle.py):
   monopole zero for the water n->pi* states.
 * ct_q_ov: reduces to trans_charges' q_ov when hole and particle fragment
   coincide (exactly the a = b case of transition_charges.rs:58).
+* ct_pair_engine: equals the monomer engine of the supermolecule whose MO
+  coefficients are block-diagonal diag(C_H, C_L) with the full overlap, the
+  occupied MOs of H and the virtual MOs of L: then the Mulliken charges of
+  transition_charges.rs vanish on the other fragment's atoms and reduce to the
+  CT formulas, so the pair A must equal that monomer A (n_atoms_h != n_atoms_l,
+  so a transposed gamma_lr block cannot pass).
 """
 import numpy as np
 
@@ -130,6 +138,30 @@ def ct_q_ov(orbs_h, orbs_l, s_hl, orbs_per_atom_h, orbs_per_atom_l, homo, lumo):
         mu = slice(off[n], off[n + 1])
         q[nh + n] = s_c_h[mu].T @ virts[mu]
     return (0.5 * q).reshape(nh + nl, o * v)
+
+
+def ct_pair_engine(h, l, s_hl, gamma_hl, gamma_lr_hl, faithful=False):
+    """PairChargeTransfer::prepare_ct_tda (ct_pair.rs:12-76) with its
+    DavidsonEngine (solvers/mod.rs:165-274).  h / l: dicts of the monomers'
+    n_atoms, orbs_per_atom, occ_indices, virt_indices, orbs, s, orbe, gamma.
+    Pair gamma from the monomer blocks and gamma_hl (:20-27); omega =
+    orbe_l[lumo:] - orbe_h[:homo+1] (:43-58); q_ov over H u L (:78-123); q_oo
+    of H and q_vv of L (:70-72); exchange with gamma_lr_hl^T (mod.rs:219-222)."""
+    nh, nl = h["n_atoms"], l["n_atoms"]
+    gamma = np.zeros((nh + nl, nh + nl))
+    gamma[:nh, :nh] = h["gamma"]
+    gamma[nh:, nh:] = l["gamma"]
+    gamma[:nh, nh:] = gamma_hl
+    gamma[nh:, :nh] = np.asarray(gamma_hl).T
+    homo, lumo = h["occ_indices"][-1], l["virt_indices"][0]
+    orbe_h, orbe_l = np.ravel(h["orbe"]), np.ravel(l["orbe"])
+    omega = orbe_differences(orbe_h[:homo + 1], orbe_l[lumo:])
+    q_ov = ct_q_ov(h["orbs"], l["orbs"], s_hl, h["orbs_per_atom"], l["orbs_per_atom"], homo, lumo)
+    q_oo = trans_charges(nh, h["orbs_per_atom"], h["orbs"], h["s"], h["occ_indices"], h["virt_indices"])[1]
+    q_vv = trans_charges(nl, l["orbs_per_atom"], l["orbs"], l["s"], l["occ_indices"], l["virt_indices"])[2]
+    o, v = homo + 1, len(orbe_l) - lumo
+    return SystemEngine(q_ov, q_oo, q_vv, gamma, gamma_lr_hl, omega, o, v, faithful=faithful,
+                        gamma_lr_transposed=True)
 
 
 # ---------------------------------------------------------------------------

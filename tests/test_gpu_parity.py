@@ -252,3 +252,130 @@ def test_ao_small_workspace_chunks():
     s = _system(m, hbm_budget_bytes=3 * 1024 * 1024)
     assert np.abs(s.compute_products(x) - eng.apply(x)).max() < 1e-12
     s.close()
+
+
+# ---------------------------------------------------------------------------
+# charge-transfer pairs (tda/ct_pair.rs, solvers/mod.rs:165-274)
+# ---------------------------------------------------------------------------
+_FRAG_KEYS = ("n_atoms", "orbs_per_atom", "xyz", "occ_indices", "virt_indices", "orbs", "s", "orbe", "gamma")
+
+
+def _ct_case(n_waters, n_h, seed=0):
+    from tincr_amd import systems as sy
+    return sy.ct_pair_inputs(sy.water_cluster(n_waters, seed=seed), 3 * n_h, seed=seed)
+
+
+def _ct_pair(ct, precision="fp64", **kw):
+    tda = _tda()
+    h = tda.Fragment(**{k: ct["h"][k] for k in _FRAG_KEYS})
+    l = tda.Fragment(**{k: ct["l"][k] for k in _FRAG_KEYS})
+    return tda.TdaCTPair(h, l, ct["s_hl"], ct["gamma_hl"], ct["gamma_lr_hl"], precision=precision, **kw)
+
+
+def _ct_oracle(ct):
+    return orc.ct_pair_engine(ct["h"], ct["l"], ct["s_hl"], ct["gamma_hl"], ct["gamma_lr_hl"])
+
+
+def _ct_xyz(ct):
+    return np.concatenate([ct["h"]["xyz"], ct["l"]["xyz"]])
+
+
+@pytest.mark.parametrize("case", [(3, 1), (4, 3), (12, 5)])
+def test_ct_pair_products(case):
+    """Pair charges (q_ov over H u L, q_oo of H, q_vv of L), omega and A.x
+    against the oracle's CT engine; N_H != N_L in every case."""
+    print(f"ct products {case}", flush=True)
+    ct = _ct_case(*case)
+    p = _ct_pair(ct)
+    eng = _ct_oracle(ct)
+    p.prepare_ct_tda()
+    assert np.abs(p.properties["q_ov"] - eng.q_ov).max() < 1e-14
+    assert np.abs(p.properties["q_oo"] - eng.q_oo).max() < 1e-14
+    assert np.abs(p.properties["q_vv"] - eng.q_vv).max() < 1e-14
+    assert np.array_equal(p.properties["omega"], eng.omega)
+    x = np.random.default_rng(21).standard_normal((eng.get_size(), 6))
+    ref = eng.apply(x)
+    assert np.abs(p.compute_products(x) - ref).max() < 1e-12 * max(np.abs(ref).max(), 1.0)
+    r = np.random.default_rng(22).standard_normal(eng.get_size())
+    assert np.abs(p.precondition(r, eng.omega[2]) - eng.precondition(r, eng.omega[2])).max() == 0.0
+    p.close()
+
+
+def test_ct_pair_run_vs_oracle():
+    """run_ct_tda (ct_pair.rs:125-180) against the oracle Davidson on the same
+    engine: same iteration count, omega, X, q_trans, f."""
+    ct = _ct_case(12, 5)
+    p = _ct_pair(ct)
+    n = 6
+    w, x = p.run_ct_tda(n, 100, 1e-6)
+    ref = orc.run_tda(_ct_oracle(ct), _ct_xyz(ct), n, 100, 1e-6)
+    assert np.abs(w - ref["omega"]).max() * HARTREE_TO_EV < 1e-6
+    for s in range(n):
+        assert abs(abs(x[:, s] @ ref["X"][:, s]) - 1.0) < 1e-5
+    assert p.last_stats["n_iter"] == ref["n_iter"]
+    assert np.abs(p.properties["oscillator_strengths"] - ref["f"]).max() < 1e-8
+    dense = orc.tda_full_matrix(_ct_oracle(ct), _ct_xyz(ct))
+    assert np.abs(w - dense["omega"][:n]).max() < 1e-9
+    p.close()
+
+
+def test_ct_pair_equals_supermolecule_on_gpu():
+    """The GPU CT pair equals the GPU monomer path on the supermolecule with
+    block-diagonal orbitals diag(C_H, C_L) (both operator forms)."""
+    ct = _ct_case(4, 3)
+    h, l = ct["h"], ct["l"]
+    nao_h, nao_l = h["orbs"].shape[0], l["orbs"].shape[0]
+    c = np.zeros((nao_h + nao_l, nao_h + nao_l))
+    c[:nao_h, :nao_h] = h["orbs"]
+    c[nao_h:, nao_h:] = l["orbs"]
+    sup = dict(n_atoms=h["n_atoms"] + l["n_atoms"],
+               orbs_per_atom=np.concatenate([h["orbs_per_atom"], l["orbs_per_atom"]]), xyz=_ct_xyz(ct),
+               occ_indices=list(h["occ_indices"]), virt_indices=[nao_h + a for a in l["virt_indices"]],
+               orbs=c, s=ct["s"], orbe=np.concatenate([h["orbe"], l["orbe"]]), gamma=ct["gamma"],
+               gamma_lr=ct["gamma_lr"])
+    p = _ct_pair(ct)
+    wp, _ = p.run_ct_tda(4, 100, 1e-7)
+    for op in ("ao", "charges"):
+        m = _system(sup, operator=op)
+        wm, _ = m.run_tda(4, 100, 1e-7)
+        assert np.abs(wp - wm).max() < 1e-10, op
+        m.close()
+    p.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+def test_ct_pair_mixed_precision(prec):
+    ct = _ct_case(12, 5)
+    p64 = _ct_pair(ct)
+    w64, _ = p64.run_ct_tda(4, 200, 1e-5)
+    plp = _ct_pair(ct, precision=prec)
+    w, _ = plp.run_ct_tda(4, 200, 1e-5)
+    assert np.abs(w - w64).max() * HARTREE_TO_EV < 1e-3
+    assert plp.last_stats["converged"] == 1 and plp.last_stats["max_residual"] < 1e-5
+    p64.close()
+    plp.close()
+
+
+def test_ct_pair_atom_blocking_and_context_reuse():
+    """A small HBM budget splits the pair exchange over L-atom blocks; the same
+    context then loads a monomer and runs it with the AO operator again."""
+    from tincr_amd import _native as nat
+    ct = _ct_case(12, 5)
+    eng = _ct_oracle(ct)
+    x = np.random.default_rng(23).standard_normal((eng.get_size(), 4))
+    p = _ct_pair(ct, hbm_budget_bytes=4 * 64 * 64 * 8 * 3)
+    assert np.abs(p.compute_products(x) - eng.apply(x)).max() < 1e-12
+    m = molecule("h2o")
+    ctx = p.ctx
+    opa = np.ascontiguousarray(m["orbs_per_atom"], dtype=np.int32)
+    orbs, s_, orbe = nat.f64(m["orbs"]), nat.f64(m["s"]), nat.f64(np.ravel(m["orbe"]))
+    g, glr, xyz = nat.f64(m["gamma"]), nat.f64(m["gamma_lr"]), nat.f64(np.asarray(m["xyz"]).reshape(-1, 3))
+    o, v = len(m["occ_indices"]), len(m["virt_indices"])
+    ctx.check(ctx.lib.tda_set_system(ctx.handle, m["n_atoms"], orbs.shape[0], nat.iptr(opa), nat.dptr(orbs),
+                                     nat.dptr(s_), nat.dptr(orbe), m["occ_indices"][0], o, m["virt_indices"][0], v,
+                                     nat.dptr(g), nat.dptr(glr), nat.dptr(xyz)))
+    xm = np.random.default_rng(24).standard_normal((o * v, 3))
+    axm = np.empty_like(xm)
+    ctx.check(ctx.lib.tda_compute_products(ctx.handle, nat.dptr(xm), 3, nat.dptr(axm)))
+    assert np.abs(axm - _oracle_engine(m).apply(xm)).max() < 1e-12
+    p.close()

@@ -189,3 +189,61 @@ def test_ct_q_ov_reduces_to_monomer():
                              m["occ_indices"], m["virt_indices"])[0]
     n = m["n_atoms"]
     assert np.abs(q[:n] + q[n:] - q_ov).max() < 1e-15
+
+
+def _ct_inputs(n_waters=3, n_h=1, seed=0):
+    from tincr_amd import systems as sy
+    mol = sy.water_cluster(n_waters, seed=seed)
+    return sy.ct_pair_inputs(mol, 3 * n_h, seed=seed)
+
+
+def _dimer_engine(ct):
+    """Monomer engine of the supermolecule with C = diag(C_H, C_L), the full
+    overlap, H's occupied and L's virtual MOs."""
+    h, l = ct["h"], ct["l"]
+    nao_h, nao_l = h["orbs"].shape[0], l["orbs"].shape[0]
+    c = np.zeros((nao_h + nao_l, nao_h + nao_l))
+    c[:nao_h, :nao_h] = h["orbs"]
+    c[nao_h:, nao_h:] = l["orbs"]
+    orbe = np.concatenate([h["orbe"], l["orbe"]])
+    occ = list(h["occ_indices"])
+    virt = [nao_h + a for a in l["virt_indices"]]
+    opa = np.concatenate([h["orbs_per_atom"], l["orbs_per_atom"]])
+    n = h["n_atoms"] + l["n_atoms"]
+    q_ov, q_oo, q_vv = orc.trans_charges(n, opa, c, ct["s"], occ, virt)
+    omega = orc.orbe_differences(orbe[occ], orbe[virt])
+    return orc.SystemEngine(q_ov, q_oo, q_vv, ct["gamma"], ct["gamma_lr"], omega, len(occ), len(virt),
+                            faithful=False), dict(n_atoms=n, orbs_per_atom=opa, orbs=c, orbe=orbe,
+                                                  occ_indices=occ, virt_indices=virt)
+
+
+def test_ct_pair_engine_equals_block_diagonal_supermolecule():
+    """ct_pair.rs:12-123 + solvers/mod.rs:165-274 against the monomer path of
+    the supermolecule diag(C_H, C_L) (transition_charges.rs:8-98 reduces to the
+    CT charges there): dense A and A.x agree; one water (3 atoms) -> two
+    waters (6 atoms), so a transposed gamma_lr block would not even contract."""
+    ct = _ct_inputs()
+    eng = orc.ct_pair_engine(ct["h"], ct["l"], ct["s_hl"], ct["gamma_hl"], ct["gamma_lr_hl"])
+    ref, _ = _dimer_engine(ct)
+    assert eng.get_size() == ref.get_size()
+    a, b = eng.dense(), ref.dense()
+    assert np.abs(a - b).max() < 1e-13 * np.abs(b).max()
+    assert np.abs(a - a.T).max() < 1e-13
+    x = np.random.default_rng(1).standard_normal((eng.get_size(), 3))
+    assert np.abs(eng.apply(x) - ref.apply(x)).max() < 1e-12
+    faithful = orc.ct_pair_engine(ct["h"], ct["l"], ct["s_hl"], ct["gamma_hl"], ct["gamma_lr_hl"], faithful=True)
+    assert np.abs(faithful.apply(x) - ref.apply(x)).max() < 1e-12
+    # the q_vv atoms are L's and the q_oo atoms H's (ct_pair.rs:70-72)
+    assert eng.q_oo.shape[0] == 3 and eng.q_vv.shape[0] == 6
+    # a planted mistake (a stray factor on the H-L overlap block) is caught
+    bad = orc.ct_pair_engine(ct["h"], ct["l"], 0.5 * ct["s_hl"], ct["gamma_hl"], ct["gamma_lr_hl"])
+    assert np.abs(bad.dense() - b).max() > 1e-6
+
+
+def test_ct_pair_davidson_vs_dense():
+    ct = _ct_inputs()
+    eng = orc.ct_pair_engine(ct["h"], ct["l"], ct["s_hl"], ct["gamma_hl"], ct["gamma_lr_hl"])
+    xyz = np.concatenate([ct["h"]["xyz"], ct["l"]["xyz"]])
+    res = orc.run_tda(eng, xyz, 4, 100, 1e-7)
+    dense = orc.tda_full_matrix(eng, xyz)
+    assert np.abs(res["omega"] - dense["omega"][:4]).max() < 1e-10

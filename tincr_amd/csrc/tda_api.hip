@@ -87,6 +87,13 @@ struct tda_ctx {
   // system
   bool has_system = false, has_lc = false, prepared = false;
   int N = 0, nao = 0, ob = 0, o = 0, vb = 0, v = 0, op = 0, vp = 0;
+  // atoms carrying q_oo (Nq) and q_vv (Nx); both N for a System/Monomer.  A
+  // charge-transfer pair (ct_pair.rs) has q_oo on the hole fragment H (Nq =
+  // N_H) and q_vv on the particle fragment L (Nx = N_L).
+  int Nq = 0, Nx = 0;
+  bool ct = false;
+  int nao_l = 0, lumo_l = 0;
+  DevBuf Cl, Sl, SCl, Shl, aooff_l, ctW, ctZ;
   int64_t ovp = 0;
   std::vector<double> xyz, h_de;
   DevBuf C, S, SC, orbe, aooff, gamma, gamma_lr;
@@ -133,7 +140,8 @@ struct tda_ctx {
     for (DevBuf* b : {&C, &S, &SC, &orbe, &aooff, &gamma, &gamma_lr, &qov, &qoo, &qvv, &G, &de, &qov32,
                       &qvv_amax, &G_amax, &ws_gemm, &T, &xlp, &u, &uu, &vv32, &ax_tmp,
                       &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &P, &W, &norms, &npart, &idx, &count,
-                      &gs_scratch, &eigw, &ao_atom, &Gt, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF})
+                      &gs_scratch, &eigw, &ao_atom, &Gt, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
+                      &Cl, &Sl, &SCl, &Shl, &aooff_l, &ctW, &ctZ})
       b->release();
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -226,7 +234,7 @@ void prepare_lowprec(tda_ctx* c, int prec) {
   if (ls.ready) return;
   const int N = c->N;
   const size_t esz = prec_bytes(prec);
-  if (c->has_lc && c->op_kind == 0) {
+  if (c->has_lc && c->op_kind == 0 && !c->ct) {
     const int64_t n2 = 2 * (int64_t)c->nao;
     struct Pair { DevBuf* src; DevBuf* dst; int64_t len; };
     Pair pairs[4] = {{&c->aoRt, &ls.Rt, n2 * c->vp}, {&c->aoLm, &ls.Lm, n2 * c->op},
@@ -246,35 +254,36 @@ void prepare_lowprec(tda_ctx* c, int prec) {
   launch_convert(c->qov.as<double>(), c->ovp, N, nullptr, c->qov32.p, PREC_FP32, c->stream);
   if (c->has_lc) {
     const int64_t lvv = (int64_t)c->vp * c->vp, loo = (int64_t)c->op * c->op;
-    ls.qvv.ensure((size_t)N * lvv * esz);
-    ls.G.ensure((size_t)N * loo * esz);
-    ls.scale.ensure((size_t)N * 4 * 3);
-    float* inv_vv = ls.scale.as<float>() + N;
-    float* inv_g = ls.scale.as<float>() + 2 * N;
+    const int NX = c->Nx;
+    ls.qvv.ensure((size_t)NX * lvv * esz);
+    ls.G.ensure((size_t)NX * loo * esz);
+    ls.scale.ensure((size_t)NX * 4 * 3);
+    float* inv_vv = ls.scale.as<float>() + NX;
+    float* inv_g = ls.scale.as<float>() + 2 * NX;
     if (prec == PREC_FP32) {
-      launch_convert(c->qvv.as<double>(), lvv, N, nullptr, ls.qvv.p, PREC_FP32, c->stream);
-      launch_convert(c->G.as<double>(), loo, N, nullptr, ls.G.p, PREC_FP32, c->stream);
+      launch_convert(c->qvv.as<double>(), lvv, NX, nullptr, ls.qvv.p, PREC_FP32, c->stream);
+      launch_convert(c->G.as<double>(), loo, NX, nullptr, ls.G.p, PREC_FP32, c->stream);
     } else {
       // per-atom scaling: stored q_vv^B / s_B and G^B / g_B; stage 1 output
       // is multiplied by s_B * g_B so that stage 3 needs no rescaling.
-      c->qvv_amax.ensure((size_t)N * 4);
-      c->G_amax.ensure((size_t)N * 4);
-      launch_block_amax(c->qvv.as<double>(), lvv, N, c->qvv_amax.as<float>(), c->stream);
-      launch_block_amax(c->G.as<double>(), loo, N, c->G_amax.as<float>(), c->stream);
-      std::vector<float> svv(N), sg(N), sc(3 * (size_t)N);
-      TDA_HIP(hipMemcpyAsync(svv.data(), c->qvv_amax.p, N * 4, hipMemcpyDeviceToHost, c->stream));
-      TDA_HIP(hipMemcpyAsync(sg.data(), c->G_amax.p, N * 4, hipMemcpyDeviceToHost, c->stream));
+      c->qvv_amax.ensure((size_t)NX * 4);
+      c->G_amax.ensure((size_t)NX * 4);
+      launch_block_amax(c->qvv.as<double>(), lvv, NX, c->qvv_amax.as<float>(), c->stream);
+      launch_block_amax(c->G.as<double>(), loo, NX, c->G_amax.as<float>(), c->stream);
+      std::vector<float> svv(NX), sg(NX), sc(3 * (size_t)NX);
+      TDA_HIP(hipMemcpyAsync(svv.data(), c->qvv_amax.p, NX * 4, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipMemcpyAsync(sg.data(), c->G_amax.p, NX * 4, hipMemcpyDeviceToHost, c->stream));
       TDA_HIP(hipStreamSynchronize(c->stream));
-      for (int b = 0; b < N; ++b) {
+      for (int b = 0; b < NX; ++b) {
         const float s1 = svv[b] > 0.f ? svv[b] : 1.f;
         const float s2 = sg[b] > 0.f ? sg[b] : 1.f;
         sc[b] = s1 * s2;
-        sc[N + b] = 1.f / s1;
-        sc[2 * N + b] = 1.f / s2;
+        sc[NX + b] = 1.f / s1;
+        sc[2 * NX + b] = 1.f / s2;
       }
       TDA_HIP(hipMemcpyAsync(ls.scale.p, sc.data(), sc.size() * 4, hipMemcpyHostToDevice, c->stream));
-      launch_convert(c->qvv.as<double>(), lvv, N, inv_vv, ls.qvv.p, prec, c->stream);
-      launch_convert(c->G.as<double>(), loo, N, inv_g, ls.G.p, prec, c->stream);
+      launch_convert(c->qvv.as<double>(), lvv, NX, inv_vv, ls.qvv.p, prec, c->stream);
+      launch_convert(c->G.as<double>(), loo, NX, inv_g, ls.G.p, prec, c->stream);
     }
   }
   TDA_HIP(hipStreamSynchronize(c->stream));
@@ -285,23 +294,53 @@ void reset_lowprec(tda_ctx* c) {
   for (auto& ls : c->lps) ls.ready = false;
 }
 
-// q_oo, q_vv and G = gamma_lr q_oo (solvers/mod.rs:328-331, once per system)
+// q_oo, q_vv and G = gamma_lr q_oo (solvers/mod.rs:328-331, once per system;
+// a CT pair contracts gamma_lr^T with the hole fragment's q_oo, :219-222)
 void build_mo_charges(tda_ctx* c) {
   if (c->mo_built) return;
-  const int N = c->N;
-  c->qoo.ensure((size_t)N * c->op * c->op * 8);
-  c->qvv.ensure((size_t)N * c->vp * c->vp * 8);
-  build_charges(c, c->C.as<double>(), c->SC.as<double>(), c->aooff.as<int>(), N, c->nao, c->ob, c->o, c->vb, c->v,
-                c->op, c->vp, nullptr, c->qoo.as<double>(), c->qvv.as<double>());
+  const int NQ = c->Nq, NX = c->Nx;
+  c->qoo.ensure((size_t)NQ * c->op * c->op * 8);
+  c->qvv.ensure((size_t)NX * c->vp * c->vp * 8);
+  if (!c->ct) {
+    build_charges(c, c->C.as<double>(), c->SC.as<double>(), c->aooff.as<int>(), NQ, c->nao, c->ob, c->o, c->vb,
+                  c->v, c->op, c->vp, nullptr, c->qoo.as<double>(), c->qvv.as<double>());
+  } else {
+    build_charges(c, c->C.as<double>(), c->SC.as<double>(), c->aooff.as<int>(), NQ, c->nao, c->ob, c->o, 0, 0,
+                  c->op, c->vp, nullptr, c->qoo.as<double>(), nullptr);
+    build_charges(c, c->Cl.as<double>(), c->SCl.as<double>(), c->aooff_l.as<int>(), NX, c->nao_l, 0, 0,
+                  c->lumo_l, c->v, c->op, c->vp, nullptr, nullptr, c->qvv.as<double>());
+  }
   if (c->has_lc) {
     const int64_t loo = (int64_t)c->op * c->op;
-    c->G.ensure((size_t)N * loo * 8);
-    GemmArgs g = gargs(c->gamma_lr.p, c->qoo.p, c->G.p, N, loo, N, plain(N), plain(1), plain(loo), plain(1),
+    c->G.ensure((size_t)NX * loo * 8);
+    GemmArgs g = gargs(c->gamma_lr.p, c->qoo.p, c->G.p, NX, loo, NQ, plain(NQ), plain(1), plain(loo), plain(1),
                        plain(loo), plain(1));
     run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, g);
   }
   TDA_HIP(hipStreamSynchronize(c->stream));
   c->mo_built = true;
+}
+
+// CT-pair q_ov (ct_pair.rs:78-123):
+//   A in H: q[A][i][a] = 1/2 sum_{mu in A} C^H_{mu,i} (S_HL C^L_v)_{mu,a}
+//   A in L: q[A][i][a] = 1/2 sum_{mu in A} (S_HL^T C^H_o)_{mu,i} C^L_{mu,a}
+void build_ct_qov(tda_ctx* c) {
+  const int nh = c->nao, nl = c->nao_l, o = c->o, v = c->v, op = c->op, vp = c->vp;
+  c->ctW.ensure((size_t)nh * vp * 8);
+  c->ctZ.ensure((size_t)nl * op * 8);
+  // W = S_HL C^L[:, lumo:]   [nh][vp]
+  GemmArgs gw = gargs(c->Shl.p, c->Cl.as<double>() + c->lumo_l, c->ctW.p, nh, v, nl, plain(nl), plain(1), plain(nl),
+                      plain(1), plain(vp), plain(1));
+  run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, gw);
+  // Z = S_HL^T C^H[:, :o]    [nl][op]
+  GemmArgs gz = gargs(c->Shl.p, c->C.as<double>() + c->ob, c->ctZ.p, nl, o, nh, plain(1), plain(nl), plain(nh),
+                      plain(1), plain(op), plain(1));
+  run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, gz);
+  const int NH = c->Nq, NL = c->Nx;
+  launch_qprod(c->C.as<double>() + c->ob, nh, c->ctW.as<double>(), vp, c->aooff.as<int>(), 0, NH, o, o, v, vp,
+               c->qov.as<double>(), c->stream);
+  launch_qprod(c->ctZ.as<double>(), op, c->Cl.as<double>() + c->lumo_l, nl, c->aooff_l.as<int>(), NH, NL, o, o, v,
+               vp, c->qov.as<double>(), c->stream);
 }
 
 void build_ao_operands(tda_ctx* c) {
@@ -333,13 +372,19 @@ void prepare(tda_ctx* c) {
   c->SC.ensure((size_t)nao * nao * 8);
   gemm_sc(c, c->S.as<double>(), c->C.as<double>(), c->SC.as<double>(), nao);
   c->qov.ensure((size_t)N * c->ovp * 8);
-  build_charges(c, c->C.as<double>(), c->SC.as<double>(), c->aooff.as<int>(), N, nao, c->ob, c->o, c->vb, c->v,
-                c->op, c->vp, c->qov.as<double>(), nullptr, nullptr);
+  if (!c->ct) {
+    build_charges(c, c->C.as<double>(), c->SC.as<double>(), c->aooff.as<int>(), N, nao, c->ob, c->o, c->vb, c->v,
+                  c->op, c->vp, c->qov.as<double>(), nullptr, nullptr);
+  } else {
+    c->SCl.ensure((size_t)c->nao_l * c->nao_l * 8);
+    gemm_sc(c, c->Sl.as<double>(), c->Cl.as<double>(), c->SCl.as<double>(), c->nao_l);
+    build_ct_qov(c);
+  }
   c->de.ensure((size_t)c->ovp * 8);
   launch_delta_eps(c->orbe.as<double>(), c->ob, c->o, c->vb, c->v, c->vp, c->de.as<double>(), c->stream);
   c->mo_built = false;
   if (c->has_lc) {
-    if (c->op_kind == 1) build_mo_charges(c);
+    if (c->op_kind == 1 || c->ct) build_mo_charges(c);
     else build_ao_operands(c);
   }
   TDA_HIP(hipStreamSynchronize(c->stream));
@@ -401,13 +446,14 @@ void apply_A(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   const void* qvv = lp ? c->lps[prec].qvv.p : c->qvv.p;
   const void* G = lp ? c->lps[prec].G.p : c->G.p;
   const float* s1scale = (lp && prec != PREC_FP32) ? c->lps[prec].scale.as<float>() : nullptr;
+  const int NX = c->Nx;
   size_t per_atom = (size_t)k * vp * op * esz;
   size_t tb = std::max<size_t>(per_atom, std::min<size_t>(c->budget, (size_t)8 << 30));
-  int NB = (int)std::max<size_t>(1, std::min<size_t>((size_t)N, tb / per_atom));
-  c->n_blocks = (int)ceil_div(N, NB);
+  int NB = (int)std::max<size_t>(1, std::min<size_t>((size_t)NX, tb / per_atom));
+  c->n_blocks = (int)ceil_div(NX, NB);
   c->T.ensure((size_t)NB * per_atom);
-  for (int B0 = 0; B0 < N; B0 += NB) {
-    const int nb = std::min(NB, N - B0);
+  for (int B0 = 0; B0 < NX; B0 += NB) {
+    const int nb = std::min(NB, NX - B0);
     const int64_t NBop = (int64_t)nb * op;
     // Stage 3 reads T's pad rows j in [o, o_p) of every atom against the zero
     // pads of G; they are never written by stage 1 and the buffer is shared by
@@ -550,7 +596,7 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
 }
 
 void products(tda_ctx* c, const double* X, int k, double* AX, int prec) {
-  const bool ao = c->has_lc && c->op_kind == 0;
+  const bool ao = c->has_lc && c->op_kind == 0 && !c->ct;
   int chunk = 64;
   if (ao) {
     // per-vector AO workspace: XR + M + F
@@ -895,7 +941,9 @@ int tda_set_system(tda_ctx* c, int n_atoms, int n_ao, const int32_t* opa, const 
     c->has_system = false;
     c->prepared = false;
     reset_lowprec(c);
+    c->ct = false;
     c->N = n_atoms;
+    c->Nq = c->Nx = n_atoms;
     c->nao = n_ao;
     c->ob = occ_begin;
     c->o = n_occ;
@@ -941,6 +989,92 @@ int tda_set_system(tda_ctx* c, int n_atoms, int n_ao, const int32_t* opa, const 
   });
 }
 
+int tda_set_ct_pair(tda_ctx* c, int n_atoms_h, int n_ao_h, const int32_t* opa_h, const double* orbs_h,
+                    const double* s_h, const double* orbe_h, int occ_begin_h, int n_occ_h, int n_atoms_l, int n_ao_l,
+                    const int32_t* opa_l, const double* orbs_l, const double* s_l, const double* orbe_l,
+                    int virt_begin_l, int n_virt_l, const double* s_hl, const double* gamma_h,
+                    const double* gamma_l, const double* gamma_hl, const double* gamma_lr_hl, const double* xyz) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] {
+    check_sizes(n_atoms_h, n_ao_h, opa_h, occ_begin_h, n_occ_h, 0, 1);
+    check_sizes(n_atoms_l, n_ao_l, opa_l, 0, 1, virt_begin_l, n_virt_l);
+    TDA_CHECK(orbs_h && s_h && orbe_h && orbs_l && s_l && orbe_l && s_hl && gamma_h && gamma_l && gamma_hl && xyz,
+              TDA_ERR_BAD_ARG, "NULL input array");
+    c->has_system = false;
+    c->prepared = false;
+    reset_lowprec(c);
+    const int NH = n_atoms_h, NL = n_atoms_l, N = NH + NL;
+    c->ct = true;
+    c->N = N;
+    c->Nq = NH;
+    c->Nx = NL;
+    c->nao = n_ao_h;
+    c->nao_l = n_ao_l;
+    c->lumo_l = virt_begin_l;
+    c->ob = occ_begin_h;
+    c->o = n_occ_h;
+    c->vb = n_ao_h + virt_begin_l;  // index into the stacked [orbe_h; orbe_l]
+    c->v = n_virt_l;
+    c->op = (int)round_up(n_occ_h, 64);
+    c->vp = (int)round_up(n_virt_l, 64);
+    c->ovp = (int64_t)c->o * c->vp;
+    c->has_lc = gamma_lr_hl != nullptr;
+    const size_t nhh = (size_t)n_ao_h * n_ao_h, nll = (size_t)n_ao_l * n_ao_l, nhl = (size_t)n_ao_h * n_ao_l;
+    c->C.ensure(nhh * 8);
+    c->S.ensure(nhh * 8);
+    c->Cl.ensure(nll * 8);
+    c->Sl.ensure(nll * 8);
+    c->Shl.ensure(nhl * 8);
+    TDA_HIP(hipMemcpyAsync(c->C.p, orbs_h, nhh * 8, hipMemcpyHostToDevice, c->stream));
+    TDA_HIP(hipMemcpyAsync(c->S.p, s_h, nhh * 8, hipMemcpyHostToDevice, c->stream));
+    TDA_HIP(hipMemcpyAsync(c->Cl.p, orbs_l, nll * 8, hipMemcpyHostToDevice, c->stream));
+    TDA_HIP(hipMemcpyAsync(c->Sl.p, s_l, nll * 8, hipMemcpyHostToDevice, c->stream));
+    TDA_HIP(hipMemcpyAsync(c->Shl.p, s_hl, nhl * 8, hipMemcpyHostToDevice, c->stream));
+    // stacked orbital energies: omega[i, a] = orbe_l[lumo + a] - orbe_h[i] (ct_pair.rs:43-58)
+    std::vector<double> e((size_t)n_ao_h + n_ao_l);
+    std::memcpy(e.data(), orbe_h, (size_t)n_ao_h * 8);
+    std::memcpy(e.data() + n_ao_h, orbe_l, (size_t)n_ao_l * 8);
+    c->orbe.ensure(e.size() * 8);
+    TDA_HIP(hipMemcpyAsync(c->orbe.p, e.data(), e.size() * 8, hipMemcpyHostToDevice, c->stream));
+    // pair gamma from the monomer blocks and the H-L block of the full gamma (ct_pair.rs:20-27)
+    std::vector<double> g((size_t)N * N);
+    for (int A = 0; A < NH; ++A)
+      for (int B = 0; B < NH; ++B) g[(size_t)A * N + B] = gamma_h[(size_t)A * NH + B];
+    for (int A = 0; A < NL; ++A)
+      for (int B = 0; B < NL; ++B) g[(size_t)(NH + A) * N + NH + B] = gamma_l[(size_t)A * NL + B];
+    for (int A = 0; A < NH; ++A)
+      for (int B = 0; B < NL; ++B) {
+        g[(size_t)A * N + NH + B] = gamma_hl[(size_t)A * NL + B];
+        g[(size_t)(NH + B) * N + A] = gamma_hl[(size_t)A * NL + B];
+      }
+    c->gamma.ensure(g.size() * 8);
+    TDA_HIP(hipMemcpyAsync(c->gamma.p, g.data(), g.size() * 8, hipMemcpyHostToDevice, c->stream));
+    std::vector<double> glr_t;
+    if (c->has_lc) {
+      // stored transposed, [NL][NH], so that G = gamma_lr^T q_oo is one GEMM
+      glr_t.resize((size_t)NL * NH);
+      for (int A = 0; A < NH; ++A)
+        for (int B = 0; B < NL; ++B) glr_t[(size_t)B * NH + A] = gamma_lr_hl[(size_t)A * NL + B];
+      c->gamma_lr.ensure(glr_t.size() * 8);
+      TDA_HIP(hipMemcpyAsync(c->gamma_lr.p, glr_t.data(), glr_t.size() * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    std::vector<int> off_h = ao_offsets(NH, opa_h), off_l = ao_offsets(NL, opa_l);
+    c->aooff.ensure(off_h.size() * sizeof(int));
+    c->aooff_l.ensure(off_l.size() * sizeof(int));
+    TDA_HIP(hipMemcpyAsync(c->aooff.p, off_h.data(), off_h.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    TDA_HIP(hipMemcpyAsync(c->aooff_l.p, off_l.data(), off_l.size() * sizeof(int), hipMemcpyHostToDevice,
+                           c->stream));
+    TDA_HIP(hipStreamSynchronize(c->stream));  // host staging vectors go out of scope
+    c->mo_built = false;
+    c->xyz.assign(xyz, xyz + (size_t)N * 3);
+    c->h_de.resize((size_t)n_occ_h * n_virt_l);
+    for (int i = 0; i < n_occ_h; ++i)
+      for (int a = 0; a < n_virt_l; ++a)
+        c->h_de[(size_t)i * n_virt_l + a] = orbe_l[virt_begin_l + a] - orbe_h[occ_begin_h + i];
+    c->has_system = true;
+  });
+}
+
 int tda_prepare(tda_ctx* c) {
   if (!c) return TDA_ERR_BAD_ARG;
   return guarded(c, [&] { prepare(c); });
@@ -966,14 +1100,14 @@ int tda_get_charges(tda_ctx* c, double* q_ov, double* q_oo, double* q_vv, double
     if ((q_oo || q_vv) && !c->has_lc) throw Error{TDA_ERR_STATE, "q_oo/q_vv are built only with gamma_lr"};
     if (q_oo || q_vv) build_mo_charges(c);
     if (q_oo) {
-      std::vector<double> h((size_t)N * op * op);
+      std::vector<double> h((size_t)c->Nq * op * op);
       TDA_HIP(hipMemcpy(h.data(), c->qoo.p, h.size() * 8, hipMemcpyDeviceToHost));
-      for (int A = 0; A < N; ++A) copy_unpad_rows(h.data() + (size_t)A * op * op, o, o, op, q_oo + (size_t)A * o * o);
+      for (int A = 0; A < c->Nq; ++A) copy_unpad_rows(h.data() + (size_t)A * op * op, o, o, op, q_oo + (size_t)A * o * o);
     }
     if (q_vv) {
-      std::vector<double> h((size_t)N * vp * vp);
+      std::vector<double> h((size_t)c->Nx * vp * vp);
       TDA_HIP(hipMemcpy(h.data(), c->qvv.p, h.size() * 8, hipMemcpyDeviceToHost));
-      for (int A = 0; A < N; ++A) copy_unpad_rows(h.data() + (size_t)A * vp * vp, v, v, vp, q_vv + (size_t)A * v * v);
+      for (int A = 0; A < c->Nx; ++A) copy_unpad_rows(h.data() + (size_t)A * vp * vp, v, v, vp, q_vv + (size_t)A * v * v);
     }
     if (omega) std::memcpy(omega, c->h_de.data(), c->h_de.size() * 8);
   });

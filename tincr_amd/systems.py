@@ -219,3 +219,62 @@ def tda_inputs(mol, seed=0, lc=True):
         gamma=gamma_atomwise(mol.z, mol.xyz, 0.0),
         gamma_lr=gamma_atomwise(mol.z, mol.xyz, LONG_RANGE_RADIUS) if lc else None,
     )
+
+
+# ---------------------------------------------------------------------------
+# charge-transfer pair inputs
+# ---------------------------------------------------------------------------
+def model_overlap(mol, seed=0, beta=0.3, sigma=1.6):
+    """A symmetric positive definite AO overlap with unit diagonal whose
+    off-diagonal elements decay with distance: S = (1 - beta) I + beta K with
+    the Gaussian kernel K_{mu,nu} = exp(-|p_mu - p_nu|^2 / (2 sigma^2)) over
+    AO centres p_mu (atom position + a seeded 0.5 bohr displacement).  Its
+    eigenvalues are >= 1 - beta.  A stand-in for the Slater-Koster overlap
+    where only its structure (SPD, decaying inter-fragment block) matters."""
+    rng = np.random.default_rng(seed + 2000)
+    centres = np.repeat(mol.xyz, mol.orbs_per_atom, axis=0)
+    d = rng.standard_normal(centres.shape)
+    centres = centres + 0.5 * d / np.linalg.norm(d, axis=1)[:, None]
+    r2 = ((centres[:, None, :] - centres[None, :, :]) ** 2).sum(-1)
+    return (1.0 - beta) * np.eye(len(centres)) + beta * np.exp(-r2 / (2.0 * sigma * sigma))
+
+
+def fragment_orbitals(s, n_occ, seed=0, gap=0.1):
+    """S-orthonormal orbitals C = S^-1/2 Q (Q Haar-random) and sorted orbital
+    energies with a HOMO-LUMO gap >= `gap` Ha."""
+    rng = np.random.default_rng(seed + 3000)
+    n = s.shape[0]
+    w, u = np.linalg.eigh(s)
+    s_mhalf = (u / np.sqrt(w)[None, :]) @ u.T
+    q, r = np.linalg.qr(rng.standard_normal((n, n)))
+    q = q * np.sign(np.diag(r))[None, :]
+    occ = np.sort(rng.uniform(-1.0, -0.3, n_occ))
+    virt = np.sort(rng.uniform(-0.3 + gap, 0.6, n - n_occ))
+    return s_mhalf @ q, np.concatenate([occ, virt])
+
+
+def ct_pair_inputs(mol, n_atoms_h, seed=0, lc=True):
+    """Inputs of a charge-transfer pair H -> L cut from `mol`: H = its first
+    `n_atoms_h` atoms, L = the rest (each must hold an even electron count).
+    Overlap from ``model_overlap`` of the whole molecule (H-L block
+    included), S-orthonormal orbitals per fragment, real gamma / gamma_lr of
+    the geometry.  Returns dict(h=..., l=..., s_hl, gamma_hl, gamma_lr_hl,
+    s, gamma, gamma_lr) with h / l dicts of the fragment's n_atoms,
+    orbs_per_atom, xyz, occ_indices, virt_indices, orbs, s, orbe, gamma."""
+    nh = n_atoms_h
+    frag_h = Molecule(mol.z[:nh], mol.xyz[:nh], mol.name + "_H")
+    frag_l = Molecule(mol.z[nh:], mol.xyz[nh:], mol.name + "_L")
+    s = model_overlap(mol, seed)
+    gamma = gamma_atomwise(mol.z, mol.xyz, 0.0)
+    gamma_lr = gamma_atomwise(mol.z, mol.xyz, LONG_RANGE_RADIUS) if lc else None
+    nao_h = frag_h.n_ao
+    out = dict(s=s, gamma=gamma, gamma_lr=gamma_lr, s_hl=s[:nao_h, nao_h:].copy(),
+               gamma_hl=gamma[:nh, nh:].copy(), gamma_lr_hl=None if gamma_lr is None else gamma_lr[:nh, nh:].copy())
+    for key, frag, ao, at, sd in (("h", frag_h, slice(0, nao_h), slice(0, nh), seed),
+                                  ("l", frag_l, slice(nao_h, None), slice(nh, None), seed + 1)):
+        sf = s[ao, ao].copy()
+        c, orbe = fragment_orbitals(sf, frag.n_occ, sd)
+        out[key] = dict(n_atoms=frag.n_atoms, orbs_per_atom=frag.orbs_per_atom, xyz=frag.xyz,
+                        occ_indices=list(range(frag.n_occ)), virt_indices=list(range(frag.n_occ, frag.n_ao)),
+                        orbs=c, s=sf, orbe=orbe, gamma=gamma[at, at].copy())
+    return out

@@ -11,6 +11,8 @@ Names, argument meaning and error behaviour follow the reference:
   (``ci_eigenvalues``, ``ci_coefficients``, ``q_trans``, ``tr_dipoles``,
   ``oscillator_strengths``, ``q_ov``/``q_oo``/``q_vv``, ``omega``, ``homo``,
   ``lumo``).
+* ``TdaCTPair``          PairChargeTransfer (tda/ct_pair.rs:12-180) with its
+  DavidsonEngine (solvers/mod.rs:165-274)
 * ``DavidsonError``      src/excited_states/solvers/davidson.rs:28-37 -- raised
   where the reference's callers ``unwrap()`` a non-converged Davidson.
 
@@ -74,32 +76,13 @@ def orbe_differences(occ_quant, virt_quant):
     return out
 
 
-class TdaSystem:
-    """DavidsonEngine + prepare_tda/run_tda of one System or Monomer."""
+class _TdaEngine:
+    """DavidsonEngine + prepare/run shared by a System/Monomer and a CT pair.
 
-    def __init__(self, n_atoms, orbs_per_atom, xyz, occ_indices, virt_indices, orbs, s, orbe, gamma,
-                 gamma_lr=None, precision="fp64", device=0, hbm_budget_bytes=0, operator="ao"):
-        self.ctx = nat.Context(device, precision, hbm_budget_bytes)
-        self.ctx.set_operator(operator)
-        self.n_atoms = n_atoms
-        self.occ_indices = list(occ_indices)
-        self.virt_indices = list(virt_indices)
-        self.ob, self.n_occ, self.vb, self.n_virt = _ranges(occ_indices, virt_indices)
-        self.opa = np.ascontiguousarray(orbs_per_atom, dtype=np.int32)
-        self.xyz = nat.f64(np.asarray(xyz).reshape(n_atoms, 3))
-        self._orbs = nat.f64(orbs)
-        self._s = nat.f64(s)
-        self._orbe = nat.f64(np.ravel(orbe))
-        self._gamma = nat.f64(gamma)
-        self._gamma_lr = nat.f64(gamma_lr)
-        self.gammafunction_lc = gamma_lr is not None
-        self.properties = {}
-        self.last_stats = None
-        n_ao = self._orbs.shape[0]
-        self.ctx.check(self.ctx.lib.tda_set_system(
-            self.ctx.handle, n_atoms, n_ao, nat.iptr(self.opa), nat.dptr(self._orbs), nat.dptr(self._s),
-            nat.dptr(self._orbe), self.ob, self.n_occ, self.vb, self.n_virt, nat.dptr(self._gamma),
-            nat.dptr(self._gamma_lr), nat.dptr(self.xyz)))
+    Subclasses load their inputs into ``self.ctx`` and set ``n_atoms`` (atoms
+    of q_ov), ``n_atoms_oo`` / ``n_atoms_vv`` (atoms of q_oo / q_vv),
+    ``n_occ``, ``n_virt``, ``occ_indices``, ``virt_indices`` and
+    ``gammafunction_lc``."""
 
     def close(self):
         self.ctx.close()
@@ -111,8 +94,8 @@ class TdaSystem:
         omega = np.empty(o * v)
         if fetch_charges:
             q_ov = np.empty((N, o * v))
-            q_oo = np.empty((N, o * o)) if self.gammafunction_lc else None
-            q_vv = np.empty((N, v * v)) if self.gammafunction_lc else None
+            q_oo = np.empty((self.n_atoms_oo, o * o)) if self.gammafunction_lc else None
+            q_vv = np.empty((self.n_atoms_vv, v * v)) if self.gammafunction_lc else None
             self.ctx.check(self.ctx.lib.tda_get_charges(self.ctx.handle, nat.dptr(q_ov), nat.dptr(q_oo),
                                                         nat.dptr(q_vv), nat.dptr(omega)))
             self.properties.update(q_ov=q_ov, q_oo=q_oo, q_vv=q_vv)
@@ -170,3 +153,89 @@ class TdaSystem:
     def tdm(self, idx):
         """Transition density of state idx as [n_occ, n_virt] (properties/getter.rs:130-148)."""
         return self.properties["ci_coefficients"][:, idx].reshape(self.n_occ, self.n_virt)
+
+
+class TdaSystem(_TdaEngine):
+    """DavidsonEngine + prepare_tda/run_tda of one System or Monomer."""
+
+    def __init__(self, n_atoms, orbs_per_atom, xyz, occ_indices, virt_indices, orbs, s, orbe, gamma,
+                 gamma_lr=None, precision="fp64", device=0, hbm_budget_bytes=0, operator="ao"):
+        self.ctx = nat.Context(device, precision, hbm_budget_bytes)
+        self.ctx.set_operator(operator)
+        self.n_atoms = self.n_atoms_oo = self.n_atoms_vv = n_atoms
+        self.occ_indices = list(occ_indices)
+        self.virt_indices = list(virt_indices)
+        self.ob, self.n_occ, self.vb, self.n_virt = _ranges(occ_indices, virt_indices)
+        self.opa = np.ascontiguousarray(orbs_per_atom, dtype=np.int32)
+        self.xyz = nat.f64(np.asarray(xyz).reshape(n_atoms, 3))
+        self._orbs = nat.f64(orbs)
+        self._s = nat.f64(s)
+        self._orbe = nat.f64(np.ravel(orbe))
+        self._gamma = nat.f64(gamma)
+        self._gamma_lr = nat.f64(gamma_lr)
+        self.gammafunction_lc = gamma_lr is not None
+        self.properties = {}
+        self.last_stats = None
+        n_ao = self._orbs.shape[0]
+        self.ctx.check(self.ctx.lib.tda_set_system(
+            self.ctx.handle, n_atoms, n_ao, nat.iptr(self.opa), nat.dptr(self._orbs), nat.dptr(self._s),
+            nat.dptr(self._orbe), self.ob, self.n_occ, self.vb, self.n_virt, nat.dptr(self._gamma),
+            nat.dptr(self._gamma_lr), nat.dptr(self.xyz)))
+
+
+class Fragment:
+    """The monomer data a CT pair reads (fmo Monomer properties): atoms'
+    AO counts, coordinates [n_atoms, 3] (bohr), occupied / virtual MO indices,
+    orbitals, AO overlap, orbital energies and the monomer gamma."""
+
+    def __init__(self, n_atoms, orbs_per_atom, xyz, occ_indices, virt_indices, orbs, s, orbe, gamma):
+        self.n_atoms = n_atoms
+        self.orbs_per_atom = np.ascontiguousarray(orbs_per_atom, dtype=np.int32)
+        self.xyz = nat.f64(np.asarray(xyz).reshape(n_atoms, 3))
+        self.occ_indices = list(occ_indices)
+        self.virt_indices = list(virt_indices)
+        self.orbs = nat.f64(orbs)
+        self.s = nat.f64(s)
+        self.orbe = nat.f64(np.ravel(orbe))
+        self.gamma = nat.f64(gamma)
+
+
+class TdaCTPair(_TdaEngine):
+    """PairChargeTransfer H -> L: ``prepare_ct_tda`` (tda/ct_pair.rs:12-76),
+    its DavidsonEngine (solvers/mod.rs:165-274) and ``run_ct_tda``
+    (ct_pair.rs:125-180).  Holes in H's occupied MOs, particles in L's
+    virtual MOs; q_ov over the H and L atoms (H first), q_oo of H, q_vv of L.
+
+    ``s_hl`` is the H-L block of the AO overlap [n_ao_h, n_ao_l];
+    ``gamma_hl`` / ``gamma_lr_hl`` the H-L blocks [n_atoms_h, n_atoms_l] of
+    the full gamma and gamma_lr (``g0.slice(H atoms, L atoms)``, :24, :37).
+    The pair always uses the per-atom-charges operator."""
+
+    def __init__(self, h, l, s_hl, gamma_hl, gamma_lr_hl, precision="fp64", device=0, hbm_budget_bytes=0):
+        self.ctx = nat.Context(device, precision, hbm_budget_bytes)
+        self.h, self.l = h, l
+        self.n_atoms = h.n_atoms + l.n_atoms
+        self.n_atoms_oo, self.n_atoms_vv = h.n_atoms, l.n_atoms
+        self.occ_indices = list(h.occ_indices)
+        self.virt_indices = list(l.virt_indices)
+        ob, self.n_occ, _, _ = _ranges(h.occ_indices, h.virt_indices)
+        _, _, vb, self.n_virt = _ranges(l.occ_indices, l.virt_indices)
+        self.xyz = nat.f64(np.concatenate([h.xyz, l.xyz]))
+        self._s_hl = nat.f64(s_hl)
+        self._gamma_hl = nat.f64(gamma_hl)
+        self._gamma_lr_hl = nat.f64(gamma_lr_hl)
+        self.gammafunction_lc = gamma_lr_hl is not None
+        self.properties = {}
+        self.last_stats = None
+        self.ctx.check(self.ctx.lib.tda_set_ct_pair(
+            self.ctx.handle, h.n_atoms, h.orbs.shape[0], nat.iptr(h.orbs_per_atom), nat.dptr(h.orbs),
+            nat.dptr(h.s), nat.dptr(h.orbe), ob, self.n_occ, l.n_atoms, l.orbs.shape[0],
+            nat.iptr(l.orbs_per_atom), nat.dptr(l.orbs), nat.dptr(l.s), nat.dptr(l.orbe), vb, self.n_virt,
+            nat.dptr(self._s_hl), nat.dptr(h.gamma), nat.dptr(l.gamma), nat.dptr(self._gamma_hl),
+            nat.dptr(self._gamma_lr_hl), nat.dptr(self.xyz)))
+
+    def prepare_ct_tda(self, fetch_charges=True):
+        self.prepare_tda(fetch_charges)
+
+    def run_ct_tda(self, n_roots, max_iter, tolerance):
+        return self.run_tda(n_roots, max_iter, tolerance)
