@@ -120,6 +120,14 @@ int tda_set_ct_pair(tda_ctx* ctx, int n_atoms_h, int n_ao_h, const int32_t* orbs
                     int virt_begin_l, int n_virt_l, const double* s_hl, const double* gamma_h,
                     const double* gamma_l, const double* gamma_hl, const double* gamma_lr_hl, const double* xyz);
 
+/* Selects the dense-matrix engine (DavidsonEngine for Array2,
+ * src/excited_states/solvers/mod.rs:36-51), as the FMO exciton Hamiltonian
+ * uses it (src/fmo/lcmo/basis.rs:497-503): h [dim, dim] row-major,
+ * products h x, preconditioner r / (w - diag(h)) with no floor, guess at the
+ * smallest diagonal elements.  tda_run then solves in FP64 and accepts only
+ * omega and X (q_trans, tr_dipoles, f must be NULL); X is [dim, n_roots]. */
+int tda_set_dense(tda_ctx* ctx, int dim, const double* h);
+
 /* prepare_tda: builds q_ov/q_oo/q_vv and omega on the GPU (and
  * gamma_lr.q_oo, once per system instead of once per product call as
  * solvers/mod.rs:328-331 does). */

@@ -34,10 +34,12 @@ void launch_cast_vectors(const double* src, int64_t n, void* dst, int dtype, hip
 
 // ---- Davidson helpers (davidson.rs:53-185, solvers/mod.rs:373-380) ----
 // out = r / d,  d = w - de,  |d| < 1e-4 -> 1.0
-void launch_precondition(const double* r, double w, const double* de, int64_t L, double* out, hipStream_t s);
+// out = r / (w - de), |w - de| < thr replaced by 1 (thr = 0: no floor)
+void launch_precondition(const double* r, double w, const double* de, int64_t L, double thr, double* out,
+                         hipStream_t s);
 // P[c] = precondition(R[idx[c]], u[idx[c]])   idx, u on device
 void launch_precondition_many(const double* R, const double* u, const int* idx, int nc, const double* de,
-                              int64_t L, double* P, hipStream_t s);
+                              int64_t L, double thr, double* P, hipStream_t s);
 // R[r][e] = AVY[r][e] - u[r] * RZ[r][e]  (in place on AVY)
 void launch_residual(double* AVY, const double* RZ, const double* u, int n, int64_t L, hipStream_t s);
 // norms[r] = || X[r] ||  (deterministic two-pass), partial: >= n*256 doubles
@@ -76,6 +78,11 @@ void launch_ao_add_coulomb(void* M, int dtype, int k, int n_ao, const int* atom,
 // H: m x m symmetric (upper triangle read), leading dim ldh; work >= eigh_work_doubles(m)
 // Outputs the n lowest eigenvalues (ascending) and eigenvectors Y[r][0..m) (row per root).
 size_t eigh_work_doubles(int m);
+
+// Mulliken transition dipoles mu [3][n] = xyz^T qt and oscillator strengths
+// f [n] = 2/3 w |mu|^2 (xyz [N][3], qt [N][n], w [n]; all device)
+void launch_dipoles(const double* xyz, const double* qt, const double* w, int N, int n, double* mu, double* f,
+                    hipStream_t s);
 // phase_clk (optional, device): wall-clock stamps at the phase boundaries
 // (copy, tridiagonalisation, bisection, inverse iteration, back-transform).
 void launch_eigh(const double* H, int ldh, int m, int n, double* work, double* evals, double* Y, int ldy,

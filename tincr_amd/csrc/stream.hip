@@ -242,21 +242,24 @@ void launch_cast_vectors(const double* src, int64_t n, void* dst, int dtype, hip
 // ---------------------------------------------------------------------------
 // Davidson helpers
 // ---------------------------------------------------------------------------
-__device__ inline double precond_denom(double w, double de) {
+// thr = 1e-4 for the TDA engines (solvers/mod.rs:378); 0 (no floor) for a
+// dense matrix engine (solvers/mod.rs:44-46)
+__device__ inline double precond_denom(double w, double de, double thr) {
   const double d = w - de;
-  return fabs(d) < 0.0001 ? 1.0 : d;  // solvers/mod.rs:378
+  return fabs(d) < thr ? 1.0 : d;
 }
 
 __global__ void __launch_bounds__(256) precondition_kernel(const double* __restrict__ r, double w,
-                                                           const double* __restrict__ de, int64_t L,
+                                                           const double* __restrict__ de, int64_t L, double thr,
                                                            double* __restrict__ out) {
   for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256)
-    out[t] = r[t] / precond_denom(w, de[t]);
+    out[t] = r[t] / precond_denom(w, de[t], thr);
 }
 
-void launch_precondition(const double* r, double w, const double* de, int64_t L, double* out, hipStream_t s) {
+void launch_precondition(const double* r, double w, const double* de, int64_t L, double thr, double* out,
+                         hipStream_t s) {
   if (L == 0) return;
-  precondition_kernel<<<grid_for(L), 256, 0, s>>>(r, w, de, L, out);
+  precondition_kernel<<<grid_for(L), 256, 0, s>>>(r, w, de, L, thr, out);
   TDA_LAUNCH_CHECK();
 }
 
@@ -264,21 +267,21 @@ __global__ void __launch_bounds__(256) precondition_many_kernel(const double* __
                                                                 const double* __restrict__ u,
                                                                 const int* __restrict__ idx, int nc,
                                                                 const double* __restrict__ de, int64_t L,
-                                                                double* __restrict__ P) {
+                                                                double thr, double* __restrict__ P) {
   const int c = blockIdx.y;
   const int r = idx[c];
   const double w = u[r];
   const double* src = R + (int64_t)r * L;
   double* dst = P + (int64_t)c * L;
   for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256)
-    dst[t] = src[t] / precond_denom(w, de[t]);
+    dst[t] = src[t] / precond_denom(w, de[t], thr);
 }
 
 void launch_precondition_many(const double* R, const double* u, const int* idx, int nc, const double* de, int64_t L,
-                              double* P, hipStream_t s) {
+                              double thr, double* P, hipStream_t s) {
   if (nc == 0 || L == 0) return;
   dim3 grid((unsigned)grid_for(L, 256, 4096), (unsigned)nc);
-  precondition_many_kernel<<<grid, 256, 0, s>>>(R, u, idx, nc, de, L, P);
+  precondition_many_kernel<<<grid, 256, 0, s>>>(R, u, idx, nc, de, L, thr, P);
   TDA_LAUNCH_CHECK();
 }
 
@@ -597,6 +600,44 @@ void launch_small_gemm(const double* A, const double* B, double* C, int M, int N
   const int64_t total = (int64_t)M * N;
   if (total == 0) return;
   small_gemm_kernel<<<grid_for(total), 256, 0, s>>>(A, B, C, M, N, K);
+  TDA_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Mulliken transition dipoles and oscillator strengths (tda/moments.rs:7-24):
+// mu[d][r] = sum_A xyz[A][d] q_trans[A][r],  f[r] = 2/3 w[r] |mu[:, r]|^2
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) dipoles_kernel(const double* __restrict__ xyz, const double* __restrict__ qt,
+                                                      const double* __restrict__ w, int N, int n,
+                                                      double* __restrict__ mu, double* __restrict__ f) {
+  __shared__ double red[3][256];
+  const int r = blockIdx.x;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int A = threadIdx.x; A < N; A += 256) {
+    const double q = qt[(int64_t)A * n + r];
+    for (int d = 0; d < 3; ++d) s[d] += xyz[(int64_t)A * 3 + d] * q;
+  }
+  for (int d = 0; d < 3; ++d) red[d][threadIdx.x] = s[d];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h)
+      for (int d = 0; d < 3; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double m2 = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      mu[(int64_t)d * n + r] = red[d][0];
+      m2 += red[d][0] * red[d][0];
+    }
+    f[r] = 2.0 / 3.0 * w[r] * m2;
+  }
+}
+
+void launch_dipoles(const double* xyz, const double* qt, const double* w, int N, int n, double* mu, double* f,
+                    hipStream_t s) {
+  if (n == 0) return;
+  dipoles_kernel<<<n, 256, 0, s>>>(xyz, qt, w, N, n, mu, f);
   TDA_LAUNCH_CHECK();
 }
 

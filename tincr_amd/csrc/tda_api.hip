@@ -94,6 +94,11 @@ struct tda_ctx {
   bool ct = false;
   int nao_l = 0, lumo_l = 0;
   DevBuf Cl, Sl, SCl, Shl, aooff_l, ctW, ctZ;
+  // dense matrix engine (solvers/mod.rs:36-51): Hd [vp][vp] zero-padded,
+  // stored as o = 1, v = dim; no preconditioner floor
+  bool dense = false;
+  DevBuf Hd, dxyz, post;
+  double pre_thr = 1e-4;
   int64_t ovp = 0;
   std::vector<double> xyz, h_de;
   DevBuf C, S, SC, orbe, aooff, gamma, gamma_lr;
@@ -141,7 +146,7 @@ struct tda_ctx {
                       &qvv_amax, &G_amax, &ws_gemm, &T, &xlp, &u, &uu, &vv32, &ax_tmp,
                       &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &P, &W, &norms, &npart, &idx, &count,
                       &gs_scratch, &eigw, &ao_atom, &Gt, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
-                      &Cl, &Sl, &SCl, &Shl, &aooff_l, &ctW, &ctZ})
+                      &Cl, &Sl, &SCl, &Shl, &aooff_l, &ctW, &ctZ, &Hd, &dxyz, &post})
       b->release();
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -364,6 +369,7 @@ void build_ao_operands(tda_ctx* c) {
 
 void prepare(tda_ctx* c) {
   TDA_CHECK(c->has_system, TDA_ERR_STATE, "tda_prepare: no system loaded");
+  if (c->dense) return;
   if (c->prepared) {
     prepare_lowprec(c, c->prec);
     return;
@@ -596,6 +602,15 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
 }
 
 void products(tda_ctx* c, const double* X, int k, double* AX, int prec) {
+  if (c->dense) {
+    // AX[c][r] = sum_s H[r][s] X[c][s]  (solvers/mod.rs:41-43), pads of H zero
+    const int64_t L = c->ovp;
+    GemmArgs g = gargs(X, c->Hd.p, AX, k, L, L, plain(L), plain(1), plain(1), plain(L), plain(L), plain(1));
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
+    c->n_products += k;
+    c->n_products_fp64 += k;
+    return;
+  }
   const bool ao = c->has_lc && c->op_kind == 0 && !c->ct;
   int chunk = 64;
   if (ao) {
@@ -721,8 +736,8 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       dim_sub += roots_lft;  // davidson.rs:159 (counts dropped vectors too)
       if (nc > 0) {
         TDA_HIP(hipMemcpyAsync(c->idx.p, idx.data(), nc * sizeof(int), hipMemcpyHostToDevice, c->stream));
-        launch_precondition_many(R, c->evals.as<double>(), c->idx.as<int>(), nc, c->de.as<double>(), L, P,
-                                 c->stream);
+        launch_precondition_many(R, c->evals.as<double>(), c->idx.as<int>(), nc, c->de.as<double>(), L,
+                                 c->pre_thr, P, c->stream);
         // classical Gram-Schmidt (davidson.rs:162-168): W = P - V (V^T P) ...
         double* C0 = c->stage2.as<double>();
         GemmArgs g0 = gargs(V, P, C0, m, nc, L, plain(L), plain(1), plain(1), plain(L), plain(nc), plain(1));
@@ -942,6 +957,8 @@ int tda_set_system(tda_ctx* c, int n_atoms, int n_ao, const int32_t* opa, const 
     c->prepared = false;
     reset_lowprec(c);
     c->ct = false;
+    c->dense = false;
+    c->pre_thr = 1e-4;
     c->N = n_atoms;
     c->Nq = c->Nx = n_atoms;
     c->nao = n_ao;
@@ -1005,6 +1022,8 @@ int tda_set_ct_pair(tda_ctx* c, int n_atoms_h, int n_ao_h, const int32_t* opa_h,
     reset_lowprec(c);
     const int NH = n_atoms_h, NL = n_atoms_l, N = NH + NL;
     c->ct = true;
+    c->dense = false;
+    c->pre_thr = 1e-4;
     c->N = N;
     c->Nq = NH;
     c->Nx = NL;
@@ -1075,6 +1094,44 @@ int tda_set_ct_pair(tda_ctx* c, int n_atoms_h, int n_ao_h, const int32_t* opa_h,
   });
 }
 
+int tda_set_dense(tda_ctx* c, int dim, const double* h) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] {
+    TDA_CHECK(dim > 0 && h, TDA_ERR_BAD_ARG, "dim must be positive and h non-NULL");
+    c->has_system = false;
+    c->prepared = false;
+    reset_lowprec(c);
+    c->ct = false;
+    c->dense = true;
+    c->pre_thr = 0.0;
+    c->has_lc = false;
+    c->N = c->Nq = c->Nx = 0;
+    c->nao = 0;
+    c->ob = c->vb = 0;
+    c->o = 1;
+    c->v = dim;
+    c->op = 64;
+    c->vp = (int)round_up(dim, 64);
+    c->ovp = c->vp;
+    const int vp = c->vp;
+    std::vector<double> hp((size_t)vp * vp, 0.0), d((size_t)vp, 1e300);
+    for (int r = 0; r < dim; ++r) {
+      std::memcpy(hp.data() + (size_t)r * vp, h + (size_t)r * dim, (size_t)dim * 8);
+      d[r] = h[(size_t)r * dim + r];
+    }
+    c->Hd.ensure(hp.size() * 8);
+    c->de.ensure(d.size() * 8);
+    TDA_HIP(hipMemcpyAsync(c->Hd.p, hp.data(), hp.size() * 8, hipMemcpyHostToDevice, c->stream));
+    // diagonal preconditioner; pad entries 1e300 so that pads stay 0
+    TDA_HIP(hipMemcpyAsync(c->de.p, d.data(), d.size() * 8, hipMemcpyHostToDevice, c->stream));
+    TDA_HIP(hipStreamSynchronize(c->stream));
+    c->h_de.assign(d.begin(), d.begin() + dim);  // guess from diag(H) (basis.rs:500)
+    c->xyz.clear();
+    c->mo_built = false;
+    c->has_system = true;
+  });
+}
+
 int tda_prepare(tda_ctx* c) {
   if (!c) return TDA_ERR_BAD_ARG;
   return guarded(c, [&] { prepare(c); });
@@ -1090,6 +1147,7 @@ int tda_clear_charges(tda_ctx* c) {
 int tda_get_charges(tda_ctx* c, double* q_ov, double* q_oo, double* q_vv, double* omega) {
   if (!c) return TDA_ERR_BAD_ARG;
   return guarded(c, [&] {
+    TDA_CHECK(!c->dense, TDA_ERR_STATE, "a dense engine has no transition charges");
     prepare(c);
     const int N = c->N, o = c->o, v = c->v, op = c->op, vp = c->vp;
     if (q_ov) {
@@ -1139,11 +1197,18 @@ int tda_precondition(tda_ctx* c, const double* r, double w, double* out) {
   return guarded(c, [&] {
     TDA_CHECK(c->has_system, TDA_ERR_STATE, "no system loaded");
     TDA_CHECK(r && out, TDA_ERR_BAD_ARG, "NULL r/out");
-    const size_t ov = c->h_de.size();
-    for (size_t t = 0; t < ov; ++t) {
-      const double d = w - c->h_de[t];
-      out[t] = r[t] / (std::fabs(d) < 0.0001 ? 1.0 : d);
-    }
+    prepare(c);
+    const int64_t ov = (int64_t)c->o * c->v;
+    c->stage.ensure((size_t)ov * 8);
+    c->ax_tmp.ensure((size_t)c->ovp * 8 * 2);
+    double* dR = c->ax_tmp.as<double>();
+    double* dP = dR + c->ovp;
+    TDA_HIP(hipMemcpyAsync(c->stage.p, r, (size_t)ov * 8, hipMemcpyHostToDevice, c->stream));
+    launch_pack_x(c->stage.as<double>(), c->o, c->v, c->vp, 1, dR, c->stream);
+    launch_precondition(dR, w, c->de.as<double>(), c->ovp, c->pre_thr, dP, c->stream);
+    launch_unpack_x(dP, c->o, c->v, c->vp, 1, c->stage.as<double>(), c->stream);
+    TDA_HIP(hipMemcpyAsync(out, c->stage.p, (size_t)ov * 8, hipMemcpyDeviceToHost, c->stream));
+    TDA_HIP(hipStreamSynchronize(c->stream));
   });
 }
 
@@ -1158,6 +1223,8 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     TDA_CHECK(n_roots >= 1 && n_roots <= 64 && n_roots <= ov, TDA_ERR_BAD_ARG,
               "n_roots must be in 1..min(64, n_occ*n_virt)");
     TDA_CHECK(max_iter >= 1 && tol > 0.0, TDA_ERR_BAD_ARG, "bad max_iter / tolerance");
+    TDA_CHECK(!c->dense || (!q_trans && !tr_dipoles && !f), TDA_ERR_BAD_ARG,
+              "a dense engine has no q_trans / tr_dipoles / f");
     c->reset_counters();
     const auto tq0 = std::chrono::steady_clock::now();
     prepare(c);
@@ -1181,7 +1248,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     }
     TDA_HIP(hipStreamSynchronize(c->stream));
     std::vector<double> h_evals, h_err;
-    if (c->prec == PREC_FP64) {
+    if (c->prec == PREC_FP64 || c->dense) {
       res = davidson(c, n, n, max_iter, tol, PREC_FP64, false, 0, h_evals, h_err);
       res64 = res;
     } else {
@@ -1208,37 +1275,37 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     }
     // outputs (tda/mod.rs:88-126)
     const int N = c->N;
+    const bool fp64_only = c->prec == PREC_FP64 || c->dense;
     if (omega) std::memcpy(omega, h_evals.data(), (size_t)n * 8);
     if (X) {
       c->stage.ensure((size_t)ov * n * 8);
       launch_unpack_x(c->RZ.as<double>(), c->o, c->v, c->vp, n, c->stage.as<double>(), c->stream);
       TDA_HIP(hipMemcpyAsync(X, c->stage.p, (size_t)ov * n * 8, hipMemcpyDeviceToHost, c->stream));
     }
-    std::vector<double> qt((size_t)N * n);
-    c->u.ensure((size_t)N * n * 8);
-    GemmArgs g = gargs(c->qov.p, c->RZ.p, c->u.p, N, n, L, plain(L), plain(1), plain(1), plain(L), plain(n), plain(1));
-    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
-    TDA_HIP(hipMemcpyAsync(qt.data(), c->u.p, qt.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    TDA_HIP(hipStreamSynchronize(c->stream));
-    if (q_trans) std::memcpy(q_trans, qt.data(), qt.size() * 8);
-    std::vector<double> mu((size_t)3 * n, 0.0);
-    for (int d = 0; d < 3; ++d)
-      for (int r = 0; r < n; ++r) {
-        double s = 0;
-        for (int A = 0; A < N; ++A) s += c->xyz[(size_t)A * 3 + d] * qt[(size_t)A * n + r];
-        mu[(size_t)d * n + r] = s;
-      }
-    if (tr_dipoles) std::memcpy(tr_dipoles, mu.data(), mu.size() * 8);
-    if (f)
-      for (int r = 0; r < n; ++r) {
-        double m2 = 0;
-        for (int d = 0; d < 3; ++d) m2 += mu[(size_t)d * n + r] * mu[(size_t)d * n + r];
-        f[r] = 2.0 / 3.0 * h_evals[r] * m2;
-      }
+    if (c->dense) TDA_HIP(hipStreamSynchronize(c->stream));
+    if (!c->dense) {
+      // q_trans = q_ov X (tda/mod.rs:91), mu = xyz^T q_trans, f = 2/3 w |mu|^2 (moments.rs:7-24)
+      c->u.ensure((size_t)N * n * 8);
+      GemmArgs g = gargs(c->qov.p, c->RZ.p, c->u.p, N, n, L, plain(L), plain(1), plain(1), plain(L), plain(n),
+                         plain(1));
+      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
+      c->dxyz.ensure((size_t)N * 3 * 8);
+      c->post.ensure((size_t)5 * n * 8);
+      double* d_w = c->post.as<double>();
+      double* d_mu = d_w + n;
+      double* d_f = d_mu + 3 * n;
+      TDA_HIP(hipMemcpyAsync(c->dxyz.p, c->xyz.data(), (size_t)N * 3 * 8, hipMemcpyHostToDevice, c->stream));
+      TDA_HIP(hipMemcpyAsync(d_w, h_evals.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+      launch_dipoles(c->dxyz.as<double>(), c->u.as<double>(), d_w, N, n, d_mu, d_f, c->stream);
+      if (q_trans) TDA_HIP(hipMemcpyAsync(q_trans, c->u.p, (size_t)N * n * 8, hipMemcpyDeviceToHost, c->stream));
+      if (tr_dipoles) TDA_HIP(hipMemcpyAsync(tr_dipoles, d_mu, (size_t)3 * n * 8, hipMemcpyDeviceToHost, c->stream));
+      if (f) TDA_HIP(hipMemcpyAsync(f, d_f, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipStreamSynchronize(c->stream));
+    }
     if (stats) {
       std::memset(stats, 0, sizeof(*stats));
-      stats->n_iter = res.iters + (c->prec == PREC_FP64 ? 0 : res64.iters);
-      stats->n_iter_fp64 = c->prec == PREC_FP64 ? res.iters : res64.iters;
+      stats->n_iter = res.iters + (fp64_only ? 0 : res64.iters);
+      stats->n_iter_fp64 = fp64_only ? res.iters : res64.iters;
       stats->n_products = c->n_products;
       stats->n_products_fp64 = c->n_products_fp64;
       stats->converged = res64.converged ? 1 : 0;

@@ -239,3 +239,52 @@ class TdaCTPair(_TdaEngine):
 
     def run_ct_tda(self, n_roots, max_iter, tolerance):
         return self.run_tda(n_roots, max_iter, tolerance)
+
+
+class DenseEngine:
+    """DavidsonEngine for a dense symmetric matrix (solvers/mod.rs:36-51):
+    products h x, preconditioner r / (w - diag(h)) without a floor."""
+
+    def __init__(self, h, device=0):
+        self.h = nat.f64(h)
+        assert self.h.ndim == 2 and self.h.shape[0] == self.h.shape[1]
+        self.ctx = nat.Context(device, "fp64")
+        self.ctx.check(self.ctx.lib.tda_set_dense(self.ctx.handle, self.h.shape[0], nat.dptr(self.h)))
+        self.last_stats = None
+
+    def close(self):
+        self.ctx.close()
+
+    def compute_products(self, x):
+        x = nat.f64(x)
+        squeeze = x.ndim == 1
+        if squeeze:
+            x = x[:, None].copy()
+        ax = np.empty_like(x)
+        self.ctx.check(self.ctx.lib.tda_compute_products(self.ctx.handle, nat.dptr(x), x.shape[1], nat.dptr(ax)))
+        return ax[:, 0] if squeeze else ax
+
+    def precondition(self, r_k, w_k):
+        r = nat.f64(r_k)
+        out = np.empty_like(r)
+        self.ctx.check(self.ctx.lib.tda_precondition(self.ctx.handle, nat.dptr(r), float(w_k), nat.dptr(out)))
+        return out
+
+    def get_size(self):
+        return self.h.shape[0]
+
+    def davidson(self, n_roots, tolerance, max_iter):
+        """Davidson::new(&mut h, initial_subspace(h.diag(), n_roots), n_roots,
+        tolerance, max_iter) (basis.rs:500-501): (eigenvalues, eigenvectors
+        [dim, n_roots]); raises DavidsonError if not converged."""
+        dim = self.get_size()
+        w = np.empty(n_roots)
+        x = np.empty((dim, n_roots))
+        st = nat.TdaStats()
+        rc = self.ctx.lib.tda_run(self.ctx.handle, n_roots, max_iter, float(tolerance), nat.dptr(w), nat.dptr(x),
+                                  None, None, None, nat.ctypes.byref(st))
+        self.last_stats = st.as_dict()
+        if rc == nat.TDA_ERR_NOT_CONVERGED:
+            raise DavidsonError("Davidson Algorithm did not converge!")
+        self.ctx.check(rc)
+        return w, x
