@@ -7,7 +7,12 @@ set-up, Davidson (low-precision products, then the FP64 continuation) and the
 outputs (omega, X, q_trans, dipoles, f) back on the host.
 
     python bench.py [--steps K] [--warmup W] [--prec fp16|bf16|fp32|fp64]
-                    [--operator ao|charges]
+                    [--operator ao|charges] [--workload fmo]
+
+--workload fmo: the batched FMO case instead -- 1024 seeded fragments (H2O,
+(H2O)2, C4..C40 polyenes) placed >= 3 A apart plus their 2048 nearest pairs
+as charge-transfer pairs, lowest 8 states each (clamped to ov), tol 1e-5, one
+batched call per step (default precision fp32).
 
 Prints one JSON line (see the end of main()).
 """
@@ -104,6 +109,142 @@ def cpu_baseline(inp, n_states, n_iter, n_products):
     }
 
 
+def _frag(d):
+    from tincr_amd.tda import Fragment
+    return Fragment(**{k: d[k] for k in ("n_atoms", "orbs_per_atom", "xyz", "occ_indices", "virt_indices",
+                                          "orbs", "s", "orbe", "gamma")})
+
+
+def fmo_batch(prec, mons, cts, n_states, max_iter, tol, operator, workers):
+    from tincr_amd.tda import TdaBatch
+    b = TdaBatch(prec, n_workers=workers, operator=operator)
+    for m in mons:
+        o, v = len(m["occ_indices"]), len(m["virt_indices"])
+        b.add_system(m["n_atoms"], m["orbs_per_atom"], m["xyz"], m["occ_indices"], m["virt_indices"], m["orbs"],
+                     m["s"], m["orbe"], m["gamma"], m["gamma_lr"], min(n_states, o * v), max_iter, tol)
+    for c in cts:
+        o, v = len(c["h"]["occ_indices"]), len(c["l"]["virt_indices"])
+        b.add_ct_pair(_frag(c["h"]), _frag(c["l"]), c["s_hl"], c["gamma_hl"], c["gamma_lr_hl"],
+                      min(n_states, o * v), max_iter, tol)
+    return b
+
+
+def fmo_cpu_baseline(mons, cts, n_states, max_iter, tol, budget_s=20.0, seed=0):
+    """The oracle (reference-faithful) solving a seeded random sample of the
+    items until `budget_s` seconds are spent; scaled to the whole batch by the
+    per-vector A.x flop estimate 2 N o v (o + v) x n_states of each item."""
+    from oracle import tda_oracle as orc
+    items = [("m", m) for m in mons] + [("c", c) for c in cts]
+
+    def dims(it):
+        kind, d = it
+        if kind == "m":
+            return d["n_atoms"], len(d["occ_indices"]), len(d["virt_indices"])
+        return d["h"]["n_atoms"] + d["l"]["n_atoms"], len(d["h"]["occ_indices"]), len(d["l"]["virt_indices"])
+
+    def ov_of(it):
+        _, o, v = dims(it)
+        return o * v
+
+    def est(it):
+        n, o, v = dims(it)
+        return (2.0 * n * o * v * (o + v) + 4.0 * n * o * v) * min(n_states, o * v)
+
+    order = np.random.default_rng(seed).permutation(len(items))
+    t0 = time.perf_counter()
+    done, est_done = 0, 0.0
+    for i in order:
+        kind, d = items[i]
+        if kind == "m":
+            q = orc.trans_charges(d["n_atoms"], d["orbs_per_atom"], d["orbs"], d["s"], d["occ_indices"],
+                                  d["virt_indices"])
+            o, v = len(d["occ_indices"]), len(d["virt_indices"])
+            eng = orc.SystemEngine(*q, d["gamma"], d["gamma_lr"],
+                                   orc.orbe_differences(d["orbe"][:o], d["orbe"][o:]), o, v, faithful=True)
+            xyz = d["xyz"]
+        else:
+            eng = orc.ct_pair_engine(d["h"], d["l"], d["s_hl"], d["gamma_hl"], d["gamma_lr_hl"], faithful=True)
+            o, v = eng.n_occ, eng.n_virt
+            xyz = np.concatenate([d["h"]["xyz"], d["l"]["xyz"]])
+        orc.run_tda(eng, xyz, min(n_states, o * v), max_iter, tol)
+        done += 1
+        est_done += est(items[i])
+        if time.perf_counter() - t0 > budget_s:
+            break
+    t = time.perf_counter() - t0
+    t_all = t * sum(est(it) for it in items) / est_done
+    n_st = sum(min(n_states, ov_of(it)) for it in items)
+    return {"value": n_st / t_all, "unit": "states/s", "cores": 1, "kind": "port",
+            "sample": (f"numpy FP64 oracle (reference-faithful engines, one thread) fully solved {done} random "
+                       f"items of {len(items)} in {t:.1f} s; scaled by the items' A.x flop estimate to "
+                       f"{t_all:.0f} s for the batch")}
+
+
+def main_fmo(args):
+    from tincr_amd import systems as sy
+    t0 = time.perf_counter()
+    frags = sy.fmo_fragments(1024, args.seed)
+    pairs = sy.fmo_pairs(frags, 2048)
+    mons, cts = sy.fmo_inputs(frags, pairs, args.seed)
+    t_gen = time.perf_counter() - t0
+    n = 8
+    prec = args.prec if args.prec_given else "fp32"
+    tol, max_iter = args.tol, 150
+
+    def run(p):
+        b = fmo_batch(p, mons, cts, n, max_iter, tol, args.operator, args.workers)
+        t = time.perf_counter()
+        res = b.run()
+        t = time.perf_counter() - t
+        b.close()
+        return t, res
+
+    for _ in range(max(args.warmup, 0)):
+        run(prec)
+    t64, r64 = run("fp64")
+    sync()
+    times, res = [], None
+    for _ in range(args.steps):
+        t, res = run(prec)
+        times.append(t)
+    elapsed = sum(times)
+    n_states = sum(len(r["ci_eigenvalues"]) for r in res)
+    value = n_states / (elapsed / args.steps)
+    dw = max(float(np.abs(a["ci_eigenvalues"] - c["ci_eigenvalues"]).max()) for a, c in zip(res, r64))
+    df = max(float(np.abs(a["oscillator_strengths"] - c["oscillator_strengths"]).max()) for a, c in zip(res, r64))
+    tot_ms, tot_fl, tot_n = np.zeros((2, 2)), np.zeros((2, 2)), np.zeros((2, 2))
+    for r in res:
+        tot_ms += np.array(r["stats"]["exch_ms"])
+        tot_fl += np.array(r["stats"]["exch_flops"])
+        tot_n += np.array(r["stats"]["exch_launches"])
+    si, pi = np.unravel_index(np.argmax(tot_ms), tot_ms.shape)
+    kprec = "fp64" if pi == 0 else prec
+    achieved = tot_fl[si, pi] / (tot_ms[si, pi] * 1e-3) / 1e12 if tot_ms[si, pi] > 0 else 0.0
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "states/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": f"{prec}+fp64" if prec != "fp64" else "fp64",
+        "data": ("synthetic: seeded fragment geometries, model AO overlap, seeded S-orthonormal orbitals, real "
+                 "gamma / gamma_lr"),
+        "config": {"workload": (f"batched FMO: {len(mons)} fragments (H2O, (H2O)2, C4-C40 polyenes, >= 3 A "
+                                f"apart) + {len(cts)} nearest CT pairs, lowest {n} states each (clamped to ov), "
+                                f"tol {tol}"),
+                   "precision": prec, "operator": args.operator, "workers": args.workers,
+                   "n_items": len(res), "n_states": n_states},
+        "fp64": {"states_per_s": round(n_states / t64, 3), "ms_per_batch": round(1e3 * t64, 1)},
+        "speedup_vs_fp64": round(value / (n_states / t64), 3),
+        "max_abs_domega_meV": dw * HARTREE_TO_EV * 1e3, "max_abs_df": df,
+        "input_generation_s": round(t_gen, 1),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[kprec],
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[kprec], 5), "traffic": None,
+                     "kernel": KERNEL_NAMES[args.operator][si] + f" ({kprec} MFMA), summed over fragments",
+                     "launches": int(tot_n[si, pi])},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = fmo_cpu_baseline(mons, cts, n, max_iter, tol)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,11 +258,16 @@ def main():
     ap.add_argument("--max-iter", type=int, default=450)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="single", choices=["single", "fmo"])
+    ap.add_argument("--workers", type=int, default=8)
     args = ap.parse_args()
+    args.prec_given = any(a.startswith("--prec") for a in sys.argv[1:])
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding is a design (DESIGN.md), not a measured path", file=sys.stderr)
         raise SystemExit(2)
+    if args.workload == "fmo":
+        return main_fmo(args)
 
     mol, inp = build_inputs(args.system, args.seed)
     n = args.states

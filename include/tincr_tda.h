@@ -162,6 +162,44 @@ int tda_precondition(tda_ctx* ctx, const double* r, double w, double* out);
 int tda_run(tda_ctx* ctx, int n_roots, int max_iter, double tolerance, double* omega, double* X, double* q_trans,
             double* tr_dipoles, double* f, tda_stats* stats);
 
+/* ------------------------------------------------------------------------
+ * Batched fragments: the FMO monomer loop (src/fmo/lcmo/basis.rs:462-465,
+ * prepare_tda + run_tda per monomer) and the CT-pair loop (basis.rs:172-238,
+ * run_ct_tda per pair) in one call.  Worker threads, one context and HIP
+ * stream each, take the fragments largest estimated cost first; each fragment
+ * converges on its own.  Input arrays are read during tda_batch_run only and
+ * must stay valid until it returns; outputs are written there.
+ * ------------------------------------------------------------------------ */
+typedef struct tda_batch tda_batch;
+
+int tda_batch_create(int device, int n_workers, size_t hbm_budget_bytes, int precision, tda_batch** out);
+void tda_batch_destroy(tda_batch* batch);
+const char* tda_batch_last_error(const tda_batch* batch);
+/* operator form of the monomer solves (TDA_OP_*); CT pairs use the charges form */
+int tda_batch_set_operator(tda_batch* batch, int op);
+/* A monomer (tda_set_system arguments) solved as run_tda(n_roots, max_iter,
+ * tol) with tda_run's outputs.  Returns the item index (>= 0) or -status. */
+int tda_batch_add_system(tda_batch* batch, int n_atoms, int n_ao, const int32_t* orbs_per_atom, const double* orbs,
+                         const double* s, const double* orbe, int occ_begin, int n_occ, int virt_begin, int n_virt,
+                         const double* gamma, const double* gamma_lr, const double* xyz, int n_roots, int max_iter,
+                         double tolerance, double* omega, double* X, double* q_trans, double* tr_dipoles, double* f);
+/* A CT pair (tda_set_ct_pair arguments) solved as run_ct_tda(n_roots,
+ * max_iter, tol).  Returns the item index (>= 0) or -status. */
+int tda_batch_add_ct_pair(tda_batch* batch, int n_atoms_h, int n_ao_h, const int32_t* orbs_per_atom_h,
+                          const double* orbs_h, const double* s_h, const double* orbe_h, int occ_begin_h, int n_occ_h,
+                          int n_atoms_l, int n_ao_l, const int32_t* orbs_per_atom_l, const double* orbs_l,
+                          const double* s_l, const double* orbe_l, int virt_begin_l, int n_virt_l, const double* s_hl,
+                          const double* gamma_h, const double* gamma_l, const double* gamma_hl,
+                          const double* gamma_lr_hl, const double* xyz, int n_roots, int max_iter, double tolerance,
+                          double* omega, double* X, double* q_trans, double* tr_dipoles, double* f);
+/* Forget the added items. */
+int tda_batch_clear(tda_batch* batch);
+/* Solves every item.  status [n_items] (optional) receives each item's
+ * tda_run status, stats [n_items] (optional) its statistics.  Returns TDA_OK,
+ * TDA_ERR_NOT_CONVERGED if some item did not converge (outputs hold its last
+ * Ritz pairs), or the first other error. */
+int tda_batch_run(tda_batch* batch, int32_t* status, tda_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

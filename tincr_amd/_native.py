@@ -26,6 +26,8 @@ EXPORTS = [
     "tda_ctx_create", "tda_ctx_destroy", "tda_last_error", "tda_set_precision", "tda_set_operator",
     "tda_trans_charges", "tda_orbe_differences", "tda_set_system", "tda_set_ct_pair", "tda_set_dense", "tda_prepare",
     "tda_clear_charges", "tda_get_charges", "tda_compute_products", "tda_precondition", "tda_run",
+    "tda_batch_create", "tda_batch_destroy", "tda_batch_last_error", "tda_batch_set_operator",
+    "tda_batch_add_system", "tda_batch_add_ct_pair", "tda_batch_clear", "tda_batch_run",
 ]
 
 
@@ -96,8 +98,20 @@ def load():
     lib.tda_compute_products.argtypes = [P, D, i, D]
     lib.tda_precondition.argtypes = [P, D, ctypes.c_double, D]
     lib.tda_run.argtypes = [P, i, i, ctypes.c_double, D, D, D, D, D, ctypes.POINTER(TdaStats)]
+    lib.tda_batch_create.argtypes = [i, i, ctypes.c_size_t, i, ctypes.POINTER(P)]
+    lib.tda_batch_destroy.argtypes = [P]
+    lib.tda_batch_destroy.restype = None
+    lib.tda_batch_last_error.argtypes = [P]
+    lib.tda_batch_last_error.restype = ctypes.c_char_p
+    lib.tda_batch_set_operator.argtypes = [P, i]
+    lib.tda_batch_add_system.argtypes = [P, i, i, I32, D, D, D, i, i, i, i, D, D, D, i, i, ctypes.c_double,
+                                         D, D, D, D, D]
+    lib.tda_batch_add_ct_pair.argtypes = [P, i, i, I32, D, D, D, i, i, i, i, I32, D, D, D, i, i, D, D, D, D, D, D,
+                                          i, i, ctypes.c_double, D, D, D, D, D]
+    lib.tda_batch_clear.argtypes = [P]
+    lib.tda_batch_run.argtypes = [P, I32, ctypes.POINTER(TdaStats)]
     for name in EXPORTS:
-        if name not in ("tda_ctx_destroy", "tda_last_error"):
+        if name not in ("tda_ctx_destroy", "tda_last_error", "tda_batch_destroy", "tda_batch_last_error"):
             getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib

@@ -224,19 +224,32 @@ def tda_inputs(mol, seed=0, lc=True):
 # ---------------------------------------------------------------------------
 # charge-transfer pair inputs
 # ---------------------------------------------------------------------------
-def model_overlap(mol, seed=0, beta=0.3, sigma=1.6):
-    """A symmetric positive definite AO overlap with unit diagonal whose
-    off-diagonal elements decay with distance: S = (1 - beta) I + beta K with
-    the Gaussian kernel K_{mu,nu} = exp(-|p_mu - p_nu|^2 / (2 sigma^2)) over
-    AO centres p_mu (atom position + a seeded 0.5 bohr displacement).  Its
-    eigenvalues are >= 1 - beta.  A stand-in for the Slater-Koster overlap
-    where only its structure (SPD, decaying inter-fragment block) matters."""
+def ao_centres(mol, seed=0):
+    """Model AO centres: each atom's position repeated over its AOs, each
+    displaced by 0.5 bohr in a seeded random direction."""
     rng = np.random.default_rng(seed + 2000)
     centres = np.repeat(mol.xyz, mol.orbs_per_atom, axis=0)
     d = rng.standard_normal(centres.shape)
-    centres = centres + 0.5 * d / np.linalg.norm(d, axis=1)[:, None]
-    r2 = ((centres[:, None, :] - centres[None, :, :]) ** 2).sum(-1)
-    return (1.0 - beta) * np.eye(len(centres)) + beta * np.exp(-r2 / (2.0 * sigma * sigma))
+    return centres + 0.5 * d / np.linalg.norm(d, axis=1)[:, None]
+
+
+def overlap_block(centres_a, centres_b, beta=0.3, sigma=1.6):
+    """beta * exp(-|p_mu - p_nu|^2 / (2 sigma^2)): the off-diagonal part of
+    ``model_overlap`` between two sets of AO centres."""
+    r2 = ((centres_a[:, None, :] - centres_b[None, :, :]) ** 2).sum(-1)
+    return beta * np.exp(-r2 / (2.0 * sigma * sigma))
+
+
+def model_overlap(mol, seed=0, beta=0.3, sigma=1.6, centres=None):
+    """A symmetric positive definite AO overlap with unit diagonal whose
+    off-diagonal elements decay with distance: S = (1 - beta) I + beta K with
+    the Gaussian kernel K_{mu,nu} = exp(-|p_mu - p_nu|^2 / (2 sigma^2)) over
+    the AO centres p_mu of ``ao_centres``.  Its eigenvalues are >= 1 - beta.
+    A stand-in for the Slater-Koster overlap where only its structure (SPD,
+    decaying inter-fragment block) matters."""
+    if centres is None:
+        centres = ao_centres(mol, seed)
+    return (1.0 - beta) * np.eye(len(centres)) + overlap_block(centres, centres, beta, sigma)
 
 
 def fragment_orbitals(s, n_occ, seed=0, gap=0.1):
@@ -278,3 +291,115 @@ def ct_pair_inputs(mol, n_atoms_h, seed=0, lc=True):
                         occ_indices=list(range(frag.n_occ)), virt_indices=list(range(frag.n_occ, frag.n_ao)),
                         orbs=c, s=sf, orbe=orbe, gamma=gamma[at, at].copy())
     return out
+
+
+# ---------------------------------------------------------------------------
+# batched FMO workload: seeded fragments in a box, nearest pairs
+# ---------------------------------------------------------------------------
+FMO_KINDS = ["H2O", "(H2O)2"] + [f"C{n}H{n + 2}" for n in range(4, 41, 2)]
+
+
+def _fragment(kind, seed):
+    if kind == "H2O":
+        return water_cluster(1, seed=seed)
+    if kind == "(H2O)2":
+        return water_cluster(2, seed=seed)
+    return polyene(int(kind[1:kind.index("H")]), seed=seed)
+
+
+def fmo_fragments(n_frag=1024, seed=0, min_dist=3.0, box=None):
+    """`n_frag` fragments drawn uniformly from FMO_KINDS (H2O, (H2O)2,
+    C_nH_{n+2} for n = 4, 6, ..., 40), each with a uniform random orientation,
+    placed by rejection sampling in a cube so that every interatomic distance
+    between fragments is >= `min_dist` A.  Returns a list of Molecules (bohr)."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    kinds = [FMO_KINDS[k] for k in rng.integers(0, len(FMO_KINDS), n_frag)]
+    if box is None:
+        n_atoms = sum(_fragment(k, 0).n_atoms for k in set(kinds)) / len(set(kinds)) * n_frag
+        box = (100.0 * n_atoms) ** (1.0 / 3.0)
+    tree, recent, frags = None, [], []
+    for i, kind in enumerate(kinds):
+        mol = _fragment(kind, seed * 100003 + i)
+        local = mol.xyz * BOHR_TO_ANGS
+        local = local - local.mean(axis=0)
+        r = np.vstack(recent) if recent else None
+        for _ in range(100000):
+            pos = local @ _random_rotation(rng).T + rng.uniform(0.0, box, 3)
+            if r is not None:
+                if (((pos[:, None, :] - r[None, :, :]) ** 2).sum(-1) < min_dist ** 2).any():
+                    continue
+            if tree is not None and np.isfinite(tree.query(pos, distance_upper_bound=min_dist)[0]).any():
+                continue
+            break
+        else:
+            raise RuntimeError("fmo_fragments: box too crowded")
+        recent.append(pos)
+        frags.append(Molecule(mol.z, pos / BOHR_TO_ANGS, f"{kind} #{i}"))
+        if len(recent) == 32:
+            tree = cKDTree(np.vstack([f.xyz for f in frags]) * BOHR_TO_ANGS)
+            recent = []
+    return frags
+
+
+def fmo_pairs(frags, n_pairs=2048, cutoff=8.0):
+    """The `n_pairs` fragment pairs (I < J) with the smallest minimum
+    interatomic distance (A), nearest first (ties by index): list of
+    (I, J, distance).  The search radius starts at `cutoff` A and grows until
+    it holds enough pairs."""
+    from scipy.spatial import cKDTree
+    pts = np.vstack([m.xyz for m in frags]) * BOHR_TO_ANGS
+    owner = np.concatenate([np.full(m.n_atoms, i) for i, m in enumerate(frags)])
+    tree = cKDTree(pts)
+    while True:
+        ab = tree.query_pairs(cutoff, output_type="ndarray")
+        fi, fj = owner[ab[:, 0]], owner[ab[:, 1]]
+        if len(np.unique(np.minimum(fi, fj)[fi != fj] * len(frags) + np.maximum(fi, fj)[fi != fj])) >= n_pairs:
+            break
+        cutoff *= 1.5
+    keep = fi != fj
+    ab, fi, fj = ab[keep], fi[keep], fj[keep]
+    d = np.linalg.norm(pts[ab[:, 0]] - pts[ab[:, 1]], axis=1)
+    lo, hi = np.minimum(fi, fj), np.maximum(fi, fj)
+    key = lo * len(frags) + hi
+    order = np.lexsort((d, key))
+    key, d = key[order], d[order]
+    first = np.r_[True, key[1:] != key[:-1]]
+    key, d = key[first], d[first]
+    rank = np.lexsort((key, d))
+    n = len(frags)
+    return [(int(key[r] // n), int(key[r] % n), float(d[r])) for r in rank[:n_pairs]]
+
+
+def gamma_block(z_a, xyz_a, z_b, xyz_b, r_lr=0.0, hubbard=None):
+    """gamma between two disjoint atom sets [N_a, N_b] (erf(c r)/r, r > 0)."""
+    sa, sb = _sigmas(np.asarray(z_a), hubbard), _sigmas(np.asarray(z_b), hubbard)
+    c = 1.0 / np.sqrt(2.0 * (sa[:, None] ** 2 + sb[None, :] ** 2 + 0.5 * r_lr ** 2))
+    d = np.linalg.norm(xyz_a[:, None, :] - xyz_b[None, :, :], axis=-1)
+    return _erf(c * d) / d
+
+
+def fmo_inputs(frags, pairs, seed=0, lc=True):
+    """Monomer inputs (model overlap, S-orthonormal seeded orbitals, real
+    gamma / gamma_lr) for every fragment and CT-pair inputs (hole on I,
+    particle on J) for every pair.  Returns (monomers, ct_pairs): monomer
+    dicts with tda_inputs' keys plus ``centres``; pair dicts with h, l (the
+    monomer dicts), s_hl, gamma_hl, gamma_lr_hl."""
+    mons = []
+    for i, mol in enumerate(frags):
+        centres = ao_centres(mol, seed * 100003 + i)
+        s = model_overlap(mol, centres=centres)
+        c, orbe = fragment_orbitals(s, mol.n_occ, seed * 100003 + i)
+        mons.append(dict(n_atoms=mol.n_atoms, orbs_per_atom=mol.orbs_per_atom, xyz=mol.xyz, z=mol.z,
+                         occ_indices=list(range(mol.n_occ)), virt_indices=list(range(mol.n_occ, mol.n_ao)),
+                         orbs=c, s=s, orbe=orbe, centres=centres,
+                         gamma=gamma_atomwise(mol.z, mol.xyz, 0.0),
+                         gamma_lr=gamma_atomwise(mol.z, mol.xyz, LONG_RANGE_RADIUS) if lc else None))
+    cts = []
+    for i, j, _ in pairs:
+        h, l = mons[i], mons[j]
+        cts.append(dict(h=h, l=l, s_hl=overlap_block(h["centres"], l["centres"]),
+                        gamma_hl=gamma_block(h["z"], h["xyz"], l["z"], l["xyz"], 0.0),
+                        gamma_lr_hl=gamma_block(h["z"], h["xyz"], l["z"], l["xyz"], LONG_RANGE_RADIUS)
+                        if lc else None))
+    return mons, cts

@@ -288,3 +288,100 @@ class DenseEngine:
             raise DavidsonError("Davidson Algorithm did not converge!")
         self.ctx.check(rc)
         return w, x
+
+
+class TdaBatch:
+    """Many fragments solved in one call: the FMO monomer loop
+    (fmo/lcmo/basis.rs:462-465: ``prepare_tda`` + ``run_tda`` per monomer) and
+    the CT-pair loop (basis.rs:172-238: ``run_ct_tda`` per pair).
+
+    ``add_system`` takes ``TdaSystem``'s arguments, ``add_ct_pair`` those of
+    ``TdaCTPair``; both take ``n_roots, max_iter, tolerance`` of the solve.
+    ``run`` returns one dict per item (``ci_eigenvalues``, ``ci_coefficients``,
+    ``q_trans``, ``tr_dipoles``, ``oscillator_strengths``, ``stats``,
+    ``status``) and raises ``DavidsonError`` if an item did not converge
+    (after every item has been solved)."""
+
+    def __init__(self, precision="fp64", n_workers=8, device=0, hbm_budget_bytes=0, operator="ao"):
+        self.lib = nat.load()
+        self.handle = nat.ctypes.c_void_p()
+        rc = self.lib.tda_batch_create(device, n_workers, hbm_budget_bytes, nat.PREC[precision],
+                                       nat.ctypes.byref(self.handle))
+        if rc != nat.TDA_OK:
+            raise nat.TdaError(rc, "tda_batch_create failed")
+        self._check(self.lib.tda_batch_set_operator(self.handle, nat.OPERATOR[operator]))
+        self.items = []
+
+    def _check(self, rc):
+        if rc != nat.TDA_OK:
+            raise nat.TdaError(rc, self.lib.tda_batch_last_error(self.handle).decode())
+
+    def close(self):
+        if self.handle:
+            self.lib.tda_batch_destroy(self.handle)
+            self.handle = nat.ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _outputs(self, n_atoms, ov, n_roots):
+        return dict(ci_eigenvalues=np.empty(n_roots), ci_coefficients=np.empty((ov, n_roots)),
+                    q_trans=np.empty((n_atoms, n_roots)), tr_dipoles=np.empty((3, n_roots)),
+                    oscillator_strengths=np.empty(n_roots))
+
+    def _add(self, idx, keep, out):
+        if idx < 0:
+            raise nat.TdaError(-idx, self.lib.tda_batch_last_error(self.handle).decode())
+        self.items.append((keep, out))
+        return idx
+
+    def add_system(self, n_atoms, orbs_per_atom, xyz, occ_indices, virt_indices, orbs, s, orbe, gamma,
+                   gamma_lr, n_roots, max_iter, tolerance):
+        ob, o, vb, v = _ranges(occ_indices, virt_indices)
+        keep = dict(opa=np.ascontiguousarray(orbs_per_atom, dtype=np.int32),
+                    xyz=nat.f64(np.asarray(xyz).reshape(n_atoms, 3)), orbs=nat.f64(orbs), s=nat.f64(s),
+                    orbe=nat.f64(np.ravel(orbe)), gamma=nat.f64(gamma), gamma_lr=nat.f64(gamma_lr))
+        out = self._outputs(n_atoms, o * v, n_roots)
+        idx = self.lib.tda_batch_add_system(
+            self.handle, n_atoms, keep["orbs"].shape[0], nat.iptr(keep["opa"]), nat.dptr(keep["orbs"]),
+            nat.dptr(keep["s"]), nat.dptr(keep["orbe"]), ob, o, vb, v, nat.dptr(keep["gamma"]),
+            nat.dptr(keep["gamma_lr"]), nat.dptr(keep["xyz"]), n_roots, max_iter, float(tolerance),
+            nat.dptr(out["ci_eigenvalues"]), nat.dptr(out["ci_coefficients"]), nat.dptr(out["q_trans"]),
+            nat.dptr(out["tr_dipoles"]), nat.dptr(out["oscillator_strengths"]))
+        return self._add(idx, keep, out)
+
+    def add_ct_pair(self, h, l, s_hl, gamma_hl, gamma_lr_hl, n_roots, max_iter, tolerance):
+        ob, o, _, _ = _ranges(h.occ_indices, h.virt_indices)
+        _, _, vb, v = _ranges(l.occ_indices, l.virt_indices)
+        keep = dict(h=h, l=l, s_hl=nat.f64(s_hl), gamma_hl=nat.f64(gamma_hl), gamma_lr_hl=nat.f64(gamma_lr_hl),
+                    xyz=nat.f64(np.concatenate([h.xyz, l.xyz])))
+        n_atoms = h.n_atoms + l.n_atoms
+        out = self._outputs(n_atoms, o * v, n_roots)
+        idx = self.lib.tda_batch_add_ct_pair(
+            self.handle, h.n_atoms, h.orbs.shape[0], nat.iptr(h.orbs_per_atom), nat.dptr(h.orbs), nat.dptr(h.s),
+            nat.dptr(h.orbe), ob, o, l.n_atoms, l.orbs.shape[0], nat.iptr(l.orbs_per_atom), nat.dptr(l.orbs),
+            nat.dptr(l.s), nat.dptr(l.orbe), vb, v, nat.dptr(keep["s_hl"]), nat.dptr(h.gamma), nat.dptr(l.gamma),
+            nat.dptr(keep["gamma_hl"]), nat.dptr(keep["gamma_lr_hl"]), nat.dptr(keep["xyz"]), n_roots, max_iter,
+            float(tolerance), nat.dptr(out["ci_eigenvalues"]), nat.dptr(out["ci_coefficients"]),
+            nat.dptr(out["q_trans"]), nat.dptr(out["tr_dipoles"]), nat.dptr(out["oscillator_strengths"]))
+        return self._add(idx, keep, out)
+
+    def run(self):
+        n = len(self.items)
+        status = np.zeros(n, dtype=np.int32)
+        stats = (nat.TdaStats * max(n, 1))()
+        rc = self.lib.tda_batch_run(self.handle, nat.iptr(status), stats)
+        results = []
+        for i, (_, out) in enumerate(self.items):
+            r = dict(out)
+            r["stats"] = stats[i].as_dict()
+            r["status"] = int(status[i])
+            results.append(r)
+        if rc == nat.TDA_ERR_NOT_CONVERGED:
+            bad = [i for i in range(n) if status[i] == nat.TDA_ERR_NOT_CONVERGED]
+            raise DavidsonError(f"Davidson Algorithm did not converge! (items {bad})")
+        self._check(rc)
+        return results
