@@ -163,6 +163,35 @@ int tda_run(tda_ctx* ctx, int n_roots, int max_iter, double tolerance, double* o
             double* tr_dipoles, double* f, tda_stats* stats);
 
 /* ------------------------------------------------------------------------
+ * Adjoint (Hellmann-Feynman) derivatives of the excitation energies w_k =
+ * x_k^T A x_k of a System/Monomer with respect to the operator's inputs, for
+ * the converged CI vectors.  No reference function computes these directly;
+ * they are the gamma / charge terms of the LC-TDA gradient
+ * (src/gradients/excited_state.rs:259-487) at frozen orbitals.
+ * X: [ov, n_states] (ci_coefficients layout) or NULL for the last tda_run's
+ * vectors.  Outputs (any may be NULL):
+ *   d_gamma    [n_states, n_atoms, n_atoms]  dw/dgamma_AB    = 2 u_A u_B, u = q_ov x
+ *   d_gamma_lr [n_states, n_atoms, n_atoms]  dw/dgamma_lr,AB = -sum_ij q_oo^A_ij (x q_vv^B x^T)_ij
+ *   c_qov      [n_states, n_atoms]           dw/dq_ov[A, ia] = c_qov[k][A] * x_ia  (= 4 (gamma u)_A)
+ *   d_qoo      [n_states, n_atoms, n_occ^2]  dw/dq_oo^A_ij   = -sum_B gamma_lr,AB (x q_vv^B x^T)_ij
+ *   d_qvv      [n_states, n_atoms, n_virt^2] dw/dq_vv^B_ab   = -(x^T G^B x)_ab, G^B = sum_A gamma_lr,BA q_oo^A
+ * Entries of gamma / gamma_lr are treated as independent (not symmetrised).
+ * ------------------------------------------------------------------------ */
+int tda_adjoint(tda_ctx* ctx, int n_states, const double* X, double* d_gamma, double* d_gamma_lr, double* c_qov,
+                double* d_qoo, double* d_qvv);
+
+/* Chains the gamma terms to nuclear coordinates at frozen charges.
+ * dgamma_dr / dgamma_lr_dr [3 n_atoms, n_atoms, n_atoms] in the layout of
+ * gamma_gradients_atomwise (src/scc/gamma_approximation.rs:256-282): entry
+ * [3i + d][i][j] = d gamma_ij / d R_i,d, the other rows zero.  Since gamma is
+ * symmetric, gamma_ji moves with R_i too:
+ *   grad[k][c] = sum_AB g1[c][A][B] (dw_k/dgamma_AB + dw_k/dgamma_BA)
+ *              + the same with gamma_lr (dgamma_lr_dr may be NULL).
+ * grad: [n_states, 3 n_atoms]. */
+int tda_adjoint_nuclear(tda_ctx* ctx, int n_states, const double* X, const double* dgamma_dr,
+                        const double* dgamma_lr_dr, double* grad);
+
+/* ------------------------------------------------------------------------
  * Batched fragments: the FMO monomer loop (src/fmo/lcmo/basis.rs:462-465,
  * prepare_tda + run_tda per monomer) and the CT-pair loop (basis.rs:172-238,
  * run_ct_tda per pair) in one call.  Worker threads, one context and HIP

@@ -21,6 +21,10 @@ It restates, in FP64 numpy, the reference's algorithm (tincr, Rust):
 * ``ct_q_ov``              src/excited_states/tda/ct_pair.rs:78-123
 * ``ct_pair_engine``       src/excited_states/tda/ct_pair.rs:12-76 +
   solvers/mod.rs:165-274
+* ``frozen_charge_gradient`` the gamma terms of d omega / dR
+* ``hf_derivatives``       d omega / d(gamma, gamma_lr, q) by Hellmann-Feynman
+  on the dense A of tda/mod.rs:180-215 (no reference function; the gamma and
+  charge terms of src/gradients/excited_state.rs:259-487 at frozen orbitals)
 
 Pinning (each checked in tests/test_oracle.py):
 
@@ -41,6 +45,12 @@ Pinning (each checked in tests/test_oracle.py):
   transition_charges.rs vanish on the other fragment's atoms and reduce to the
   CT formulas, so the pair A must equal that monomer A (n_atoms_h != n_atoms_l,
   so a transposed gamma_lr block cannot pass).
+* frozen_charge_gradient: central differences (step 1e-4 bohr) of the dense
+  eigenvalues with gamma / gamma_lr recomputed at displaced geometries.
+* hf_derivatives: central finite differences of the dense eigenvalues of A
+  with one input perturbed along a random direction (tests/test_oracle.py);
+  A is linear in gamma / gamma_lr and bilinear in the charges, so a dropped
+  factor 2, a sign or a transposed P^B fails at 1e-6 relative.
 """
 import numpy as np
 
@@ -376,6 +386,41 @@ def tda_full_matrix(engine, xyz):
     q_trans = engine.q_ov @ x
     mu = mulliken_dipoles(q_trans, xyz)
     return dict(omega=w, X=x, q_trans=q_trans, tr_dipoles=mu, f=oscillator_strength(w, mu))
+
+
+def hf_derivatives(engine, x):
+    """Hellmann-Feynman derivatives of w = x^T A x (x normalised, A of
+    SystemEngine.dense, tda/mod.rs:180-215) w.r.t. the operator's inputs,
+    entries treated as independent.  Returns dict d_gamma [N, N],
+    d_gamma_lr [N, N], d_qov [N, ov], d_qoo [N, o*o], d_qvv [N, v*v]."""
+    o, v = engine.n_occ, engine.n_virt
+    q_ov, gam = engine.q_ov, engine.gamma
+    u = q_ov @ x
+    out = dict(d_gamma=2.0 * np.outer(u, u), d_qov=4.0 * np.outer(gam @ u, x))
+    if engine.gamma_lr is not None:
+        xm = x.reshape(o, v)
+        n_b = engine.q_vv.shape[0]
+        qvv = engine.q_vv.reshape(n_b, v, v)
+        p = np.einsum("ia,bab,jb->bij".replace("bab", "Bab").replace("->bij", "->Bij"), xm, qvv, xm)
+        p = p.reshape(n_b, o * o)                     # P^B_ij = (x q_vv^B x^T)_ij
+        out["d_gamma_lr"] = -engine.q_oo @ p.T        # [A, B]
+        out["d_qoo"] = -engine.gamma_lr @ p           # [A, ij]
+        g = (engine.gamma_lr @ engine.q_oo).reshape(-1, o, o)   # G^B = sum_A gamma_lr[B, A] q_oo^A
+        out["d_qvv"] = -np.einsum("ia,Bij,jb->Bab", xm, g, xm).reshape(-1, v * v)
+    return out
+
+
+def frozen_charge_gradient(engine, x, dgamma, dgamma_lr):
+    """d omega / dR at frozen orbitals and charges: the gamma terms of the
+    LC-TDA gradient.  dgamma[_lr]: [3N, N, N] in the layout of
+    gamma_gradients_atomwise (gamma_approximation.rs:256-282, row i of
+    d gamma / d R_i only), hence the symmetrised derivatives."""
+    d = hf_derivatives(engine, x)
+    n3 = dgamma.shape[0]
+    g = dgamma.reshape(n3, -1) @ (d["d_gamma"] + d["d_gamma"].T).ravel()
+    if engine.gamma_lr is not None and dgamma_lr is not None:
+        g = g + dgamma_lr.reshape(n3, -1) @ (d["d_gamma_lr"] + d["d_gamma_lr"].T).ravel()
+    return g
 
 
 def generate_diagonal_dominant(rng, dim, sparsity):

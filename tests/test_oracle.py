@@ -247,3 +247,70 @@ def test_ct_pair_davidson_vs_dense():
     res = orc.run_tda(eng, xyz, 4, 100, 1e-7)
     dense = orc.tda_full_matrix(eng, xyz)
     assert np.abs(res["omega"] - dense["omega"][:4]).max() < 1e-10
+
+
+@pytest.mark.parametrize("mol", ["h2o", "benzene"])
+def test_hf_derivatives_vs_finite_differences(mol):
+    """d omega_k / d(gamma, gamma_lr, q_ov, q_oo, q_vv) along random
+    directions vs central differences of the dense eigenvalues (step 1e-6;
+    benzene state 2 lies 1.4 mHa below state 3, so larger steps see the
+    curvature).
+    The directions keep A symmetric (symmetric gamma, gamma_lr and per-atom
+    q_oo / q_vv blocks), as every input A is built from is."""
+    m = molecule(mol)
+    eng = _engine(m)
+    w, x = np.linalg.eigh(eng.dense())
+    rng = np.random.default_rng(7)
+    h = 1e-6
+    for k in (0, 2):
+        d = orc.hf_derivatives(eng, x[:, k])
+        for name, key in (("gamma", "d_gamma"), ("gamma_lr", "d_gamma_lr"), ("q_ov", "d_qov"),
+                          ("q_oo", "d_qoo"), ("q_vv", "d_qvv")):
+            base = getattr(eng, name)
+            e = rng.standard_normal(base.shape)
+            if name in ("gamma", "gamma_lr"):
+                e = e + e.T
+            elif name in ("q_oo", "q_vv"):
+                nb = e.shape[0]
+                dim = int(round(np.sqrt(e.shape[1])))
+                e = e.reshape(nb, dim, dim)
+                e = (e + e.transpose(0, 2, 1)).reshape(nb, dim * dim)
+            vals = []
+            for sgn in (1.0, -1.0):
+                setattr(eng, name, base + sgn * h * e)
+                vals.append(np.linalg.eigh(eng.dense())[0][k])
+            setattr(eng, name, base)
+            fd = (vals[0] - vals[1]) / (2 * h)
+            an = float((d[key] * e).sum())
+            assert abs(fd - an) < 1e-6 * max(1.0, abs(fd)), (mol, k, name, fd, an)
+
+
+def test_frozen_charge_gradient_vs_finite_differences():
+    """Chain through gamma_gradients_atomwise's layout (row i of dgamma/dR_i)
+    vs central differences, step 1e-4 bohr, of the dense lowest two
+    eigenvalues of a (H2O)_4 cluster with gamma / gamma_lr recomputed."""
+    from tincr_amd import systems as sy
+    mol = sy.water_cluster(4, seed=0)
+    inp = sy.tda_inputs(mol, seed=0)
+    o, v = len(inp["occ_indices"]), len(inp["virt_indices"])
+    q = orc.trans_charges(inp["n_atoms"], inp["orbs_per_atom"], inp["orbs"], inp["s"], inp["occ_indices"],
+                          inp["virt_indices"])
+    omega = orc.orbe_differences(inp["orbe"][:o], inp["orbe"][o:])
+
+    def engine(xyz):
+        return orc.SystemEngine(*q, sy.gamma_atomwise(mol.z, xyz, 0.0),
+                                sy.gamma_atomwise(mol.z, xyz, sy.LONG_RANGE_RADIUS), omega, o, v)
+
+    w, x = np.linalg.eigh(engine(mol.xyz).dense())
+    dg = sy.gamma_gradients_atomwise(mol.z, mol.xyz, 0.0)
+    dglr = sy.gamma_gradients_atomwise(mol.z, mol.xyz, sy.LONG_RANGE_RADIUS)
+    h = 1e-4
+    for k in (0, 1):
+        g = orc.frozen_charge_gradient(engine(mol.xyz), x[:, k], dg, dglr)
+        for c in range(0, 3 * mol.n_atoms, 5):
+            vals = []
+            for sgn in (1.0, -1.0):
+                xyz = mol.xyz.copy()
+                xyz[c // 3, c % 3] += sgn * h
+                vals.append(np.linalg.eigh(engine(xyz).dense())[0][k])
+            assert abs((vals[0] - vals[1]) / (2 * h) - g[c]) < 1e-9

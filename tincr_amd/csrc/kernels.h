@@ -79,6 +79,13 @@ void launch_ao_add_coulomb(void* M, int dtype, int k, int n_ao, const int* atom,
 // Outputs the n lowest eigenvalues (ascending) and eigenvectors Y[r][0..m) (row per root).
 size_t eigh_work_doubles(int m);
 
+// Hellmann-Feynman Coulomb terms: dgam[k][A][B] = 2 u[A][k] u[B][k],
+// cq[k][A] = 4 w[A][k] (u, w [N][n]; cq may be NULL)
+void launch_hf_coulomb(const double* u, const double* w, int N, int n, double* dgam, double* cq, hipStream_t s);
+
+// S[k] = D[k] + D[k]^T for n matrices N x N
+void launch_sym_add(const double* D, int n, int N, double* S, hipStream_t s);
+
 // Mulliken transition dipoles mu [3][n] = xyz^T qt and oscillator strengths
 // f [n] = 2/3 w |mu|^2 (xyz [N][3], qt [N][n], w [n]; all device)
 void launch_dipoles(const double* xyz, const double* qt, const double* w, int N, int n, double* mu, double* f,

@@ -641,4 +641,41 @@ void launch_dipoles(const double* xyz, const double* qt, const double* w, int N,
   TDA_LAUNCH_CHECK();
 }
 
+// ---------------------------------------------------------------------------
+// Hellmann-Feynman Coulomb derivatives of w_k = x_k^T A x_k (A.x with the
+// singlet Coulomb 2 q_ov^T gamma q_ov):  u = q_ov x_k [N][n], w = gamma u,
+//   dgam[k][A][B] = d w_k / d gamma_AB = 2 u[A][k] u[B][k]
+//   cq[k][A]: d w_k / d q_ov[A, ia] = cq[k][A] x_k[ia],  cq = 4 w[A][k]
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) hf_coulomb_kernel(const double* __restrict__ u, const double* __restrict__ w,
+                                                         int N, int n, double* __restrict__ dgam,
+                                                         double* __restrict__ cq) {
+  const int A = blockIdx.x, k = blockIdx.y;
+  const double uA = u[(int64_t)A * n + k];
+  double* row = dgam + ((int64_t)k * N + A) * N;
+  for (int B = threadIdx.x; B < N; B += 256) row[B] = 2.0 * uA * u[(int64_t)B * n + k];
+  if (threadIdx.x == 0 && cq) cq[(int64_t)k * N + A] = 4.0 * w[(int64_t)A * n + k];
+}
+
+void launch_hf_coulomb(const double* u, const double* w, int N, int n, double* dgam, double* cq, hipStream_t s) {
+  if (N == 0 || n == 0) return;
+  hf_coulomb_kernel<<<dim3((unsigned)N, (unsigned)n), 256, 0, s>>>(u, w, N, n, dgam, cq);
+  TDA_LAUNCH_CHECK();
+}
+
+// S[k][A][B] = D[k][A][B] + D[k][B][A]  (k < n, A, B < N)
+__global__ void __launch_bounds__(256) sym_add_kernel(const double* __restrict__ D, int N,
+                                                      double* __restrict__ S) {
+  const int A = blockIdx.x, k = blockIdx.y;
+  const double* d = D + (int64_t)k * N * N;
+  double* o = S + ((int64_t)k * N + A) * N;
+  for (int B = threadIdx.x; B < N; B += 256) o[B] = d[(int64_t)A * N + B] + d[(int64_t)B * N + A];
+}
+
+void launch_sym_add(const double* D, int n, int N, double* S, hipStream_t s) {
+  if (n == 0 || N == 0) return;
+  sym_add_kernel<<<dim3((unsigned)N, (unsigned)n), 256, 0, s>>>(D, N, S);
+  TDA_LAUNCH_CHECK();
+}
+
 }  // namespace tda

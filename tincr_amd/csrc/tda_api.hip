@@ -98,6 +98,8 @@ struct tda_ctx {
   // stored as o = 1, v = dim; no preconditioner floor
   bool dense = false;
   DevBuf Hd, dxyz, post;
+  DevBuf adjP, adjQ, adjX, adjD;  // adjoint work buffers
+  int last_roots = 0;             // CI vectors of the last tda_run held in RZ
   double pre_thr = 1e-4;
   int64_t ovp = 0;
   std::vector<double> xyz, h_de;
@@ -146,7 +148,8 @@ struct tda_ctx {
                       &qvv_amax, &G_amax, &ws_gemm, &T, &xlp, &u, &uu, &vv32, &ax_tmp,
                       &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &P, &W, &norms, &npart, &idx, &count,
                       &gs_scratch, &eigw, &ao_atom, &Gt, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
-                      &Cl, &Sl, &SCl, &Shl, &aooff_l, &ctW, &ctZ, &Hd, &dxyz, &post})
+                      &Cl, &Sl, &SCl, &Shl, &aooff_l, &ctW, &ctZ, &Hd, &dxyz, &post,
+                      &adjP, &adjQ, &adjX, &adjD})
       b->release();
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -629,6 +632,99 @@ void products(tda_ctx* c, const double* X, int k, double* AX, int prec) {
 }
 
 // ---------------------------------------------------------------------------
+// Hellmann-Feynman derivatives of w_k = x_k^T A x_k for normalised x_k (the
+// converged CI vectors; A symmetric), with respect to the operator's inputs:
+//   d w_k / d gamma_AB      = 2 u_A u_B,              u = q_ov x_k
+//   d w_k / d q_ov[A, ia]   = 4 (gamma u)_A x_ia
+//   d w_k / d gamma_lr,AB   = -sum_ij q_oo^A_ij P^B_ij, P^B = x q_vv^B x^T  (o x o)
+//   d w_k / d q_oo^A_ij     = -sum_B gamma_lr,AB P^B_ij
+//   d w_k / d q_vv^B_ab     = -(x^T G^B x)_ab,         G^B = sum_A gamma_lr,BA q_oo^A
+// (x as the o x v matrix; the terms of A = diag(de) + 2 q_ov^T gamma q_ov - K
+// of solvers/mod.rs:307-360).  Xd: device [n][o][v_p].  Device outputs
+// dgam, dglr [n][N][N] and cq [n][N]; host outputs dqoo [n][N][o^2],
+// dqvv [n][N][v^2] (any may be NULL).
+// ---------------------------------------------------------------------------
+void copy_unpad_rows(const double* src, int rows, int cols, int ld, double* dst);
+
+void hf_derivatives(tda_ctx* c, const double* Xd, int n, double* dgam, double* dglr, double* cq, double* dqoo,
+                    double* dqvv) {
+  const int N = c->N, o = c->o, v = c->v, op = c->op, vp = c->vp;
+  const int64_t ovp = c->ovp, loo = (int64_t)op * op;
+  c->u.ensure((size_t)N * n * 8);
+  c->uu.ensure((size_t)N * n * 8);
+  GemmArgs gu = gargs(c->qov.p, Xd, c->u.p, N, n, ovp, plain(ovp), plain(1), plain(1), plain(ovp), plain(n),
+                      plain(1));
+  run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, gu);
+  launch_small_gemm(c->gamma.as<double>(), c->u.as<double>(), c->uu.as<double>(), N, n, N, c->stream);
+  if (dgam) launch_hf_coulomb(c->u.as<double>(), c->uu.as<double>(), N, n, dgam, cq, c->stream);
+  if (!c->has_lc || (!dglr && !dqoo && !dqvv)) return;
+  build_mo_charges(c);
+  const int64_t NBop = (int64_t)N * op;
+  c->T.ensure((size_t)vp * NBop * 8);
+  c->adjP.ensure((size_t)N * loo * 8);
+  std::vector<double> h;
+  for (int k = 0; k < n; ++k) {
+    const double* xk = Xd + (size_t)k * ovp;
+    // T[a][(B,j)] = sum_b q_vv^B[a][b] x[j][b]
+    launch_zero_runs(c->T.p, (int64_t)vp * N, op, o, op - o, 8, c->stream);
+    GemmArgs g1 = gargs(c->qvv.p, xk, c->T.p, v, o, vp, plain(vp), plain(1), plain(1), plain(vp), plain(NBop),
+                        plain(1));
+    g1.batch = N;
+    g1.a_batch = (int64_t)vp * vp;
+    g1.b_batch = 0;
+    g1.c_batch = op;
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g1);
+    // P[B][i][j] = sum_a T[a][(B,i)] x[j][a]   (rows i in [o, o_p) are zero)
+    launch_fill(c->adjP.as<double>(), 0.0, (int64_t)N * loo, c->stream);
+    GemmArgs g2 = gargs(c->T.p, xk, c->adjP.p, NBop, o, v, plain(1), plain(NBop), plain(1), plain(vp), plain(op),
+                        plain(1));
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, true, g2);
+    if (dglr) {
+      // dglr[k][A][B] = -sum_(ij) q_oo[A][(ij)] P[B][(ij)]
+      GemmArgs g3 = gargs(c->qoo.p, c->adjP.p, dglr + (size_t)k * N * N, N, N, loo, plain(loo), plain(1), plain(1),
+                          plain(loo), plain(N), plain(1));
+      g3.alpha = -1.0;
+      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g3);
+    }
+    if (dqoo) {
+      // -gamma_lr P  [N][o_p^2]
+      c->adjQ.ensure((size_t)N * loo * 8);
+      GemmArgs g4 = gargs(c->gamma_lr.p, c->adjP.p, c->adjQ.p, N, loo, N, plain(N), plain(1), plain(loo), plain(1),
+                          plain(loo), plain(1));
+      g4.alpha = -1.0;
+      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, g4);
+      h.resize((size_t)N * loo);
+      TDA_HIP(hipMemcpyAsync(h.data(), c->adjQ.p, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipStreamSynchronize(c->stream));
+      for (int A = 0; A < N; ++A)
+        copy_unpad_rows(h.data() + (size_t)A * loo, o, o, op, dqoo + ((size_t)k * N + A) * o * o);
+    }
+    if (dqvv) {
+      // Y[B][i][b] = sum_j G[B][i][j] x[j][b];  Z[B][a][b] = -sum_i x[i][a] Y[B][i][b]
+      c->adjQ.ensure((size_t)N * std::max<int64_t>((int64_t)o * vp, (int64_t)v * v) * 8 * 2);
+      double* Y = c->adjQ.as<double>();
+      double* Z = Y + (size_t)N * o * vp;
+      GemmArgs g5 = gargs(c->G.p, xk, Y, o, v, o, plain(op), plain(1), plain(vp), plain(1), plain(vp), plain(1));
+      g5.batch = N;
+      g5.a_batch = loo;
+      g5.b_batch = 0;
+      g5.c_batch = (int64_t)o * vp;
+      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, g5);
+      GemmArgs g6 = gargs(xk, Y, Z, v, v, o, plain(1), plain(vp), plain(vp), plain(1), plain(v), plain(1));
+      g6.batch = N;
+      g6.a_batch = 0;
+      g6.b_batch = (int64_t)o * vp;
+      g6.c_batch = (int64_t)v * v;
+      g6.alpha = -1.0;
+      run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, g6);
+      TDA_HIP(hipMemcpyAsync(dqvv + (size_t)k * N * v * v, Z, (size_t)N * v * v * 8, hipMemcpyDeviceToHost,
+                             c->stream));
+    }
+  }
+  TDA_HIP(hipStreamSynchronize(c->stream));
+}
+
+// ---------------------------------------------------------------------------
 // Davidson (solvers/davidson.rs:53-185) on device-resident vectors
 // ---------------------------------------------------------------------------
 struct DavResult {
@@ -959,6 +1055,7 @@ int tda_set_system(tda_ctx* c, int n_atoms, int n_ao, const int32_t* opa, const 
     c->ct = false;
     c->dense = false;
     c->pre_thr = 1e-4;
+    c->last_roots = 0;
     c->N = n_atoms;
     c->Nq = c->Nx = n_atoms;
     c->nao = n_ao;
@@ -1024,6 +1121,7 @@ int tda_set_ct_pair(tda_ctx* c, int n_atoms_h, int n_ao_h, const int32_t* opa_h,
     c->ct = true;
     c->dense = false;
     c->pre_thr = 1e-4;
+    c->last_roots = 0;
     c->N = N;
     c->Nq = NH;
     c->Nx = NL;
@@ -1104,6 +1202,7 @@ int tda_set_dense(tda_ctx* c, int dim, const double* h) {
     c->ct = false;
     c->dense = true;
     c->pre_thr = 0.0;
+    c->last_roots = 0;
     c->has_lc = false;
     c->N = c->Nq = c->Nx = 0;
     c->nao = 0;
@@ -1212,6 +1311,87 @@ int tda_precondition(tda_ctx* c, const double* r, double w, double* out) {
   });
 }
 
+namespace {
+// CI vectors for the adjoint: host X [ov, n] packed to the device, or the
+// last tda_run's vectors
+const double* adjoint_vectors(tda_ctx* c, int n, const double* X) {
+  TDA_CHECK(c->has_system, TDA_ERR_STATE, "no system loaded");
+  TDA_CHECK(!c->ct && !c->dense, TDA_ERR_STATE, "the adjoint is defined for a System/Monomer");
+  TDA_CHECK(n >= 1, TDA_ERR_BAD_ARG, "n_states must be positive");
+  prepare(c);
+  if (!X) {
+    TDA_CHECK(n <= c->last_roots, TDA_ERR_STATE, "no CI vectors: run tda_run first or pass X");
+    return c->RZ.as<double>();
+  }
+  const int64_t ov = (int64_t)c->o * c->v;
+  c->stage.ensure((size_t)ov * n * 8);
+  c->adjX.ensure((size_t)c->ovp * n * 8);
+  TDA_HIP(hipMemcpyAsync(c->stage.p, X, (size_t)ov * n * 8, hipMemcpyHostToDevice, c->stream));
+  launch_pack_x(c->stage.as<double>(), c->o, c->v, c->vp, n, c->adjX.as<double>(), c->stream);
+  return c->adjX.as<double>();
+}
+}  // namespace
+
+int tda_adjoint(tda_ctx* c, int n_states, const double* X, double* d_gamma, double* d_gamma_lr, double* c_qov,
+                double* d_qoo, double* d_qvv) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] {
+    const double* Xd = adjoint_vectors(c, n_states, X);
+    TDA_CHECK(c->has_lc || (!d_gamma_lr && !d_qoo && !d_qvv), TDA_ERR_STATE,
+              "gamma_lr / q_oo / q_vv derivatives need gamma_lr");
+    const int N = c->N, n = n_states;
+    const size_t nn = (size_t)N * N * n;
+    c->adjD.ensure((nn * 2 + (size_t)N * n) * 8);
+    double* dg = c->adjD.as<double>();
+    double* dglr = dg + nn;
+    double* cq = dglr + nn;
+    hf_derivatives(c, Xd, n, dg, d_gamma_lr ? dglr : nullptr, cq, d_qoo, d_qvv);
+    if (d_gamma) TDA_HIP(hipMemcpyAsync(d_gamma, dg, nn * 8, hipMemcpyDeviceToHost, c->stream));
+    if (d_gamma_lr) TDA_HIP(hipMemcpyAsync(d_gamma_lr, dglr, nn * 8, hipMemcpyDeviceToHost, c->stream));
+    if (c_qov) TDA_HIP(hipMemcpyAsync(c_qov, cq, (size_t)N * n * 8, hipMemcpyDeviceToHost, c->stream));
+    TDA_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tda_adjoint_nuclear(tda_ctx* c, int n_states, const double* X, const double* dgamma_dr,
+                        const double* dgamma_lr_dr, double* grad) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] {
+    TDA_CHECK(dgamma_dr && grad, TDA_ERR_BAD_ARG, "NULL dgamma_dr / grad");
+    const double* Xd = adjoint_vectors(c, n_states, X);
+    const bool lr = c->has_lc && dgamma_lr_dr;
+    const int N = c->N, n = n_states;
+    const int64_t nsq = (int64_t)N * N, n3 = 3 * (int64_t)N;
+    const size_t nn = (size_t)nsq * n;
+    c->adjD.ensure((nn * 3 + (size_t)N * n + (size_t)n3 * n) * 8);
+    double* dg = c->adjD.as<double>();
+    double* dglr = dg + nn;
+    double* sym = dglr + nn;
+    double* cq = sym + nn;
+    double* g = cq + (size_t)N * n;
+    hf_derivatives(c, Xd, n, dg, lr ? dglr : nullptr, cq, nullptr, nullptr);
+    // The gradient array holds row i of dgamma/dR_i only (gamma_approximation.rs:
+    // 263-279: entry [3i + d][i][j]); gamma_ji moves with R_i as well, so
+    // grad[k][c] = sum_AB g1[c][A][B] (dw_k/dgamma_AB + dw_k/dgamma_BA)
+    // (+ the gamma_lr term).
+    c->adjQ.ensure((size_t)n3 * nsq * 8);
+    const double* srcs[2] = {dgamma_dr, lr ? dgamma_lr_dr : nullptr};
+    const double* dws[2] = {dg, dglr};
+    for (int t = 0; t < 2; ++t) {
+      if (!srcs[t]) continue;
+      launch_sym_add(dws[t], n, N, sym, c->stream);
+      TDA_HIP(hipMemcpyAsync(c->adjQ.p, srcs[t], (size_t)n3 * nsq * 8, hipMemcpyHostToDevice, c->stream));
+      GemmArgs gg = gargs(sym, c->adjQ.p, g, n, n3, nsq, plain(nsq), plain(1), plain(1), plain(nsq), plain(n3),
+                          plain(1));
+      gg.beta = t == 0 ? 0.0 : 1.0;
+      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, gg);
+      TDA_HIP(hipStreamSynchronize(c->stream));  // adjQ is reloaded next
+    }
+    TDA_HIP(hipMemcpyAsync(grad, g, (size_t)n3 * n * 8, hipMemcpyDeviceToHost, c->stream));
+    TDA_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
 int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, double* X, double* q_trans,
             double* tr_dipoles, double* f, tda_stats* stats) {
   if (!c) return TDA_ERR_BAD_ARG;
@@ -1275,6 +1455,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     }
     // outputs (tda/mod.rs:88-126)
     const int N = c->N;
+    c->last_roots = n;
     const bool fp64_only = c->prec == PREC_FP64 || c->dense;
     if (omega) std::memcpy(omega, h_evals.data(), (size_t)n * 8);
     if (X) {

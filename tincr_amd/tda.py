@@ -182,6 +182,39 @@ class TdaSystem(_TdaEngine):
             nat.dptr(self._orbe), self.ob, self.n_occ, self.vb, self.n_virt, nat.dptr(self._gamma),
             nat.dptr(self._gamma_lr), nat.dptr(self.xyz)))
 
+    # -- adjoint (Hellmann-Feynman) -----------------------------------------
+    def adjoint(self, n_states, X=None, charges=False):
+        """d omega_k / d(gamma, gamma_lr, q) for the first n_states CI vectors
+        (X [ov, n_states], default: the last run_tda's).  Returns a dict with
+        d_gamma, d_gamma_lr [n, N, N], c_qov [n, N] (d omega_k / d q_ov[A, ia]
+        = c_qov[k, A] X[ia, k]) and, with charges=True, d_qoo [n, N, o*o] and
+        d_qvv [n, N, v*v]."""
+        N, o, v, n = self.n_atoms, self.n_occ, self.n_virt, n_states
+        x = None if X is None else nat.f64(np.asarray(X)[:, :n])
+        out = dict(d_gamma=np.empty((n, N, N)), c_qov=np.empty((n, N)))
+        if self.gammafunction_lc:
+            out["d_gamma_lr"] = np.empty((n, N, N))
+            if charges:
+                out["d_qoo"] = np.empty((n, N, o * o))
+                out["d_qvv"] = np.empty((n, N, v * v))
+        self.ctx.check(self.ctx.lib.tda_adjoint(
+            self.ctx.handle, n, nat.dptr(x), nat.dptr(out["d_gamma"]), nat.dptr(out.get("d_gamma_lr")),
+            nat.dptr(out["c_qov"]), nat.dptr(out.get("d_qoo")), nat.dptr(out.get("d_qvv"))))
+        return out
+
+    def nuclear_gradient(self, n_states, dgamma_dr, dgamma_lr_dr=None, X=None):
+        """d omega_k / dR at frozen charges [n_states, 3N] from the gamma
+        gradients [3N, N, N] in the layout of gamma_gradients_atomwise
+        (gamma_approximation.rs:256-282: row i of d gamma / d R_i only)."""
+        N, n = self.n_atoms, n_states
+        x = None if X is None else nat.f64(np.asarray(X)[:, :n])
+        g = np.empty((n, 3 * N))
+        dg = nat.f64(np.asarray(dgamma_dr).reshape(3 * N, N, N))
+        dglr = None if dgamma_lr_dr is None else nat.f64(np.asarray(dgamma_lr_dr).reshape(3 * N, N, N))
+        self.ctx.check(self.ctx.lib.tda_adjoint_nuclear(self.ctx.handle, n, nat.dptr(x), nat.dptr(dg),
+                                                        nat.dptr(dglr), nat.dptr(g)))
+        return g
+
 
 class Fragment:
     """The monomer data a CT pair reads (fmo Monomer properties): atoms'
