@@ -51,6 +51,12 @@ struct IdxMap {
   int64_t s1;
   int64_t s0;
   __host__ __device__ inline int64_t operator()(int64_t x) const {
+    if (div >= (INT64_C(1) << 62)) return x * s0;  // plain map: no division
+    if ((uint64_t)(x | div) < (UINT64_C(1) << 32)) {
+      // 32-bit quotient: a few instructions instead of the 64-bit division routine
+      const uint32_t q = (uint32_t)x / (uint32_t)div;
+      return (int64_t)q * s1 + (x - (int64_t)q * div) * s0;
+    }
     return (x / div) * s1 + (x % div) * s0;
   }
 };

@@ -34,7 +34,25 @@ __device__ inline int sturm_count(const double* d, const double* e2, int m, doub
   double q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   cnt += (q < 0.0);
-  for (int i = 1; i < m; ++i) {
+  int i = 1;
+  // four steps per trip with their LDS loads issued ahead of the divisions
+  for (; i + 4 <= m; i += 4) {
+    const double d0 = d[i] - x, d1 = d[i + 1] - x, d2 = d[i + 2] - x, d3 = d[i + 3] - x;
+    const double f0 = e2[i - 1], f1 = e2[i], f2 = e2[i + 1], f3 = e2[i + 2];
+    q = d0 - f0 / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += (q < 0.0);
+    q = d1 - f1 / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += (q < 0.0);
+    q = d2 - f2 / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += (q < 0.0);
+    q = d3 - f3 / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += (q < 0.0);
+  }
+  for (; i < m; ++i) {
     q = d[i] - x - e2[i - 1] / q;
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += (q < 0.0);
@@ -53,54 +71,46 @@ __device__ inline double start_entry(int i, int r) {
   return ((double)(h & 0xffffff) / 16777216.0) * 2.0 - 1.0 + 1e-3;
 }
 
-// Partially pivoted LU of (T - lam I): ri = reciprocal of the U diagonal,
-// u1, u2 = U super-diagonals, l = multipliers, piv = +1 (no swap) / -1 (swap).
-__device__ void tri_lu(const double* d, const double* e, int m, double lam, double pivmin, double* ri, double* u1,
-                       double* u2, double* l, double* piv) {
+// Partially pivoted LU of (T - lam I) with the forward substitution (row
+// swaps and L) applied to x on the fly: ri = reciprocal of the U diagonal,
+// u1, u2 = U super-diagonals; then back substitution with U.  x[i] of the
+// forward pass is carried in a register.
+__device__ void tri_lu_solve(const double* d, const double* e, int m, double lam, double pivmin, double* ri,
+                             double* u1, double* u2, double* x) {
   double cd = d[0] - lam;
   double cs1 = (m > 1) ? e[0] : 0.0;
   double cs2 = 0.0;
+  double xi = x[0];
   for (int i = 0; i < m - 1; ++i) {
     const double sub = e[i];
     const double nd = d[i + 1] - lam;
     const double ns1 = (i + 1 < m - 1) ? e[i + 1] : 0.0;
+    const double xn = x[i + 1];
     if (fabs(cd) >= fabs(sub)) {
       double p = cd;
       if (fabs(p) < pivmin) p = (p < 0.0) ? -pivmin : pivmin;
       const double rp = 1.0 / p;
       const double mult = sub * rp;
-      ri[i] = rp; u1[i] = cs1; u2[i] = cs2; l[i] = mult; piv[i] = 1.0;
+      ri[i] = rp; u1[i] = cs1; u2[i] = cs2;
       cd = nd - mult * cs1;
       cs1 = ns1 - mult * cs2;
       cs2 = 0.0;
+      x[i] = xi;
+      xi = xn - mult * xi;
     } else {
       const double rp = 1.0 / sub;
       const double mult = cd * rp;
-      ri[i] = rp; u1[i] = nd; u2[i] = ns1; l[i] = mult; piv[i] = -1.0;
+      ri[i] = rp; u1[i] = nd; u2[i] = ns1;
       cd = cs1 - mult * nd;
       cs1 = cs2 - mult * ns1;
       cs2 = 0.0;
+      x[i] = xn;
+      xi = xi - mult * xn;
     }
   }
   double p = cd;
   if (fabs(p) < pivmin) p = (p < 0.0) ? -pivmin : pivmin;
   ri[m - 1] = 1.0 / p; u1[m - 1] = 0.0; u2[m - 1] = 0.0;
-}
-
-__device__ void tri_solve(const double* ri, const double* u1, const double* u2, const double* l, const double* piv,
-                          int m, double* x) {
-  // forward (row swaps and L); x[i] is carried in a register
-  double xi = x[0];
-  for (int i = 0; i < m - 1; ++i) {
-    const double xn = x[i + 1];
-    if (piv[i] < 0.0) {
-      x[i] = xn;
-      xi = xi - l[i] * xn;
-    } else {
-      x[i] = xi;
-      xi = xn - l[i] * xi;
-    }
-  }
   x[m - 1] = xi;
   double x1 = 0.0, x2 = 0.0;  // x[i+1], x[i+2]
   for (int i = m - 1; i >= 0; --i) {
@@ -109,6 +119,18 @@ __device__ void tri_solve(const double* ri, const double* u1, const double* u2, 
     x2 = x1;
     x1 = v;
   }
+}
+
+// orders the LDS / global accesses of one wave (lanes exchange data)
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
 }
 
 __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __restrict__ H, int ldh, int m, int n,
@@ -158,15 +180,40 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
   EIGH_STAMP(1)
   // 2. Householder tridiagonalisation (lower; A(k, j) == A(j, k)).  Element
   //    (i, j) of the working matrix is Am[(i - k0) * lda + (j - k0)].  The
-  //    rank-2 update of step k-1 is kept pending and applied inside step k's
-  //    mat-vec pass, so each step reads and writes the trailing block once.
-  //    Mat-vec in column form: thread (i, seg) sums A(j, i) v_j over its
-  //    segment of j (coalesced over i), partial sums combined through LDS.
+  //    rank-2 update A -= v w^T + w v^T of step k is applied in two places:
+  //    to row k+1 right away (together with the norm of its tail, which the
+  //    next reflector needs), and to the rest of the trailing block inside
+  //    step k+1's mat-vec pass, so each step reads and writes the trailing
+  //    block once.  Mat-vec in column form: thread (i, seg) sums A(j, i) v_j
+  //    over its segment of j (coalesced over i), partial sums combined
+  //    through LDS.  Three or four barriers per step.
   double* Am = A;
   int k0 = 0, lda = m;
   bool in_lds = false;
-  bool pending = false;  // (pv_v, pv_w) = (v_{k-1}, w_{k-1}) not yet applied to rows/cols >= k
   double* part = pool + EIGH_POOL - 2048;  // segment partial sums (global phase only)
+  __shared__ double red2[2][EIGH_NW];
+  int rb = 0;
+  // one-barrier block sum (alternating buffers)
+  auto block_sum1 = [&](double v) -> double {
+    v = wave_sum(v);
+    if (lane == 0) red2[rb][wave] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < EIGH_NW; ++w) s += red2[rb][w];
+    rb ^= 1;
+    return s;
+  };
+  for (int j = tid; j < m; j += EIGH_THREADS) {
+    sv[j] = 0.0;
+    sw[j] = 0.0;
+  }
+  double sigma = 0.0;  // |A(k, k+2:)|^2 of the current row k
+  {
+    double ps = 0.0;
+    for (int j = 2 + tid; j < m; j += EIGH_THREADS) ps += A[j] * A[j];
+    sigma = block_sum1(ps);  // also orders the sv / sw clearing
+  }
   for (int k = 0; k < m - 2; ++k) {
     const int nt = m - k - 1;  // trailing size (rows/cols k+1..m-1)
     if (!in_lds && (m - k) * (m - k) <= EIGH_POOL - 2048) {
@@ -182,14 +229,7 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
       lda = nb;
       in_lds = true;
     }
-    double* rowk = Am + (int64_t)(k - k0) * lda - k0;  // rowk[j] = A(k, j)
-    // bring row k up to date
-    if (pending)
-      for (int j = k + tid; j < m; j += EIGH_THREADS) rowk[j] -= sv[k] * sw[j] + sw[k] * sv[j];
-    __syncthreads();
-    double part_s = 0.0;
-    for (int j = k + 2 + tid; j < m; j += EIGH_THREADS) part_s += rowk[j] * rowk[j];
-    const double sigma = block_sum(part_s);
+    const double* rowk = Am + (int64_t)(k - k0) * lda - k0;  // rowk[j] = A(k, j), up to date
     const double alpha = rowk[k + 1];
     if (tid == 0) sd[k] = rowk[k];
     double tk = 0.0, beta = alpha, sc = 0.0;
@@ -198,71 +238,76 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
       tk = (beta - alpha) / beta;
       sc = 1.0 / (alpha - beta);
     }
-    // new Householder vector into snv (kept in global row k for the back-transformation)
+    // Householder vector into snv (kept in global row k for the back-transformation)
     for (int j = k + 1 + tid; j < m; j += EIGH_THREADS) {
       const double vj = (j == k + 1) ? 1.0 : rowk[j] * sc;
       snv[j] = vj;
       if (j >= k + 2) A[(int64_t)k * m + j] = vj;
     }
-    __syncthreads();
-    // fused pass over the trailing block: apply the pending update, then p = A v
+    __syncthreads();  // B1: v complete
+    // fused pass over the trailing block: apply the pending (v_{k-1}, w_{k-1}), then p = A v
     const int segs = max(1, min(8, EIGH_THREADS / nt));
+    const int i = k + 1 + tid % nt;
+    double acc = 0.0;
     {
-      const int i = k + 1 + tid % nt;
       const int seg = tid / nt;
-      double acc = 0.0;
       if (seg < segs) {
         double* ci = Am + (i - k0) - (int64_t)k0 * lda;  // ci[j * lda] = A(j, i)
-        const double vpi = pending ? sv[i] : 0.0, wpi = pending ? sw[i] : 0.0;
+        const double vpi = sv[i], wpi = sw[i];
         const int len = (nt + segs - 1) / segs;
         const int j0 = k + 1 + seg * len, j1 = min(m, j0 + len);
-#pragma unroll 4
+#pragma unroll 8
         for (int j = j0; j < j1; ++j) {
-          double a = ci[(int64_t)j * lda];
-          if (pending) {
-            a -= sv[j] * wpi + sw[j] * vpi;
-            ci[(int64_t)j * lda] = a;
-          }
+          double a = ci[(int64_t)j * lda] - (sv[j] * wpi + sw[j] * vpi);
+          ci[(int64_t)j * lda] = a;
           acc += a * snv[j];
         }
         if (segs > 1) part[seg * nt + (i - k - 1)] = acc;
       }
       if (segs > 1) {
-        __syncthreads();
+        __syncthreads();  // B2: partial sums complete
         if (tid < nt) {
-          double s = 0.0;
-          for (int q = 0; q < segs; ++q) s += part[q * nt + tid];
-          acc = s;
+          double s2 = 0.0;
+          for (int q = 0; q < segs; ++q) s2 += part[q * nt + tid];
+          acc = s2;
         }
       }
-      __syncthreads();  // all reads of sv / sw (pending update) done
-      if (tid < nt) snp[k + 1 + tid] = acc * tk;
     }
-    __syncthreads();
-    double pvp = 0.0;
-    for (int j = k + 1 + tid; j < m; j += EIGH_THREADS) pvp += snp[j] * snv[j];
-    const double ptv = block_sum(pvp);
+    // p_i = tk * (A v)_i for i = k+1+tid (tid < nt); p . v by block sum
+    double pi = 0.0, vi = 0.0;
+    if (tid < nt) {
+      pi = acc * tk;
+      vi = snv[i];
+      if (tid == 0) snp[k + 1] = pi;
+    }
+    const double ptv = block_sum1(pi * vi);  // B3: also publishes snp[k+1]; the pass is over
     const double K = 0.5 * tk * ptv;
-    // the new (v, w) become the pending update
-    for (int j = k + 1 + tid; j < m; j += EIGH_THREADS) {
-      sv[j] = snv[j];
-      sw[j] = snp[j] - K * snv[j];
+    if (tid == 0) {
+      se[k] = beta;
+      tau[k] = tk;
     }
-    pending = (sigma != 0.0);
-    if (tid == 0) { se[k] = beta; tau[k] = tk; }
-    __syncthreads();
+    // new pending (v_k, w_k); row k+1 gets it now, with the norm of its tail
+    double ps = 0.0;
+    if (tid < nt) {
+      const double wi = pi - K * vi;
+      const double w1 = snp[k + 1] - K;  // w_{k+1} (v_{k+1} = 1)
+      sv[i] = vi;
+      sw[i] = wi;
+      double* rowk1 = Am + (int64_t)(k + 1 - k0) * lda - k0;
+      const double a = rowk1[i] - (wi + w1 * vi);
+      rowk1[i] = a;
+      if (i >= k + 3) ps = a * a;
+    }
+    sigma = block_sum1(ps);  // B4: row k+1, sv, sw complete
   }
   if (tid == 0) {
-    auto at = [&](int i, int j) {
-      double a = Am[(int64_t)(i - k0) * lda + (j - k0)];
-      if (pending) a -= sv[i] * sw[j] + sw[i] * sv[j];
-      return a;
-    };
+    // rows m-2 and m-1: row m-2 is up to date; A(m-1, m-1) still lacks (v_{m-3}, w_{m-3})
+    auto at = [&](int i2, int j2) { return Am[(int64_t)(i2 - k0) * lda + (j2 - k0)]; };
     if (m >= 2) {
       sd[m - 2] = at(m - 2, m - 2);
-      se[m - 2] = at(m - 1, m - 2);
+      se[m - 2] = at(m - 2, m - 1);
     }
-    sd[m - 1] = at(m - 1, m - 1);
+    sd[m - 1] = at(m - 1, m - 1) - 2.0 * sv[m - 1] * sw[m - 1];
     for (int k = (m >= 2 ? m - 2 : 0); k < m; ++k) tau[k] = 0.0;
   }
   __syncthreads();
@@ -325,55 +370,64 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
   }
   __syncthreads();
   const int nclus = snclus;
-  // LDS slots of 6*m doubles (u0, u1, u2, l, piv, z); clusters in rounds
-  const int slot_sz = 6 * m;
-  const int nslots = max(1, min(64, EIGH_POOL / slot_sz));
-  double* gslots = tau + m + (size_t)64 * m;
-  for (int base = 0; base < nclus; base += nslots) {
-    const int c = base + tid;
-    if (tid < nslots && c < nclus) {
-      double* sl = (slot_sz <= EIGH_POOL) ? pool + (size_t)tid * slot_sz : gslots + (size_t)c * slot_sz;
-      double *u0 = sl, *u1 = sl + m, *u2 = sl + 2 * m, *l = sl + 3 * m, *piv = sl + 4 * m, *z = sl + 5 * m;
+  // one wave per cluster with an LDS slot of 4*m doubles (ri, u1, u2, z):
+  // lane 0 runs the LU recurrences, the wave does the vector operations
+  const int slot_sz = 4 * m;
+  const int nslots = max(1, min(EIGH_NW, EIGH_POOL / slot_sz));
+  if (wave < nslots) {
+    double* sl = pool + (size_t)wave * slot_sz;
+    double *ri = sl, *u1 = sl + m, *u2 = sl + 2 * m, *z = sl + 3 * m;
+    for (int c = wave; c < nclus; c += nslots) {
       for (int r = cl_start[c]; r < cl_start[c + 1]; ++r) {
         double lam = evals[r];
         if (r > cl_start[c] && lam - evals[r - 1] < 10.0 * 2.2e-16 * tnorm)
           lam = evals[r - 1] + 10.0 * 2.2e-16 * tnorm;
-        tri_lu(sd, se, m, lam, pivmin, u0, u1, u2, l, piv);
         double nrm = 0.0;
-        for (int i = 0; i < m; ++i) { z[i] = start_entry(i, r); nrm += z[i] * z[i]; }
-        nrm = 1.0 / sqrt(nrm);
-        for (int i = 0; i < m; ++i) z[i] *= nrm;
-        for (int it = 0; it < 1; ++it) {
-          tri_solve(u0, u1, u2, l, piv, m, z);
-          for (int s2 = cl_start[c]; s2 < r; ++s2) {
-            const double* zs = Y + (size_t)s2 * ldy;
-            double dt = 0.0;
-            for (int i = 0; i < m; ++i) dt += zs[i] * z[i];
-            for (int i = 0; i < m; ++i) z[i] -= dt * zs[i];
-          }
-          double s2 = 0.0;
-          for (int i = 0; i < m; ++i) s2 += z[i] * z[i];
-          if (!(s2 > 0.0) || !isfinite(s2)) {  // rescale by the max entry first
-            double mx = 0.0;
-            for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(z[i]));
-            if (!(mx > 0.0) || !isfinite(mx)) {
-              for (int i = 0; i < m; ++i) z[i] = 0.0;
-              z[r % m] = 1.0;
-              mx = 1.0;
-            }
-            const double im = 1.0 / mx;
-            s2 = 0.0;
-            for (int i = 0; i < m; ++i) { z[i] *= im; s2 += z[i] * z[i]; }
-          }
-          const double inv = 1.0 / sqrt(s2);
-          for (int i = 0; i < m; ++i) z[i] *= inv;
+        for (int i = lane; i < m; i += 64) {
+          const double zi = start_entry(i, r);
+          z[i] = zi;
+          nrm += zi * zi;
         }
+        nrm = 1.0 / sqrt(wave_sum(nrm));
+        for (int i = lane; i < m; i += 64) z[i] *= nrm;
+        wave_sync();
+        if (lane == 0) tri_lu_solve(sd, se, m, lam, pivmin, ri, u1, u2, z);
+        wave_sync();
+        for (int s2 = cl_start[c]; s2 < r; ++s2) {  // modified Gram-Schmidt inside the cluster
+          const double* zs = Y + (size_t)s2 * ldy;
+          double dt = 0.0;
+          for (int i = lane; i < m; i += 64) dt += zs[i] * z[i];
+          dt = wave_sum(dt);
+          for (int i = lane; i < m; i += 64) z[i] -= dt * zs[i];
+        }
+        double s2 = 0.0;
+        for (int i = lane; i < m; i += 64) s2 += z[i] * z[i];
+        s2 = wave_sum(s2);
+        if (!(s2 > 0.0) || !isfinite(s2)) {  // rescale by the max entry first
+          double mx = 0.0;
+          for (int i = lane; i < m; i += 64) mx = fmax(mx, fabs(z[i]));
+          mx = wave_max(mx);
+          if (!(mx > 0.0) || !isfinite(mx)) {
+            for (int i = lane; i < m; i += 64) z[i] = (i == r % m) ? 1.0 : 0.0;
+            mx = 1.0;
+          }
+          const double im = 1.0 / mx;
+          s2 = 0.0;
+          for (int i = lane; i < m; i += 64) {
+            z[i] *= im;
+            s2 += z[i] * z[i];
+          }
+          s2 = wave_sum(s2);
+        }
+        const double inv = 1.0 / sqrt(s2);
         double* y = Y + (size_t)r * ldy;
-        for (int i = 0; i < m; ++i) y[i] = z[i];
+        for (int i = lane; i < m; i += 64) y[i] = z[i] * inv;
+        __threadfence_block();  // later roots of the cluster read y
+        wave_sync();
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
 
   EIGH_STAMP(4)
   // 5. back-transformation Y = H_0 H_1 ... H_{m-3} Z in blocks of BW

@@ -353,6 +353,27 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
   TO* C = (TO*)g.C + zb * g.c_batch;
   double alpha = g.alpha;
   if (g.batch_scale) alpha *= (double)g.batch_scale[zb];
+  // Column parts of the output / epilogue addresses, once per column.
+  int64_t cn_off[NI], en_off[NI], xn_off[NI];
+  bool n_ok[NI];
+#pragma unroll
+  for (int j = 0; j < NI; ++j) {
+    const int64_t n = n0 + wn0 + j * 16 + (lane & 15);
+    n_ok[j] = n < g.N;
+    cn_off[j] = n_ok[j] ? g.cn(n) : 0;
+    en_off[j] = 0;
+    xn_off[j] = 0;
+    if constexpr (EPI == EPI_DIAG) {
+      if (n_ok[j]) {
+        en_off[j] = g.dmap(n);
+        xn_off[j] = g.xn(n);
+      }
+    } else if constexpr (EPI == EPI_HADAMARD) {
+      if (n_ok[j]) en_off[j] = g.hn(n);
+    }
+  }
+  const double* xs = nullptr;
+  if constexpr (EPI == EPI_DIAG) xs = g.xsrc + zb * g.x_batch;
 #pragma unroll
   for (int i = 0; i < MI; ++i)
 #pragma unroll
@@ -360,33 +381,47 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
       const int64_t m = m0 + wm0 + i * 16 + MF::row(lane, r);
       if (m >= g.M) continue;
       const int64_t cmo = g.cm(m);
+      int64_t emo = 0, xmo = 0;
+      if constexpr (EPI == EPI_DIAG) {
+        emo = g.dm(m);
+        xmo = g.xm(m);
+      } else if constexpr (EPI == EPI_HADAMARD) {
+        emo = g.hm(m);
+      }
 #pragma unroll
       for (int j = 0; j < NI; ++j) {
-        const int64_t n = n0 + wn0 + j * 16 + (lane & 15);
-        if (n >= g.N) continue;
-        TO* p = C + cmo + g.cn(n);
-        const double v = epi_value<EPI, TO>(g, zb, m, n, (double)acc[i][j][r], alpha, p);
+        if (!n_ok[j]) continue;
+        TO* p = C + cmo + cn_off[j];
+        double v = alpha * (double)acc[i][j][r];
+        if constexpr (EPI == EPI_DIAG) {
+          v += g.diag[emo + en_off[j]] * xs[xmo + xn_off[j]];
+        } else if constexpr (EPI == EPI_HADAMARD) {
+          v *= g.hsrc[emo + en_off[j]];
+        } else {
+          if (g.beta != 0.0) v += g.beta * to_double<TO>(*p);
+        }
         store_out<TO>(p, v);
       }
     }
 }
 
 // Sum split-K partials and apply the epilogue.
+// Grid: x over column chunks of 256, y over rows (strided), z over batches.
 template <typename TA, typename TO, int EPI>
 __global__ void __launch_bounds__(256) splitk_reduce_kernel(GemmArgs g) {
   const int64_t MN = g.M * g.N;
-  const int64_t total = MN * g.batch;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int zb = (int)(t / MN);
-    const int64_t mn = t % MN;
-    const int64_t m = mn / g.N, n = mn % g.N;
+  const int zb = blockIdx.z;
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= g.N) return;
+  double alpha = g.alpha;
+  if (g.batch_scale) alpha *= (double)g.batch_scale[zb];
+  const int64_t cno = g.cn(n);
+  for (int64_t m = blockIdx.y; m < g.M; m += gridDim.y) {
+    const int64_t mn = m * g.N + n;
     const TA* ws = (const TA*)g.ws + (int64_t)zb * g.ksplit * MN + mn;
     double s = 0.0;
     for (int z = 0; z < g.ksplit; ++z) s += (double)ws[z * MN];
-    double alpha = g.alpha;
-    if (g.batch_scale) alpha *= (double)g.batch_scale[zb];
-    TO* p = (TO*)g.C + zb * g.c_batch + g.cm(m) + g.cn(n);
+    TO* p = (TO*)g.C + zb * g.c_batch + g.cm(m) + cno;
     const double v = epi_value<EPI, TO>(g, zb, m, n, s, alpha, p);
     *p = to_type<TO>(v);
   }
@@ -425,9 +460,11 @@ void launch_t(int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s) {
 
 template <typename TA, typename TO, int EPI>
 void launch_reduce(GemmArgs& g, hipStream_t s) {
-  const int64_t total = g.M * g.N * g.batch;
-  const int blocks = (int)std::min<int64_t>(ceil_div(total, 256), 8192);
-  splitk_reduce_kernel<TA, TO, EPI><<<blocks, 256, 0, s>>>(g);
+  const int64_t nx = ceil_div(g.N, 256);
+  // enough row blocks for ~2048 workgroups in total, at most one per row
+  const int64_t ny = std::max<int64_t>(1, std::min<int64_t>(g.M, std::min<int64_t>(65535, ceil_div(2048, nx * g.batch))));
+  dim3 grid((unsigned)nx, (unsigned)ny, (unsigned)g.batch);
+  splitk_reduce_kernel<TA, TO, EPI><<<grid, 256, 0, s>>>(g);
   TDA_LAUNCH_CHECK();
 }
 
