@@ -37,6 +37,24 @@ KERNEL_NAMES = {
 }
 
 
+CALIB_COPY_BYTES = 256 << 20
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of the dominant kernel for this workload, from the
+    committed counter summaries (profiles/*.md lines
+    `traffic_bytes_per_launch[<key>]: <bytes>`); None when not measured."""
+    import glob
+    import re
+    pat = re.compile(r"traffic_bytes_per_launch\[" + re.escape(key) + r"\]:\s*([0-9.eE+]+)")
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as fh:
+            m = pat.search(fh.read())
+        if m:
+            return float(m.group(1))
+    return None
+
+
 def build_inputs(system, seed):
     from tincr_amd import systems as sy
     if system.startswith("polyene"):
@@ -260,12 +278,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="single", choices=["single", "fmo"])
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 256 MiB calibration copy first (reference for counter runs)")
     args = ap.parse_args()
     args.prec_given = any(a.startswith("--prec") for a in sys.argv[1:])
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding is a design (DESIGN.md), not a measured path", file=sys.stderr)
         raise SystemExit(2)
+    if args.calibrate_copy:
+        from tincr_amd import _native as nat
+        nat.debug_copy(CALIB_COPY_BYTES, 3)
     if args.workload == "fmo":
         return main_fmo(args)
 
@@ -310,9 +333,11 @@ def main():
     si, pi = np.unravel_index(np.argmax(tot_ms), tot_ms.shape)
     kprec = "fp64" if pi == 0 else args.prec
     achieved = tot_fl[si, pi] / (tot_ms[si, pi] * 1e-3) / 1e12 if tot_ms[si, pi] > 0 else 0.0
+    traffic_key = f"{args.system}/{kprec}/{args.operator}/stage{si}"
     roofline = {
         "bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_TFLOPS[kprec], "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK_TFLOPS[kprec], 4), "traffic": None,
+        "frac": round(achieved / PEAK_TFLOPS[kprec], 4), "traffic": measured_traffic(traffic_key),
+        "traffic_key": traffic_key,
         "kernel": KERNEL_NAMES[args.operator][si] + f" ({kprec} MFMA)",
         "launches": int(tot_n[si, pi]),
         "avg_launch_ms": round(tot_ms[si, pi] / max(tot_n[si, pi], 1), 4),

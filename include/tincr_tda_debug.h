@@ -25,6 +25,10 @@ int tda_debug_eigh(int m, int n, const double* H, double* evals, double* Y);
  * back-transformation. */
 int tda_debug_eigh_timed(int m, int n, const double* H, double* evals, double* Y, double* phase_us);
 
+/* `reps` launches of calib_copy_kernel reading and writing `bytes` each (a
+ * known-traffic reference for hardware-counter runs). */
+int tda_debug_copy(int64_t bytes, int reps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -182,6 +182,8 @@ def load_debug():
     lib.tda_debug_eigh.restype = ctypes.c_int
     lib.tda_debug_eigh_timed.argtypes = [i, i, D, D, D, D]
     lib.tda_debug_eigh_timed.restype = ctypes.c_int
+    lib.tda_debug_copy.argtypes = [ctypes.c_int64, i]
+    lib.tda_debug_copy.restype = ctypes.c_int
     return lib
 
 
@@ -211,3 +213,11 @@ def debug_eigh(h, n, timed=False):
     if rc != TDA_OK:
         raise TdaError(rc, "tda_debug_eigh failed")
     return (ev, y, ph) if timed else (ev, y)
+
+
+def debug_copy(nbytes, reps=1):
+    """`reps` launches of calib_copy_kernel moving `nbytes` in and out (a
+    known-traffic reference for counter runs)."""
+    rc = load_debug().tda_debug_copy(int(nbytes), int(reps))
+    if rc != TDA_OK:
+        raise TdaError(rc, "tda_debug_copy failed")

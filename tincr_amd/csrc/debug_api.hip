@@ -84,3 +84,26 @@ extern "C" int tda_debug_eigh_timed(int m, int n, const double* H, double* evals
     TDA_HIP(hipMemcpy(Y, y.p, (size_t)n * m * 8, hipMemcpyDeviceToHost));
   });
 }
+
+namespace {
+// Reads n doubles and writes n doubles: the HBM-traffic calibration of a
+// counter run (FETCH_SIZE / WRITE_SIZE per dispatch for known bytes).
+__global__ void __launch_bounds__(256) calib_copy_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                                         int64_t n) {
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) dst[t] = src[t];
+}
+}  // namespace
+
+extern "C" int tda_debug_copy(int64_t bytes, int reps) {
+  return guard([&] {
+    TDA_CHECK(bytes >= 8 && reps >= 1, TDA_ERR_BAD_ARG, "bad copy size");
+    const int64_t n = bytes / 8;
+    Buf a(n * 8), b(n * 8);
+    TDA_HIP(hipMemset(a.p, 0, n * 8));
+    for (int r = 0; r < reps; ++r) {
+      calib_copy_kernel<<<4096, 256, 0, 0>>>((const double*)a.p, (double*)b.p, n);
+      TDA_LAUNCH_CHECK();
+    }
+    TDA_HIP(hipDeviceSynchronize());
+  });
+}

@@ -256,8 +256,21 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
         const double vpi = sv[i], wpi = sw[i];
         const int len = (nt + segs - 1) / segs;
         const int j0 = k + 1 + seg * len, j1 = min(m, j0 + len);
-#pragma unroll 8
-        for (int j = j0; j < j1; ++j) {
+        // 16 independent loads in flight per thread: the global phase is
+        // latency-bound (one CU streams the whole trailing block)
+        int j = j0;
+        for (; j + 16 <= j1; j += 16) {
+          double a[16];
+#pragma unroll
+          for (int q = 0; q < 16; ++q) a[q] = ci[(int64_t)(j + q) * lda];
+#pragma unroll
+          for (int q = 0; q < 16; ++q) {
+            a[q] -= sv[j + q] * wpi + sw[j + q] * vpi;
+            ci[(int64_t)(j + q) * lda] = a[q];
+            acc += a[q] * snv[j + q];
+          }
+        }
+        for (; j < j1; ++j) {
           double a = ci[(int64_t)j * lda] - (sv[j] * wpi + sw[j] * vpi);
           ci[(int64_t)j * lda] = a;
           acc += a * snv[j];

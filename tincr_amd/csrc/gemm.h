@@ -13,7 +13,7 @@ namespace tda {
 enum Epi : int {
   EPI_STORE = 0,     // C = alpha*acc*scale[z] + beta*C
   EPI_DIAG = 1,      // C = alpha*acc + diag[dm(m) + dmap(n)] * X[z*x_batch + xm(m) + xn(n)]
-  EPI_HADAMARD = 2,  // C = alpha*acc * H[hm(m) + hn(n)]
+  EPI_HADAMARD = 2,  // C = alpha*acc * H[hm(r(m)) + hn(r(n))], r(x) = hidx ? hidx[x % hmod] : x
 };
 
 struct GemmArgs {
@@ -40,7 +40,18 @@ struct GemmArgs {
   // EPI_HADAMARD
   const double* hsrc;
   IdxMap hm, hn;
+  const int* hidx;  // optional row / column gather table (e.g. AO -> atom)
+  int64_t hmod;     // its length; indices are reduced modulo hmod first
 };
+
+// Row / column index into H of the Hadamard epilogue.
+__host__ __device__ inline int64_t hada_index(const GemmArgs& g, int64_t x) {
+  if (!g.hidx) return x;
+  const int64_t r = (uint64_t)x < (UINT64_C(1) << 32) && (uint64_t)g.hmod < (UINT64_C(1) << 32)
+                        ? (int64_t)((uint32_t)x % (uint32_t)g.hmod)
+                        : x % g.hmod;
+  return g.hidx[r];
+}
 
 // Element type of an operand / result buffer.
 enum DType : int { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2, DT_F16 = 3 };

@@ -103,7 +103,7 @@ __device__ inline double epi_value(const GemmArgs& g, int zb, int64_t m, int64_t
   if constexpr (EPI == EPI_DIAG) {
     v += g.diag[g.dm(m) + g.dmap(n)] * g.xsrc[zb * g.x_batch + g.xm(m) + g.xn(n)];
   } else if constexpr (EPI == EPI_HADAMARD) {
-    v *= g.hsrc[g.hm(m) + g.hn(n)];
+    v *= g.hsrc[g.hm(hada_index(g, m)) + g.hn(hada_index(g, n))];
   } else {
     if (g.beta != 0.0) v += g.beta * to_double<TO>(*p);
   }
@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
         xn_off[j] = g.xn(n);
       }
     } else if constexpr (EPI == EPI_HADAMARD) {
-      if (n_ok[j]) en_off[j] = g.hn(n);
+      if (n_ok[j]) en_off[j] = g.hn(hada_index(g, n));
     }
   }
   const double* xs = nullptr;
@@ -386,7 +386,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
         emo = g.dm(m);
         xmo = g.xm(m);
       } else if constexpr (EPI == EPI_HADAMARD) {
-        emo = g.hm(m);
+        emo = g.hm(hada_index(g, m));
       }
 #pragma unroll
       for (int j = 0; j < NI; ++j) {
@@ -406,7 +406,31 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
 }
 
 // Sum split-K partials and apply the epilogue.
-// Grid: x over column chunks of 256, y over rows (strided), z over batches.
+// Many partials per output (ksplit >= 16): one wave per output element, the
+// 64 lanes split the partials and combine them with a wave sum.
+template <typename TA, typename TO, int EPI>
+__global__ void __launch_bounds__(256) splitk_reduce_wave_kernel(GemmArgs g) {
+  const int64_t MN = g.M * g.N;
+  const int64_t out = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (out >= MN * g.batch) return;
+  const int zb = (int)(out / MN);
+  const int64_t mn = out - (int64_t)zb * MN;
+  const int64_t m = mn / g.N, n = mn - m * g.N;
+  const TA* ws = (const TA*)g.ws + (int64_t)zb * g.ksplit * MN + mn;
+  double s = 0.0;
+  for (int z = lane; z < g.ksplit; z += 64) s += (double)ws[z * MN];
+  s = wave_sum(s);
+  if (lane != 0) return;
+  double alpha = g.alpha;
+  if (g.batch_scale) alpha *= (double)g.batch_scale[zb];
+  TO* p = (TO*)g.C + zb * g.c_batch + g.cm(m) + g.cn(n);
+  const double v = epi_value<EPI, TO>(g, zb, m, n, s, alpha, p);
+  *p = to_type<TO>(v);
+}
+
+// Few partials per output: one thread per output element.  Grid: x over
+// column chunks of 256, y over rows (strided), z over batches.
 template <typename TA, typename TO, int EPI>
 __global__ void __launch_bounds__(256) splitk_reduce_kernel(GemmArgs g) {
   const int64_t MN = g.M * g.N;
@@ -460,11 +484,17 @@ void launch_t(int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s) {
 
 template <typename TA, typename TO, int EPI>
 void launch_reduce(GemmArgs& g, hipStream_t s) {
-  const int64_t nx = ceil_div(g.N, 256);
-  // enough row blocks for ~2048 workgroups in total, at most one per row
-  const int64_t ny = std::max<int64_t>(1, std::min<int64_t>(g.M, std::min<int64_t>(65535, ceil_div(2048, nx * g.batch))));
-  dim3 grid((unsigned)nx, (unsigned)ny, (unsigned)g.batch);
-  splitk_reduce_kernel<TA, TO, EPI><<<grid, 256, 0, s>>>(g);
+  if (g.ksplit >= 16) {
+    const int64_t outs = g.M * g.N * g.batch;
+    splitk_reduce_wave_kernel<TA, TO, EPI><<<(unsigned)ceil_div(outs, 4), 256, 0, s>>>(g);
+  } else {
+    const int64_t nx = ceil_div(g.N, 256);
+    // enough row blocks for ~4096 workgroups in total, at most one per row
+    const int64_t ny =
+        std::max<int64_t>(1, std::min<int64_t>(g.M, std::min<int64_t>(65535, ceil_div(4096, nx * g.batch))));
+    dim3 grid((unsigned)nx, (unsigned)ny, (unsigned)g.batch);
+    splitk_reduce_kernel<TA, TO, EPI><<<grid, 256, 0, s>>>(g);
+  }
   TDA_LAUNCH_CHECK();
 }
 

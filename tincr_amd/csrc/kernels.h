@@ -44,12 +44,14 @@ void launch_precondition_many(const double* R, const double* u, const int* idx, 
 void launch_residual(double* AVY, const double* RZ, const double* u, int n, int64_t L, hipStream_t s);
 // norms[r] = || X[r] ||  (deterministic two-pass), partial: >= n*256 doubles
 void launch_norms(const double* X, int n, int64_t L, double* partial, double* norms, hipStream_t s);
-// classical Gram-Schmidt of one new vector against accepted new vectors
-//   d[s]  = Q[s] . P_t           for s < *count
-//   W_t  -= sum_s d[s] Q[s];  norm = ||W_t||
-//   if norm > 1e-8: Q[*count] = W_t / norm; ++*count
-void launch_gs_step(const double* Pt, double* Wt, double* Q, int64_t L, int* count, double* scratch,
-                    int max_new, hipStream_t s);
+// Gram-Schmidt of nc new vectors among themselves (davidson.rs:162-168),
+// after their projection against the old basis: W[j] (= P[j] - V V^T P[j])
+// becomes u_j = W[j] - sum_{s<j accepted} q_s (q_s . P[j]); u_j is accepted
+// when |u_j| > 1e-8; the accepted q_j = u_j / |u_j| go to Q[0..*count) in
+// order.  scratch: gs_block_scratch_doubles(nc) doubles.
+void launch_gs_block(const double* P, double* W, double* Q, int64_t L, int nc, double* scratch, int* count,
+                     hipStream_t s);
+size_t gs_block_scratch_doubles(int nc);
 void launch_copy(const double* src, double* dst, int64_t n, hipStream_t s);
 void launch_fill(double* dst, double v, int64_t n, hipStream_t s);
 // zero columns [c0, c0 + cw) of `rows` rows of length ld (elements of `esz`
@@ -65,11 +67,9 @@ void launch_small_gemm(const double* A, const double* B, double* C, int M, int N
 // for col < P, 0 for P <= col < P_pad.  transpose=1 writes out[col][r].
 void launch_ao_operand(const double* C, const double* SC, int n_ao, int base, int P, int P_pad, int swap,
                        int transpose, double* out, hipStream_t s);
-// Gt[nu][mu] = gamma_lr[atom[nu]][atom[mu]]
-void launch_ao_gamma(const double* gamma_lr, const int* atom, int n_ao, int n_atoms, double* Gt, hipStream_t s);
 // Coulomb from the D-blocks of M (M = -1/4 gamma~ o D, 2n_ao x 2n_ao per vector,
 // element type dtype): u[A*k + c] = 1/2 sum_{mu in A} (D[(0,mu),(1,mu)] + D[(1,mu),(0,mu)])
-void launch_ao_charges(const void* M, int dtype, int k, int n_ao, const int* ao_off, int n_atoms, const double* Gt,
+void launch_ao_charges(const void* M, int dtype, int k, int n_ao, const int* ao_off, int n_atoms, const double* gamma_lr,
                        double* u, hipStream_t s);
 // M[c][(0,mu),(1,mu)] += w[atom(mu)*k + c] and M[c][(1,mu),(0,mu)] += same
 void launch_ao_add_coulomb(void* M, int dtype, int k, int n_ao, const int* atom, const double* w, hipStream_t s);

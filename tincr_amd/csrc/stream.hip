@@ -340,100 +340,108 @@ void launch_norms(const double* X, int n, int64_t L, double* partial, double* no
   TDA_LAUNCH_CHECK();
 }
 
-// ---- Gram-Schmidt step for one new vector (davidson.rs:162-168) ----
-// scratch layout: [0, NB_NORM*max_new) dot partials, then d[max_new],
-// then norm partials [NB_NORM], then {norm, accept}.
-__global__ void __launch_bounds__(256) gs_dots_kernel(const double* __restrict__ Pt, const double* __restrict__ Q,
-                                                      int64_t L, const int* __restrict__ count,
-                                                      double* __restrict__ scratch) {
-  const int s = blockIdx.y;
-  if (s >= *count) return;
-  const double* q = Q + (int64_t)s * L;
-  double acc = 0.0;
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256)
-    acc += q[t] * Pt[t];
-  acc = wave_sum(acc);
+// ---- Gram-Schmidt of a block of new vectors among themselves ----
+// After the projection against the old basis (W = P - V V^T P), vector j is
+//   u_j = w_j - sum_{s < j accepted} q_s (q_s . p_j),  q_s = u_s / |u_s|,
+// accepted when |u_j| > 1e-8 (davidson.rs:162-168).  With q_s . p_j =
+// (u_s . p_j) / |u_s| the update reads u_j = w_j - sum_s (u_s . p_j) / |u_s|^2 u_s.
+// Step j updates w_j in place and, right-looking, forms its partial dots
+// u_j . p_l (l > j) and |u_j|^2 per block; a one-block kernel sums them.
+// scr layout: partial [nc][nc][NB_GS] (slot l == j holds |u_j|^2), then
+// D [nc][nc] (D[s][l] = u_s . p_l, D[j][j] = |u_j|^2), then acc [nc].
+constexpr int NB_GS = 1024;
+__global__ void __launch_bounds__(256) gs_block_update_kernel(const double* __restrict__ P, double* __restrict__ W,
+                                                              int64_t L, int j, int nc, double* __restrict__ scr) {
+  const double* D = scr + (int64_t)nc * nc * NB_GS;
+  const double* acc = D + (int64_t)nc * nc;
+  __shared__ double coef[64];
   __shared__ double red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) scratch[(int64_t)s * NB_NORM + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
-}
-
-__global__ void __launch_bounds__(256) gs_dots_final_kernel(const int* __restrict__ count, int max_new,
-                                                            double* __restrict__ scratch) {
-  const int s = blockIdx.x;
-  if (s >= *count) return;
-  double acc = wave_sum(scratch[(int64_t)s * NB_NORM + threadIdx.x]);
-  __shared__ double red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) scratch[(int64_t)NB_NORM * max_new + s] = (red[0] + red[1]) + (red[2] + red[3]);
-}
-
-__global__ void __launch_bounds__(256) gs_update_kernel(double* __restrict__ Wt, const double* __restrict__ Q,
-                                                        int64_t L, const int* __restrict__ count, int max_new,
-                                                        double* __restrict__ scratch) {
-  const int cnt = *count;
-  const double* d = scratch + (int64_t)NB_NORM * max_new;
-  double acc = 0.0;
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256) {
-    double w = Wt[t];
-    for (int s = 0; s < cnt; ++s) w -= d[s] * Q[(int64_t)s * L + t];
-    Wt[t] = w;
-    acc += w * w;
+  if (threadIdx.x < j) {
+    const int s = threadIdx.x;
+    coef[s] = acc[s] != 0.0 ? D[(int64_t)s * nc + j] / D[(int64_t)s * nc + s] : 0.0;
   }
-  acc = wave_sum(acc);
-  __shared__ double red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0)
-    scratch[(int64_t)NB_NORM * max_new + max_new + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+  double* wj = W + (int64_t)j * L;
+  double nn = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256) {
+    double u = wj[t];
+    for (int s2 = 0; s2 < j; ++s2) u -= coef[s2] * W[(int64_t)s2 * L + t];
+    wj[t] = u;
+    nn += u * u;
+  }
+  double* part = scr + (int64_t)j * nc * NB_GS;
+  for (int l = j; l < nc; ++l) {
+    double a = 0.0;
+    if (l == j) {
+      a = nn;
+    } else {
+      const double* pl = P + (int64_t)l * L;
+      for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256)
+        a += wj[t] * pl[t];
+    }
+    a = wave_sum(a);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) part[(int64_t)l * NB_GS + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+    __syncthreads();
+  }
 }
 
-__global__ void __launch_bounds__(256) gs_norm_final_kernel(int max_new, double* __restrict__ scratch) {
-  double* np = scratch + (int64_t)NB_NORM * max_new + max_new;
-  double acc = wave_sum(np[threadIdx.x]);
+// one block per l in [j, nc)
+__global__ void __launch_bounds__(256) gs_block_final_kernel(int j, int nc, double* __restrict__ scr) {
+  double* D = scr + (int64_t)nc * nc * NB_GS;
+  double* acc = D + (int64_t)nc * nc;
+  const int l = j + blockIdx.x;
+  const double* part = scr + ((int64_t)j * nc + l) * NB_GS;
   __shared__ double red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  double a = 0.0;
+  for (int b = threadIdx.x; b < NB_GS; b += 256) a += part[b];
+  a = wave_sum(a);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
   __syncthreads();
   if (threadIdx.x == 0) {
-    const double nrm = sqrt((red[0] + red[1]) + (red[2] + red[3]));
-    np[NB_NORM] = nrm;
-    np[NB_NORM + 1] = (nrm > 1.0e-8) ? 1.0 : 0.0;  // davidson.rs:165
+    const double v = (red[0] + red[1]) + (red[2] + red[3]);
+    D[(int64_t)j * nc + l] = v;
+    if (l == j) acc[j] = (sqrt(v) > 1.0e-8) ? 1.0 : 0.0;  // davidson.rs:165
   }
 }
 
-__global__ void __launch_bounds__(256) gs_append_kernel(const double* __restrict__ Wt, double* __restrict__ Q,
-                                                        int64_t L, const int* __restrict__ count, int max_new,
-                                                        const double* __restrict__ scratch) {
-  const double* np = scratch + (int64_t)NB_NORM * max_new + max_new;
-  if (np[NB_NORM + 1] == 0.0) return;
-  const double nrm = np[NB_NORM];
-  double* q = Q + (int64_t)(*count) * L;
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256)
-    q[t] = Wt[t] / nrm;
+// Q[rank(j)] = u_j / |u_j| for the accepted j, in order; *count = accepted
+__global__ void __launch_bounds__(256) gs_block_append_kernel(const double* __restrict__ W, double* __restrict__ Q,
+                                                              int64_t L, int nc, const double* __restrict__ scr,
+                                                              int* __restrict__ count) {
+  const double* D = scr + (int64_t)nc * nc * NB_GS;
+  const double* acc = D + (int64_t)nc * nc;
+  const int j = blockIdx.y;
+  int rank = 0;
+  for (int s = 0; s < j; ++s) rank += acc[s] != 0.0;
+  if (blockIdx.x == 0 && j == 0 && threadIdx.x == 0) {
+    int tot = 0;
+    for (int s = 0; s < nc; ++s) tot += acc[s] != 0.0;
+    *count = tot;
+  }
+  if (acc[j] == 0.0) return;
+  const double inv = 1.0 / sqrt(D[(int64_t)j * nc + j]);
+  const double* u = W + (int64_t)j * L;
+  double* q = Q + (int64_t)rank * L;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256) q[t] = u[t] * inv;
 }
 
-__global__ void gs_count_kernel(int* count, int max_new, const double* __restrict__ scratch) {
-  const double* np = scratch + (int64_t)NB_NORM * max_new + max_new;
-  if (threadIdx.x == 0 && np[NB_NORM + 1] != 0.0) *count += 1;
+void launch_gs_block(const double* P, double* W, double* Q, int64_t L, int nc, double* scratch, int* count,
+                     hipStream_t s) {
+  if (nc == 0) return;
+  for (int j = 0; j < nc; ++j) {
+    gs_block_update_kernel<<<NB_GS, 256, 0, s>>>(P, W, L, j, nc, scratch);
+    TDA_LAUNCH_CHECK();
+    gs_block_final_kernel<<<nc - j, 256, 0, s>>>(j, nc, scratch);
+    TDA_LAUNCH_CHECK();
+  }
+  gs_block_append_kernel<<<dim3((unsigned)grid_for(L, 256, 1024), (unsigned)nc), 256, 0, s>>>(W, Q, L, nc, scratch,
+                                                                                              count);
+  TDA_LAUNCH_CHECK();
 }
 
-void launch_gs_step(const double* Pt, double* Wt, double* Q, int64_t L, int* count, double* scratch, int max_new,
-                    hipStream_t s) {
-  gs_dots_kernel<<<dim3(NB_NORM, max_new), 256, 0, s>>>(Pt, Q, L, count, scratch);
-  TDA_LAUNCH_CHECK();
-  gs_dots_final_kernel<<<max_new, 256, 0, s>>>(count, max_new, scratch);
-  TDA_LAUNCH_CHECK();
-  gs_update_kernel<<<NB_NORM, 256, 0, s>>>(Wt, Q, L, count, max_new, scratch);
-  TDA_LAUNCH_CHECK();
-  gs_norm_final_kernel<<<1, 256, 0, s>>>(max_new, scratch);
-  TDA_LAUNCH_CHECK();
-  gs_append_kernel<<<grid_for(L, 256, 4096), 256, 0, s>>>(Wt, Q, L, count, max_new, scratch);
-  TDA_LAUNCH_CHECK();
-  gs_count_kernel<<<1, 64, 0, s>>>(count, max_new, scratch);
-  TDA_LAUNCH_CHECK();
-}
+size_t gs_block_scratch_doubles(int nc) { return (size_t)nc * nc * NB_GS + (size_t)nc * nc + nc; }
 
 __global__ void copy_kernel(const double* __restrict__ src, double* __restrict__ dst, int64_t n) {
   for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) dst[t] = src[t];
@@ -513,30 +521,19 @@ void launch_ao_operand(const double* C, const double* SC, int n_ao, int base, in
   TDA_LAUNCH_CHECK();
 }
 
-__global__ void __launch_bounds__(256) ao_gamma_kernel(const double* __restrict__ g, const int* __restrict__ atom,
-                                                       int n_ao, int n_atoms, double* __restrict__ Gt) {
-  const int64_t total = (int64_t)n_ao * n_ao;
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256)
-    Gt[t] = g[(int64_t)atom[t / n_ao] * n_atoms + atom[t % n_ao]];
-}
-
-void launch_ao_gamma(const double* gamma_lr, const int* atom, int n_ao, int n_atoms, double* Gt, hipStream_t s) {
-  ao_gamma_kernel<<<grid_for((int64_t)n_ao * n_ao), 256, 0, s>>>(gamma_lr, atom, n_ao, n_atoms, Gt);
-  TDA_LAUNCH_CHECK();
-}
-
 template <typename T>
 __global__ void __launch_bounds__(64) ao_charges_kernel(const T* __restrict__ M, int k, int n_ao,
-                                                        const int* __restrict__ ao_off, const double* __restrict__ Gt,
+                                                        const int* __restrict__ ao_off,
+                                                        const double* __restrict__ gamma_lr, int n_atoms,
                                                         double* __restrict__ u) {
   const int A = blockIdx.x;
   const int c = blockIdx.y;
   const int64_t n2 = 2 * (int64_t)n_ao;
   const T* Mc = M + (int64_t)c * n2 * n2;
+  // M = -1/4 gamma~ o D, gamma~_mu,mu = gamma_lr[A][A] for mu on atom A  =>  D = M / (-1/4 gamma_lr[A][A])
+  const double inv = -4.0 / gamma_lr[(int64_t)A * n_atoms + A];
   double acc = 0.0;
   for (int mu = ao_off[A] + threadIdx.x; mu < ao_off[A + 1]; mu += 64) {
-    // M = -1/4 gamma~ o D  =>  D = M / (-1/4 gamma~_mu,mu)
-    const double inv = -4.0 / Gt[(int64_t)mu * n_ao + mu];
     const double d2 = to_double<T>(Mc[(int64_t)mu * n2 + n_ao + mu]);
     const double d3 = to_double<T>(Mc[(int64_t)(n_ao + mu) * n2 + mu]);
     acc += 0.5 * (d2 + d3) * inv;
@@ -545,14 +542,22 @@ __global__ void __launch_bounds__(64) ao_charges_kernel(const T* __restrict__ M,
   if (threadIdx.x == 0) u[(int64_t)A * k + c] = acc;
 }
 
-void launch_ao_charges(const void* M, int dtype, int k, int n_ao, const int* ao_off, int n_atoms, const double* Gt,
-                       double* u, hipStream_t s) {
+void launch_ao_charges(const void* M, int dtype, int k, int n_ao, const int* ao_off, int n_atoms,
+                       const double* gamma_lr, double* u, hipStream_t s) {
   dim3 grid(n_atoms, k);
   switch (dtype) {
-    case PREC_FP64: ao_charges_kernel<double><<<grid, 64, 0, s>>>((const double*)M, k, n_ao, ao_off, Gt, u); break;
-    case PREC_FP32: ao_charges_kernel<float><<<grid, 64, 0, s>>>((const float*)M, k, n_ao, ao_off, Gt, u); break;
-    case PREC_BF16: ao_charges_kernel<__bf16><<<grid, 64, 0, s>>>((const __bf16*)M, k, n_ao, ao_off, Gt, u); break;
-    default: ao_charges_kernel<_Float16><<<grid, 64, 0, s>>>((const _Float16*)M, k, n_ao, ao_off, Gt, u); break;
+    case PREC_FP64:
+      ao_charges_kernel<double><<<grid, 64, 0, s>>>((const double*)M, k, n_ao, ao_off, gamma_lr, n_atoms, u);
+      break;
+    case PREC_FP32:
+      ao_charges_kernel<float><<<grid, 64, 0, s>>>((const float*)M, k, n_ao, ao_off, gamma_lr, n_atoms, u);
+      break;
+    case PREC_BF16:
+      ao_charges_kernel<__bf16><<<grid, 64, 0, s>>>((const __bf16*)M, k, n_ao, ao_off, gamma_lr, n_atoms, u);
+      break;
+    default:
+      ao_charges_kernel<_Float16><<<grid, 64, 0, s>>>((const _Float16*)M, k, n_ao, ao_off, gamma_lr, n_atoms, u);
+      break;
   }
   TDA_LAUNCH_CHECK();
 }

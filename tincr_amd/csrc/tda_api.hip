@@ -100,6 +100,12 @@ struct tda_ctx {
   DevBuf Hd, dxyz, post;
   DevBuf adjP, adjQ, adjX, adjD;  // adjoint work buffers
   int last_roots = 0;             // CI vectors of the last tda_run held in RZ
+  // thick restart of the FP16 / BF16 Davidson phase, in multiples of n_roots
+  // (environment TINCR_TDA_LP_CAP / TINCR_TDA_LP_KEEP); the FP32 and FP64
+  // phases keep the reference rule (cap 15 n, collapse to n)
+  int lp_cap_mult = 8, lp_keep_mult = 2;
+  int lp_keep(int n) const { return std::min(64, std::max(n, lp_keep_mult * n)); }
+  int lp_cap(int n) const { return std::min(15 * n, std::max(lp_cap_mult * n, lp_keep(n) + n)); }
   double pre_thr = 1e-4;
   int64_t ovp = 0;
   std::vector<double> xyz, h_de;
@@ -110,8 +116,8 @@ struct tda_ctx {
   int op_kind = 0;
   bool mo_built = false;  // q_oo / q_vv present
   // AO-factored operands: Rt [2n_ao][v_p], Lm [2n_ao][o_p], Rpt [v_p][2n_ao],
-  // Lpt [o_p][2n_ao], Gt = AO-expanded gamma_lr [n_ao][n_ao]
-  DevBuf ao_atom, Gt, aoRt, aoLm, aoRpt, aoLpt, aoXR, aoM, aoF;
+  // Lpt [o_p][2n_ao]; the AO-expanded gamma_lr is gathered through ao_atom
+  DevBuf ao_atom, aoRt, aoLm, aoRpt, aoLpt, aoXR, aoM, aoF;
   // Low-precision operand copies, one set per precision (index PREC_*): the
   // precision ladder of tda_run uses the context's precision and FP32.
   struct LpSet {
@@ -147,7 +153,7 @@ struct tda_ctx {
     for (DevBuf* b : {&C, &S, &SC, &orbe, &aooff, &gamma, &gamma_lr, &qov, &qoo, &qvv, &G, &de, &qov32,
                       &qvv_amax, &G_amax, &ws_gemm, &T, &xlp, &u, &uu, &vv32, &ax_tmp,
                       &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &P, &W, &norms, &npart, &idx, &count,
-                      &gs_scratch, &eigw, &ao_atom, &Gt, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
+                      &gs_scratch, &eigw, &ao_atom, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
                       &Cl, &Sl, &SCl, &Shl, &aooff_l, &ctW, &ctZ, &Hd, &dxyz, &post,
                       &adjP, &adjQ, &adjX, &adjD})
       b->release();
@@ -205,6 +211,8 @@ GemmArgs gargs(const void* A, const void* B, void* C, int64_t M, int64_t N, int6
   g.beta = 0.0;
   g.dm = plain(0); g.dmap = plain(1); g.xm = plain(0); g.xn = plain(0);
   g.hm = plain(0); g.hn = plain(0);
+  g.hidx = nullptr;
+  g.hmod = 1;
   return g;
 }
 
@@ -232,6 +240,12 @@ void build_charges(tda_ctx* c, const double* dC, const double* dSC, const int* d
 void gemm_sc(tda_ctx* c, const double* dS, const double* dC, double* dSC, int nao) {
   // SC = S C  (transition_charges.rs:35)
   GemmArgs g = gargs(dS, dC, dSC, nao, nao, nao, plain(nao), plain(1), plain(nao), plain(1), plain(nao), plain(1));
+  run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, g);
+}
+
+// uu = gamma u  (gamma [N][N], u / uu [N][k]) on the FP64 MFMA GEMM
+void gamma_times(tda_ctx* c, int N, int k) {
+  GemmArgs g = gargs(c->gamma.p, c->u.p, c->uu.p, N, k, N, plain(N), plain(1), plain(k), plain(1), plain(k), plain(1));
   run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, g);
 }
 
@@ -360,14 +374,12 @@ void build_ao_operands(tda_ctx* c) {
   c->aoLm.ensure((size_t)n2 * c->op * 8);
   c->aoRpt.ensure((size_t)n2 * c->vp * 8);
   c->aoLpt.ensure((size_t)n2 * c->op * 8);
-  c->Gt.ensure((size_t)nao * nao * 8);
   // Rt rows (SC_v; C_v), Lm rows (SC_o; C_o): they build D = L X R^T
   launch_ao_operand(C, SC, nao, c->vb, c->v, c->vp, 0, 0, c->aoRt.as<double>(), c->stream);
   launch_ao_operand(C, SC, nao, c->ob, c->o, c->op, 0, 0, c->aoLm.as<double>(), c->stream);
   // the back-contraction pairs each D block with the other operator: (C; SC), stored transposed
   launch_ao_operand(C, SC, nao, c->vb, c->v, c->vp, 1, 1, c->aoRpt.as<double>(), c->stream);
   launch_ao_operand(C, SC, nao, c->ob, c->o, c->op, 1, 1, c->aoLpt.as<double>(), c->stream);
-  launch_ao_gamma(c->gamma_lr.as<double>(), c->ao_atom.as<int>(), nao, c->N, c->Gt.as<double>(), c->stream);
 }
 
 void prepare(tda_ctx* c) {
@@ -423,7 +435,7 @@ void apply_A(tda_ctx* c, const double* X, int k, double* AX, int prec) {
                        plain(k), plain(1));
     run_gemm(c, DT_F32, DT_F64, EPI_STORE, true, true, g);
   }
-  launch_small_gemm(c->gamma.as<double>(), c->u.as<double>(), c->uu.as<double>(), N, k, N, c->stream);
+  gamma_times(c, N, k);
   {
     GemmArgs g;
     if (!lp) {
@@ -551,7 +563,7 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
                        plain(op), split(o, n2 * op, 1));
     run_gemm(c, dt, dt, EPI_STORE, true, true, g);
   }
-  // G2: M[c] = -1/4 Gt o (Lm XR[c]^T)
+  // G2: M[c] = -1/4 gamma~ o (Lm XR[c]^T)
   if (c->timing) TDA_HIP(hipEventRecord(e[0], c->stream));
   {
     GemmArgs g = gargs(Lm, c->aoXR.p, c->aoM.p, n2, n2, o, plain(op), plain(1), plain(1), plain(op), plain(n2),
@@ -560,17 +572,20 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
     g.b_batch = n2 * op;
     g.c_batch = n2 * n2;
     g.alpha = -0.25;
-    g.hsrc = c->Gt.as<double>();
-    g.hm = split(nao, 0, nao);
-    g.hn = split(nao, 0, 1);
+    // gamma~_{(s,mu),(t,nu)} = gamma_lr[atom(mu)][atom(nu)], gathered per row / column
+    g.hsrc = c->gamma_lr.as<double>();
+    g.hidx = c->ao_atom.as<int>();
+    g.hmod = nao;
+    g.hm = plain(N);
+    g.hn = plain(1);
     run_gemm(c, dt, dt, EPI_HADAMARD, true, true, g, 0, e[1]);
   }
   // Coulomb: u = q_ov x from the D diagonals, w = gamma u, added to M
   c->u.ensure((size_t)N * k * 8);
   c->uu.ensure((size_t)N * k * 8);
-  launch_ao_charges(c->aoM.p, lp ? prec : PREC_FP64, k, nao, c->aooff.as<int>(), N, c->Gt.as<double>(),
+  launch_ao_charges(c->aoM.p, lp ? prec : PREC_FP64, k, nao, c->aooff.as<int>(), N, c->gamma_lr.as<double>(),
                     c->u.as<double>(), c->stream);
-  launch_small_gemm(c->gamma.as<double>(), c->u.as<double>(), c->uu.as<double>(), N, k, N, c->stream);
+  gamma_times(c, N, k);
   launch_ao_add_coulomb(c->aoM.p, lp ? prec : PREC_FP64, k, nao, c->ao_atom.as<int>(), c->uu.as<double>(),
                         c->stream);
   // G3: F[c][a][(t,nu)] = sum_(s,mu) Rpt[a][(s,mu)] M[c][(t,nu)][(s,mu)]
@@ -655,7 +670,7 @@ void hf_derivatives(tda_ctx* c, const double* Xd, int n, double* dgam, double* d
   GemmArgs gu = gargs(c->qov.p, Xd, c->u.p, N, n, ovp, plain(ovp), plain(1), plain(1), plain(ovp), plain(n),
                       plain(1));
   run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, gu);
-  launch_small_gemm(c->gamma.as<double>(), c->u.as<double>(), c->uu.as<double>(), N, n, N, c->stream);
+  gamma_times(c, N, n);
   if (dgam) launch_hf_coulomb(c->u.as<double>(), c->uu.as<double>(), N, n, dgam, cq, c->stream);
   if (!c->has_lc || (!dglr && !dqoo && !dqvv)) return;
   build_mo_charges(c);
@@ -738,10 +753,21 @@ struct DavResult {
 // reference (used by the FP64 continuation, which starts from Ritz vectors).
 // stop_stall > 0: leave early when the max residual has not dropped below
 // 0.5x its best value for that many iterations (low-precision phase).
+// cap: maximum basis size; keep: Ritz vectors retained at a collapse.  The
+// reference rule is cap = 15 n (davidson.rs:74), keep = n with all products
+// recomputed after the collapse (:172-176, solvers/mod.rs:290-302).  With
+// keep > n (low-precision phases: thick restart) the basis and its products
+// are rotated onto the keep lowest Ritz vectors instead (V <- Y V,
+// AV <- Y AV), so no product is recomputed and the projection restarts as
+// diag(theta).
 DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec, bool first_can_converge,
-                   int stop_stall, std::vector<double>& h_evals, std::vector<double>& h_err) {
+                   int stop_stall, std::vector<double>& h_evals, std::vector<double>& h_err, int cap_in = 0,
+                   int keep = 0, int* kk_out = nullptr) {
   const int64_t L = c->ovp;
-  const int cap = 15 * n;  // max_space (davidson.rs:74)
+  const int cap = cap_in > 0 ? cap_in : 15 * n;  // max_space (davidson.rs:74)
+  const int ldh = 15 * n;                         // leading dimension of H and Y
+  if (keep < n) keep = n;
+  const bool thick = keep > n;
   double* V = c->V.as<double>();
   double* AV = c->AV.as<double>();
   double* H = c->H.as<double>();
@@ -757,27 +783,49 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
   std::vector<int> idx;
   double best = 1e300;
   int stall = 0;
+  bool rotated = false;
+  int n_eig_last = n, m_last = m;
+  // on exit in thick mode: the n_eig_last lowest Ritz vectors into W (a
+  // thicker start for the next phase)
+  auto finish = [&]() {
+    if (kk_out) *kk_out = n;
+    if (!thick || !kk_out || n_eig_last <= n) return;
+    GemmArgs gw = gargs(Y, V, W, n_eig_last, L, m_last, plain(ldh), plain(1), plain(L), plain(1), plain(L),
+                        plain(1));
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, gw);
+    *kk_out = n_eig_last;
+  };
   for (int it = 0; it < max_iter; ++it) {
     res.iters = it + 1;
     // 1. products for the new columns (cache semantics solvers/mod.rs:290-302)
-    if (m <= n_done) n_done = 0;
+    int proj_from = n_done;
+    if (rotated) {
+      proj_from = 0;
+      rotated = false;
+    } else if (m <= n_done) {
+      n_done = 0;
+      proj_from = 0;
+    }
+    const int n_eig = thick ? std::min(keep, m) : n;
+    n_eig_last = n_eig;
+    m_last = m;
     {
       Timer t;
       TDA_HIP(hipEventRecord(t.a, c->stream));
       products(c, V + (size_t)n_done * L, m - n_done, AV + (size_t)n_done * L, prec);
       TDA_HIP(hipEventRecord(t.b, c->stream));
       // 1.1 projection H[p][q] = V_p . AV_q for the new q (davidson.rs:89)
-      GemmArgs g = gargs(V, AV + (size_t)n_done * L, H + n_done, m, m - n_done, L, plain(L), plain(1), plain(1),
-                         plain(L), plain(cap), plain(1));
+      GemmArgs g = gargs(V, AV + (size_t)proj_from * L, H + proj_from, m, m - proj_from, L, plain(L), plain(1),
+                         plain(1), plain(L), plain(ldh), plain(1));
       run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
       n_done = m;
       // 2. subspace eigenproblem (davidson.rs:93-97)
-      launch_eigh(H, cap, m, n, c->eigw.as<double>(), c->evals.as<double>(), Y, cap, c->stream);
+      launch_eigh(H, ldh, m, n_eig, c->eigw.as<double>(), c->evals.as<double>(), Y, ldh, c->stream);
       // 3.1 Ritz vectors RZ = Y V, 3.2 residuals R = Y AV - u RZ, 3.3 norms
       // (davidson.rs:101-107)
-      GemmArgs gr = gargs(Y, V, RZ, n, L, m, plain(cap), plain(1), plain(L), plain(1), plain(L), plain(1));
+      GemmArgs gr = gargs(Y, V, RZ, n, L, m, plain(ldh), plain(1), plain(L), plain(1), plain(L), plain(1));
       run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, gr);
-      GemmArgs ga = gargs(Y, AV, R, n, L, m, plain(cap), plain(1), plain(L), plain(1), plain(L), plain(1));
+      GemmArgs ga = gargs(Y, AV, R, n, L, m, plain(ldh), plain(1), plain(L), plain(1), plain(L), plain(1));
       run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, ga);
       launch_residual(R, RZ, c->evals.as<double>(), n, L, c->stream);
       launch_norms(R, n, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
@@ -814,6 +862,7 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
     const int roots_lft = n - roots_cvd;
     if (roots_lft == 0 && (it > 0 || first_can_converge)) {  // davidson.rs:121
       res.converged = true;
+      finish();
       return res;
     }
     if (stop_stall > 0) {
@@ -821,6 +870,7 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
         best = max_err;
         stall = 0;
       } else if (++stall >= stop_stall) {
+        finish();
         return res;
       }
     }
@@ -844,43 +894,54 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
         g1.beta = 1.0;
         run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, g1);
         // ... then against the accepted new vectors, in order, 1e-8 drop
-        launch_zero_bytes(d_count, sizeof(int), c->stream);
-        for (int j = 0; j < nc; ++j)
-          launch_gs_step(P + (size_t)j * L, W + (size_t)j * L, V + (size_t)m * L, L, d_count,
-                         c->gs_scratch.as<double>(), n, c->stream);
+        launch_gs_block(P, W, V + (size_t)m * L, L, nc, c->gs_scratch.as<double>(), d_count, c->stream);
         int h_count = 0;
         TDA_HIP(hipMemcpyAsync(&h_count, d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         TDA_HIP(hipStreamSynchronize(c->stream));
         m += h_count;
       }
+    } else if (thick) {
+      // thick restart: rotate basis and products onto the kk lowest Ritz vectors
+      const int kk = n_eig;
+      GemmArgs gv = gargs(Y, V, W, kk, L, m, plain(ldh), plain(1), plain(L), plain(1), plain(L), plain(1));
+      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, gv);
+      GemmArgs gav = gargs(Y, AV, P, kk, L, m, plain(ldh), plain(1), plain(L), plain(1), plain(L), plain(1));
+      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, gav);
+      launch_copy(W, V, (int64_t)kk * L, c->stream);
+      launch_copy(P, AV, (int64_t)kk * L, c->stream);
+      m = n_done = dim_sub = kk;
+      rotated = true;
     } else {
-      // collapse to the Ritz vectors (davidson.rs:172-176)
-      dim_sub = dim_sub_origin;
+      // collapse to the Ritz vectors (davidson.rs:172-176); RZ holds n of
+      // them, so a phase started from more than n vectors collapses to n
+      dim_sub = std::min(dim_sub_origin, n);
       launch_copy(RZ, V, (int64_t)dim_sub * L, c->stream);
       m = dim_sub;
       n_done = 0;
     }
   }
+  finish();
   return res;
 }
 
-void alloc_davidson(tda_ctx* c, int n) {
+void alloc_davidson(tda_ctx* c, int n, int keep) {
   const int64_t L = c->ovp;
   const int cap = 15 * n;
+  keep = std::max(keep, n);
   c->V.ensure((size_t)cap * L * 8);
   c->AV.ensure((size_t)cap * L * 8);
   c->H.ensure((size_t)cap * cap * 8);
-  c->Y.ensure((size_t)n * cap * 8);
-  c->evals.ensure((size_t)n * 8);
+  c->Y.ensure((size_t)keep * cap * 8);
+  c->evals.ensure((size_t)keep * 8);
   c->RZ.ensure((size_t)n * L * 8);
   c->R.ensure((size_t)n * L * 8);
-  c->P.ensure((size_t)n * L * 8);
-  c->W.ensure((size_t)n * L * 8);
+  c->P.ensure((size_t)keep * L * 8);
+  c->W.ensure((size_t)keep * L * 8);
   c->norms.ensure((size_t)n * 8);
   c->npart.ensure((size_t)n * 256 * 8);
   c->idx.ensure((size_t)n * sizeof(int));
   c->count.ensure(sizeof(int) * 4);
-  c->gs_scratch.ensure(((size_t)256 * n + n + 256 + 8) * 8);
+  c->gs_scratch.ensure(gs_block_scratch_doubles(n) * 8);
   c->eigw.ensure(eigh_work_doubles(cap) * 8);
   c->stage2.ensure((size_t)cap * n * 8);
 }
@@ -949,6 +1010,10 @@ int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx**
   {
     const char* e = getenv("TINCR_TDA_VERBOSE");
     c->verbose = e && e[0] && e[0] != '0';
+    const char* ec = getenv("TINCR_TDA_LP_CAP");
+    const char* ek = getenv("TINCR_TDA_LP_KEEP");
+    if (ec && atoi(ec) >= 2) c->lp_cap_mult = atoi(ec);
+    if (ek && atoi(ek) >= 1) c->lp_keep_mult = atoi(ek);
   }
   if (hbm_budget_bytes) c->budget = hbm_budget_bytes;
   int rc = guarded(c, [&] {
@@ -1411,7 +1476,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     const double tq = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tq0).count();
     const int n = n_roots;
     const int64_t L = c->ovp;
-    alloc_davidson(c, n);
+    alloc_davidson(c, n, c->prec == PREC_FP64 || c->dense ? n : c->lp_keep(n));
     // initial subspace: unit vectors at the n smallest omega (solvers/utils.rs:17-26)
     std::vector<int64_t> order(ov);
     std::iota(order.begin(), order.end(), 0);
@@ -1436,20 +1501,27 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       // (stagnation); FP32 products then drive the residuals to `tol`; FP64
       // products of the n Ritz vectors verify (and, if needed, continue).
       // Each phase restarts from the previous phase's Ritz vectors.
+      // The FP16 / BF16 phase uses a thick restart (basis cap lp_cap, keep
+      // lp_keep Ritz vectors) and hands its lp_keep lowest Ritz vectors to the
+      // FP32 phase, which follows the reference rule.
       int used = 0;
       bool done = false;
+      int start = n, kk = n;
       if (c->prec == PREC_FP16 || c->prec == PREC_BF16) {
-        res = davidson(c, n, n, max_iter, tol, c->prec, false, 3, h_evals, h_err);
+        res = davidson(c, n, n, max_iter, tol, c->prec, false, 3, h_evals, h_err, c->lp_cap(n), c->lp_keep(n), &kk);
         used += res.iters;
         done = res.converged;
-        launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
+        if (kk > n) launch_copy(c->W.as<double>(), c->V.as<double>(), (int64_t)kk * L, c->stream);
+        else launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
+        start = kk;
       }
       if (!done) {
-        DavResult r32 = davidson(c, n, n, std::max(1, max_iter - used), tol, PREC_FP32, used > 0, 8, h_evals, h_err);
+        DavResult r32 = davidson(c, n, start, std::max(1, max_iter - used), tol, PREC_FP32, used > 0, 8, h_evals,
+                                 h_err);
         used += r32.iters;
         res.iters = used;
-        launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
       }
+      launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
       res64 = davidson(c, n, n, std::max(1, max_iter - used), tol, PREC_FP64, true, 0, h_evals, h_err);
       res.iters = used;
     }
