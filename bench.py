@@ -6,7 +6,7 @@ full solve from device-resident orbitals: transition-charge build, operator
 set-up, Davidson (low-precision products, then the FP64 continuation) and the
 outputs (omega, X, q_trans, dipoles, f) back on the host.
 
-    python bench.py [--steps K] [--warmup W] [--prec fp32|fp16|bf16|fp64]
+    python bench.py [--steps K] [--warmup W] [--prec fp16|fp32|bf16|fp64]
                     [--operator ao|charges] [--workload fmo]
 
 --workload fmo: the batched FMO case instead -- 1024 seeded fragments (H2O,
@@ -268,7 +268,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--prec", default="fp32", choices=["fp64", "fp32", "bf16", "fp16"])
+    ap.add_argument("--prec", default="fp16", choices=["fp64", "fp32", "bf16", "fp16"])
     ap.add_argument("--system", default="polyene200")
     ap.add_argument("--operator", default="ao", choices=["ao", "charges"])
     ap.add_argument("--states", type=int, default=20)

@@ -9,7 +9,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libtincr_tda.so")
+LIB_PATH = os.environ.get("TINCR_TDA_LIB") or os.path.join(_HERE, "lib", "libtincr_tda.so")
 
 TDA_OK = 0
 TDA_ERR_NOT_CONVERGED = 1

@@ -16,6 +16,17 @@
 #pragma once
 #include "gemm.h"
 
+// TDA_XCD_REMAP: XCD-grouped tile order (below).  Measured on the C200H202
+// AO products it did not beat the plain order, so it is off by default.
+#ifndef TDA_XCD_REMAP
+#define TDA_XCD_REMAP 0
+#endif
+// Two workgroups per CU: caps the registers so that the FP64 128x128 tile
+// (64 FP64 accumulators per lane) runs 2 waves per SIMD instead of 1.
+#ifndef TDA_GEMM_MIN_BLOCKS
+#define TDA_GEMM_MIN_BLOCKS 2
+#endif
+
 namespace tda {
 
 template <typename TI>
@@ -111,7 +122,7 @@ __device__ inline double epi_value(const GemmArgs& g, int zb, int64_t m, int64_t
 }
 
 template <typename TI, typename TO, typename Cfg, bool A_KC, bool B_KC, int EPI>
-__global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
+__global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs g) {
   using MF = Mfma<TI>;
   using TA = typename MF::TA;
   constexpr int BM = Cfg::BM, BN = Cfg::BN, WGM = Cfg::WGM, WGN = Cfg::WGN;
@@ -133,10 +144,36 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
   const int wm0 = (wave / WGN) * WTM;
   const int wn0 = (wave % WGN) * WTN;
 
+#if TDA_XCD_REMAP
+  // Workgroup w of the launch is dispatched to XCD w % 8, each XCD with its
+  // own L2.  Give every XCD a contiguous run of the tile sequence, and walk
+  // the tiles in groups of 8 tile rows, so that the tiles sharing A rows / B
+  // columns run on the same L2 at about the same time.
+  int64_t w = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+  {
+    const int64_t nwg = (int64_t)gridDim.x * gridDim.y * gridDim.z;
+    const int64_t q = nwg / 8, r = nwg % 8, x = w % 8, j = w / 8;
+    w = (x < r) ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
+  }
+  const int64_t per_z = (int64_t)gridDim.x * gridDim.y;
+  const int zz = (int)(w / per_z);
+  const int64_t t = w - (int64_t)zz * per_z;
+  constexpr int GM = 8;
+  const int64_t gsize = (int64_t)GM * gridDim.x;
+  const int64_t first_m = (t / gsize) * GM;
+  const int64_t gm = min((int64_t)gridDim.y - first_m, (int64_t)GM);
+  const int64_t tile_m = first_m + (t % gsize) % gm;
+  const int64_t tile_n = (t % gsize) / gm;
+  const int zb = zz / g.ksplit;
+  const int zs = zz % g.ksplit;
+  const int64_t m0 = tile_m * BM;
+  const int64_t n0 = tile_n * BN;
+#else
   const int zb = blockIdx.z / g.ksplit;
   const int zs = blockIdx.z % g.ksplit;
   const int64_t m0 = (int64_t)blockIdx.y * BM;
   const int64_t n0 = (int64_t)blockIdx.x * BN;
+#endif
   const TI* A = (const TI*)g.A + zb * g.a_batch;
   const TI* B = (const TI*)g.B + zb * g.b_batch;
 
@@ -337,7 +374,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
 
   // ---- epilogue ----
   if (g.ksplit > 1) {
-    TA* ws = (TA*)g.ws + (int64_t)blockIdx.z * g.M * g.N;
+    TA* ws = (TA*)g.ws + ((int64_t)zb * g.ksplit + zs) * g.M * g.N;
 #pragma unroll
     for (int i = 0; i < MI; ++i)
 #pragma unroll
