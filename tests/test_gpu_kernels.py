@@ -25,14 +25,17 @@ def _round(x, prec):
     return x
 
 
-# relative tolerance of the product (inputs rounded exactly as the device does;
-# what is left is the accumulation order / accumulator precision)
-TOL = {"fp64": 1e-13, "fp32": 1e-5, "bf16": 1e-5, "fp16": 1e-5}
+# relative tolerance of the product (inputs rounded as the device does; what is
+# left is the accumulation order / accumulator precision).  BF16: the device
+# and torch may round an exact halfway input differently (one BF16 ulp,
+# 2^-8 |a_ik b_kj| in one term); the largest shapes hit a few such inputs.
+TOL = {"fp64": 1e-13, "fp32": 1e-5, "bf16": 2.5e-5, "fp16": 1e-5}
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16", "fp16"])
 @pytest.mark.parametrize("a_kc,b_kc", [(True, False), (True, True), (False, False), (False, True)])
-@pytest.mark.parametrize("shape", [(16, 16, 64), (37, 53, 70), (130, 129, 257), (300, 20, 4100), (7, 600, 96)])
+@pytest.mark.parametrize("shape", [(16, 16, 64), (37, 53, 70), (130, 129, 257), (300, 20, 4100), (7, 600, 96),
+                                   (600, 520, 700), (520, 600, 4200)])
 def test_gemm(prec, a_kc, b_kc, shape):
     nat = _nat()
     print(f"gemm {prec} a_kc={a_kc} b_kc={b_kc} shape={shape}", flush=True)

@@ -1,6 +1,10 @@
 // MFMA GEMM: planning, dispatch and the split-K reduction.
 #include "gemm_impl.h"
 
+#ifndef TDA_BIG_K
+#define TDA_BIG_K 0
+#endif
+
 namespace tda {
 
 GemmPlan gemm_plan(DType in, int64_t M, int64_t N, int64_t K, int batch, bool, bool, int ksplit_hint) {
@@ -61,9 +65,18 @@ void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g,
   TDA_CHECK(g.M >= 0 && g.N >= 0 && g.K >= 0, 2, "gemm: negative extent");
   if (g.M == 0 || g.N == 0) return;
   if (g.batch < 1) g.batch = 1;
-  g.ksplit = plan.ksplit;
-  g.k_chunk = plan.k_chunk;
-  const int bk = bk_of(in);
+  GemmPlan pl = plan;
+  // 16-bit inputs on the big tile: 128-deep K slices when the K maps allow
+  // (TDA_BIG_K; off by default: it spills registers at two workgroups per CU)
+  if (TDA_BIG_K && pl.cfg == CFG_BIG && (in == DT_F16 || in == DT_BF16) && g.K >= 256 &&
+      (g.ak.div >= g.K || g.ak.div % 128 == 0) && (g.bk.div >= g.K || g.bk.div % 128 == 0)) {
+    pl.cfg = CFG_BIG_K;
+    pl.k_chunk = round_up(pl.k_chunk, 128);
+    pl.ksplit = (int)ceil_div(g.K, pl.k_chunk);  // <= the planned split: the workspace suffices
+  }
+  g.ksplit = pl.ksplit;
+  g.k_chunk = pl.k_chunk;
+  const int bk = bk_of(in, pl.cfg);
   TDA_CHECK(g.k_chunk % bk == 0, 2, "gemm: k_chunk must be a multiple of BK");
   // K tiles must not straddle a K-map boundary
   TDA_CHECK(g.ak.div >= g.K || g.ak.div % bk == 0, 2, "gemm: A K-map boundary not tile aligned");
@@ -73,15 +86,15 @@ void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g,
   else TDA_CHECK(g.am.s0 == 1 && g.am.div >= g.M, 2, "gemm: A M-contiguous needs plain unit am");
   if (b_kc) TDA_CHECK(g.bk.s0 == 1, 2, "gemm: B K-contiguous needs bk.s0 == 1");
   else TDA_CHECK(g.bn.s0 == 1 && g.bn.div >= g.N, 2, "gemm: B N-contiguous needs plain unit bn");
-  if (plan.ksplit > 1) TDA_CHECK(g.ws != nullptr, 2, "gemm: split-K needs a workspace");
+  if (pl.ksplit > 1) TDA_CHECK(g.ws != nullptr, 2, "gemm: split-K needs a workspace");
   switch (in) {
-    case DT_F64: gemm_dispatch_f64(epi, out, plan.cfg, a_kc, b_kc, g, stream); break;
-    case DT_F32: gemm_dispatch_f32(epi, out, plan.cfg, a_kc, b_kc, g, stream); break;
-    case DT_F16: gemm_dispatch_f16(epi, out, plan.cfg, a_kc, b_kc, g, stream); break;
-    default: gemm_dispatch_bf16(epi, out, plan.cfg, a_kc, b_kc, g, stream); break;
+    case DT_F64: gemm_dispatch_f64(epi, out, pl.cfg, a_kc, b_kc, g, stream); break;
+    case DT_F32: gemm_dispatch_f32(epi, out, pl.cfg, a_kc, b_kc, g, stream); break;
+    case DT_F16: gemm_dispatch_f16(epi, out, pl.cfg, a_kc, b_kc, g, stream); break;
+    default: gemm_dispatch_bf16(epi, out, pl.cfg, a_kc, b_kc, g, stream); break;
   }
   if (after_main) TDA_HIP(hipEventRecord(after_main, stream));
-  if (plan.ksplit > 1) {
+  if (pl.ksplit > 1) {
     if (epi == EPI_DIAG) dispatch_reduce<EPI_DIAG>(in, out, g, stream);
     else if (epi == EPI_HADAMARD) dispatch_reduce<EPI_HADAMARD>(in, out, g, stream);
     else dispatch_reduce<EPI_STORE>(in, out, g, stream);

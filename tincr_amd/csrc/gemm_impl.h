@@ -14,6 +14,8 @@
 // LDS holds A as [k][m] (E=1 types) or [m][k] (E=8 types) so that fragment
 // reads are single ds_read instructions.
 #pragma once
+#include <type_traits>
+
 #include "gemm.h"
 
 // TDA_XCD_REMAP: XCD-grouped tile order (below).  Measured on the C200H202
@@ -84,21 +86,24 @@ struct alignas(16) VecN {
 
 // Tile configurations.
 struct CfgBig {  // 128x128, waves 2x2 (64x64 each)
-  static constexpr int BM = 128, BN = 128, WGM = 2, WGN = 2;
+  static constexpr int BM = 128, BN = 128, WGM = 2, WGN = 2, BK16 = 64;
+};
+struct CfgBigK {  // 128x128 with 128-deep K slices (16-bit inputs): twice the MFMA work per barrier pair
+  static constexpr int BM = 128, BN = 128, WGM = 2, WGN = 2, BK16 = 128;
 };
 struct CfgSkinnyN {  // 64x32, waves 2x2 (32x16 each): N <= 32-ish
-  static constexpr int BM = 64, BN = 32, WGM = 2, WGN = 2;
+  static constexpr int BM = 64, BN = 32, WGM = 2, WGN = 2, BK16 = 64;
 };
 struct CfgSkinnyM {  // 32x128, waves 1x4 (32x32 each): M <= 32-ish
-  static constexpr int BM = 32, BN = 128, WGM = 1, WGN = 4;
+  static constexpr int BM = 32, BN = 128, WGM = 1, WGN = 4, BK16 = 64;
 };
 struct CfgSmall {  // 64x64, waves 2x2 (32x32 each)
-  static constexpr int BM = 64, BN = 64, WGM = 2, WGN = 2;
+  static constexpr int BM = 64, BN = 64, WGM = 2, WGN = 2, BK16 = 64;
 };
 
-template <typename TI>
+template <typename TI, typename Cfg>
 struct BKOf {
-  static constexpr int value = Mfma<TI>::E == 1 ? 16 : 64;
+  static constexpr int value = Mfma<TI>::E == 1 ? 16 : Cfg::BK16;
 };
 
 template <typename TO>
@@ -126,7 +131,7 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
   using MF = Mfma<TI>;
   using TA = typename MF::TA;
   constexpr int BM = Cfg::BM, BN = Cfg::BN, WGM = Cfg::WGM, WGN = Cfg::WGN;
-  constexpr int BK = BKOf<TI>::value;
+  constexpr int BK = BKOf<TI, Cfg>::value;
   constexpr int E = MF::E, KSTEP = MF::KSTEP;
   constexpr int WTM = BM / WGM, WTN = BN / WGN;
   constexpr int MI = WTM / 16, NI = WTN / 16;
@@ -192,8 +197,12 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
   // Precompute the (tile-invariant) row/col parts of addresses.
   // For A_KC: vector idx -> (m = idx / (BK/VEC), kv = idx % (BK/VEC))
   // else      : vector idx -> (k = idx / (BM/VEC), mv = idx % (BM/VEC))
+  // Row / column parts of the addresses and the per-vector eligibility for
+  // the 16-byte load path (tile-invariant; the K part is checked per tile).
+  const bool a_base_al = (((uintptr_t)A) & 15) == 0;
+  const bool b_base_al = (((uintptr_t)B) & 15) == 0;
   int64_t a_row_off[A_PER];
-  bool a_row_ok[A_PER];
+  bool a_row_ok[A_PER], a_vec_ok[A_PER];
 #pragma unroll
   for (int i = 0; i < A_PER; ++i) {
     int idx = tid + i * 256;
@@ -202,9 +211,11 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
     else m = m0 + (idx % (BM / VEC)) * VEC;
     a_row_ok[i] = (idx < A_VECS) && (m < g.M);
     a_row_off[i] = a_row_ok[i] ? g.am(m) : 0;
+    if (A_KC) a_vec_ok[i] = a_row_ok[i] && a_base_al && (a_row_off[i] % VEC == 0);
+    else a_vec_ok[i] = a_row_ok[i] && a_base_al && (m + VEC <= g.M) && (g.ak.s0 % VEC == 0);
   }
   int64_t b_col_off[B_PER];
-  bool b_col_ok[B_PER];
+  bool b_col_ok[B_PER], b_vec_ok[B_PER];
 #pragma unroll
   for (int i = 0; i < B_PER; ++i) {
     int idx = tid + i * 256;
@@ -213,71 +224,73 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
     else n = n0 + (idx % (BN / VEC)) * VEC;
     b_col_ok[i] = (idx < B_VECS) && (n < g.N);
     b_col_off[i] = b_col_ok[i] ? g.bn(n) : 0;
+    if (B_KC) b_vec_ok[i] = b_col_ok[i] && b_base_al && (b_col_off[i] % VEC == 0);
+    else b_vec_ok[i] = b_col_ok[i] && b_base_al && (n + VEC <= g.N) && (g.bk.s0 % VEC == 0);
   }
 
   auto load_tile = [&](int kt) {
     const int64_t k0 = kbeg + (int64_t)kt * BK;
     // K-map evaluated once per tile (tiles never straddle a K-map boundary).
     const int64_t ak0 = g.ak(k0), bk0 = g.bk(k0);
+    const bool full_k = k0 + BK <= kend;
+    const bool a_k_al = (ak0 % VEC) == 0, b_k_al = (bk0 % VEC) == 0;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
       int idx = tid + i * 256;
-      VecN<TI, VEC> v;
-#pragma unroll
-      for (int e = 0; e < VEC; ++e) v.v[e] = (TI)0;
       if (A_KC) {
-        int kk = (idx % (BK / VEC)) * VEC;
-        int64_t k = k0 + kk;
-        if (a_row_ok[i]) {
-          const TI* p = A + a_row_off[i] + ak0 + kk * g.ak.s0;
-          if (k + VEC <= kend && (((uintptr_t)p & 15) == 0)) v = *(const VecN<TI, VEC>*)p;
-          else
-            for (int e = 0; e < VEC; ++e)
-              if (k + e < kend) v.v[e] = p[e];
+        const int kk = (idx % (BK / VEC)) * VEC;
+        const TI* p = A + a_row_off[i] + ak0 + kk;  // ak.s0 == 1 (host check)
+        if (a_vec_ok[i] && full_k && a_k_al) {
+          ra[i] = *(const VecN<TI, VEC>*)p;
+        } else {
+          VecN<TI, VEC> v;
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) v.v[e] = (a_row_ok[i] && k0 + kk + e < kend) ? p[e] : (TI)0;
+          ra[i] = v;
         }
       } else {
-        int kk = idx / (BM / VEC);
-        int64_t k = k0 + kk;
-        int64_t m = m0 + (idx % (BM / VEC)) * VEC;
-        if (idx < A_VECS && k < kend && m < g.M) {
-          const TI* p = A + a_row_off[i] + ak0 + g.ak.s0 * kk;
-          if (m + VEC <= g.M && (((uintptr_t)p & 15) == 0)) v = *(const VecN<TI, VEC>*)p;
-          else
-            for (int e = 0; e < VEC; ++e)
-              if (m + e < g.M) v.v[e] = p[e];
+        const int kk = idx / (BM / VEC);
+        const int64_t m = m0 + (idx % (BM / VEC)) * VEC;
+        const TI* p = A + a_row_off[i] + ak0 + g.ak.s0 * kk;
+        const bool krow = (idx < A_VECS) && (k0 + kk < kend);
+        if (a_vec_ok[i] && krow && a_k_al) {
+          ra[i] = *(const VecN<TI, VEC>*)p;
+        } else {
+          VecN<TI, VEC> v;
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) v.v[e] = (krow && a_row_ok[i] && m + e < g.M) ? p[e] : (TI)0;
+          ra[i] = v;
         }
       }
-      ra[i] = v;
     }
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
       int idx = tid + i * 256;
-      VecN<TI, VEC> v;
-#pragma unroll
-      for (int e = 0; e < VEC; ++e) v.v[e] = (TI)0;
       if (B_KC) {
-        int kk = (idx % (BK / VEC)) * VEC;
-        int64_t k = k0 + kk;
-        if (b_col_ok[i]) {
-          const TI* p = B + b_col_off[i] + bk0 + kk * g.bk.s0;
-          if (k + VEC <= kend && (((uintptr_t)p & 15) == 0)) v = *(const VecN<TI, VEC>*)p;
-          else
-            for (int e = 0; e < VEC; ++e)
-              if (k + e < kend) v.v[e] = p[e];
+        const int kk = (idx % (BK / VEC)) * VEC;
+        const TI* p = B + b_col_off[i] + bk0 + kk;  // bk.s0 == 1 (host check)
+        if (b_vec_ok[i] && full_k && b_k_al) {
+          rb[i] = *(const VecN<TI, VEC>*)p;
+        } else {
+          VecN<TI, VEC> v;
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) v.v[e] = (b_col_ok[i] && k0 + kk + e < kend) ? p[e] : (TI)0;
+          rb[i] = v;
         }
       } else {
-        int kk = idx / (BN / VEC);
-        int64_t k = k0 + kk;
-        int64_t n = n0 + (idx % (BN / VEC)) * VEC;
-        if (idx < B_VECS && k < kend && n < g.N) {
-          const TI* p = B + b_col_off[i] + bk0 + g.bk.s0 * kk;
-          if (n + VEC <= g.N && (((uintptr_t)p & 15) == 0)) v = *(const VecN<TI, VEC>*)p;
-          else
-            for (int e = 0; e < VEC; ++e)
-              if (n + e < g.N) v.v[e] = p[e];
+        const int kk = idx / (BN / VEC);
+        const int64_t n = n0 + (idx % (BN / VEC)) * VEC;
+        const TI* p = B + b_col_off[i] + bk0 + g.bk.s0 * kk;
+        const bool krow = (idx < B_VECS) && (k0 + kk < kend);
+        if (b_vec_ok[i] && krow && b_k_al) {
+          rb[i] = *(const VecN<TI, VEC>*)p;
+        } else {
+          VecN<TI, VEC> v;
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) v.v[e] = (krow && b_col_ok[i] && n + e < g.N) ? p[e] : (TI)0;
+          rb[i] = v;
         }
       }
-      rb[i] = v;
     }
   };
 
@@ -390,6 +403,7 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
   TO* C = (TO*)g.C + zb * g.c_batch;
   double alpha = g.alpha;
   if (g.batch_scale) alpha *= (double)g.batch_scale[zb];
+  const float alpha_f = (float)alpha;
   // Column parts of the output / epilogue addresses, once per column.
   int64_t cn_off[NI], en_off[NI], xn_off[NI];
   bool n_ok[NI];
@@ -429,15 +443,28 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
       for (int j = 0; j < NI; ++j) {
         if (!n_ok[j]) continue;
         TO* p = C + cmo + cn_off[j];
-        double v = alpha * (double)acc[i][j][r];
-        if constexpr (EPI == EPI_DIAG) {
-          v += g.diag[emo + en_off[j]] * xs[xmo + xn_off[j]];
-        } else if constexpr (EPI == EPI_HADAMARD) {
-          v *= g.hsrc[emo + en_off[j]];
+        if constexpr (std::is_same<TO, double>::value) {
+          double v = alpha * (double)acc[i][j][r];
+          if constexpr (EPI == EPI_DIAG) {
+            v += g.diag[emo + en_off[j]] * xs[xmo + xn_off[j]];
+          } else if constexpr (EPI == EPI_HADAMARD) {
+            v *= g.hsrc[emo + en_off[j]];
+          } else {
+            if (g.beta != 0.0) v += g.beta * to_double<TO>(*p);
+          }
+          *p = v;
         } else {
-          if (g.beta != 0.0) v += g.beta * to_double<TO>(*p);
+          // FP32 / FP16 / BF16 outputs: the epilogue in FP32 (the accumulator's precision)
+          float v = alpha_f * (float)acc[i][j][r];
+          if constexpr (EPI == EPI_HADAMARD) {
+            v *= (float)g.hsrc[emo + en_off[j]];
+          } else if constexpr (EPI == EPI_STORE) {
+            if (g.beta != 0.0) v += (float)g.beta * (float)to_double<TO>(*p);
+          } else {
+            v += (float)(g.diag[emo + en_off[j]] * xs[xmo + xn_off[j]]);
+          }
+          *p = (TO)v;
         }
-        store_out<TO>(p, v);
       }
     }
 }
@@ -491,11 +518,18 @@ __global__ void __launch_bounds__(256) splitk_reduce_kernel(GemmArgs g) {
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-enum CfgId : int { CFG_BIG = 0, CFG_SKINNY_N = 1, CFG_SKINNY_M = 2, CFG_SMALL = 3 };
+enum CfgId : int { CFG_BIG = 0, CFG_SKINNY_N = 1, CFG_SKINNY_M = 2, CFG_SMALL = 3, CFG_BIG_K = 4 };
 
-inline int cfg_bm(int c) { return c == CFG_BIG ? 128 : c == CFG_SKINNY_N ? 64 : c == CFG_SKINNY_M ? 32 : 64; }
-inline int cfg_bn(int c) { return c == CFG_BIG ? 128 : c == CFG_SKINNY_N ? 32 : c == CFG_SKINNY_M ? 128 : 64; }
-inline int bk_of(DType in) { return (in == DT_F64 || in == DT_F32) ? 16 : 64; }
+inline int cfg_bm(int c) {
+  return (c == CFG_BIG || c == CFG_BIG_K) ? 128 : c == CFG_SKINNY_N ? 64 : c == CFG_SKINNY_M ? 32 : 64;
+}
+inline int cfg_bn(int c) {
+  return (c == CFG_BIG || c == CFG_BIG_K) ? 128 : c == CFG_SKINNY_N ? 32 : c == CFG_SKINNY_M ? 128 : 64;
+}
+inline int bk_of(DType in, int cfg = CFG_BIG) {
+  if (in == DT_F64 || in == DT_F32) return 16;
+  return cfg == CFG_BIG_K ? 128 : 64;
+}
 inline size_t acc_bytes(DType in) { return in == DT_F64 ? 8 : 4; }
 
 template <typename TI, typename TO, typename Cfg, int EPI>
@@ -513,6 +547,10 @@ template <typename TI, typename TO, int EPI>
 void launch_t(int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s) {
   switch (cfg) {
     case CFG_BIG: launch_cfg<TI, TO, CfgBig, EPI>(a_kc, b_kc, g, s); break;
+    case CFG_BIG_K:
+      if constexpr (Mfma<TI>::E == 8) launch_cfg<TI, TO, CfgBigK, EPI>(a_kc, b_kc, g, s);
+      else launch_cfg<TI, TO, CfgBig, EPI>(a_kc, b_kc, g, s);
+      break;
     case CFG_SKINNY_N: launch_cfg<TI, TO, CfgSkinnyN, EPI>(a_kc, b_kc, g, s); break;
     case CFG_SKINNY_M: launch_cfg<TI, TO, CfgSkinnyM, EPI>(a_kc, b_kc, g, s); break;
     default: launch_cfg<TI, TO, CfgSmall, EPI>(a_kc, b_kc, g, s); break;
