@@ -752,7 +752,11 @@ struct DavResult {
 // vectors already in V.  `first_can_converge` relaxes the i>0 rule of the
 // reference (used by the FP64 continuation, which starts from Ritz vectors).
 // stop_stall > 0: leave early when the max residual has not dropped below
-// 0.5x its best value for that many iterations (low-precision phase).
+// stall_factor x its best value for that many iterations (low-precision
+// phases: 0.5 for FP16/BF16, which hit their accuracy floor quickly; 0.9 for
+// FP32, which can reach the tolerance and only stops on true stagnation).
+// Stagnation is only counted while max residual < stall_floor (FP32: 10 tol,
+// so a slowly converging spectrum is not mistaken for the precision floor).
 // cap: maximum basis size; keep: Ritz vectors retained at a collapse.  The
 // reference rule is cap = 15 n (davidson.rs:74), keep = n with all products
 // recomputed after the collapse (:172-176, solvers/mod.rs:290-302).  With
@@ -762,12 +766,13 @@ struct DavResult {
 // diag(theta).
 DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec, bool first_can_converge,
                    int stop_stall, std::vector<double>& h_evals, std::vector<double>& h_err, int cap_in = 0,
-                   int keep = 0, int* kk_out = nullptr) {
+                   int keep = 0, int* kk_out = nullptr, double stall_factor = 0.5,
+                   double stall_floor = 1e300, bool thick_restart = true) {
   const int64_t L = c->ovp;
   const int cap = cap_in > 0 ? cap_in : 15 * n;  // max_space (davidson.rs:74)
   const int ldh = 15 * n;                         // leading dimension of H and Y
   if (keep < n) keep = n;
-  const bool thick = keep > n;
+  const bool thick = thick_restart && keep > n;
   double* V = c->V.as<double>();
   double* AV = c->AV.as<double>();
   double* H = c->H.as<double>();
@@ -787,13 +792,20 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
   int n_eig_last = n, m_last = m;
   // on exit in thick mode: the n_eig_last lowest Ritz vectors into W (a
   // thicker start for the next phase)
-  auto finish = [&]() {
+  auto finish = [&](bool converged) {
     if (kk_out) *kk_out = n;
-    if (!thick || !kk_out || n_eig_last <= n) return;
-    GemmArgs gw = gargs(Y, V, W, n_eig_last, L, m_last, plain(ldh), plain(1), plain(L), plain(1), plain(L),
-                        plain(1));
+    if (!kk_out || keep <= n || converged) return;
+    int kk = n_eig_last;
+    if (kk < std::min(keep, m_last)) {
+      // the phase kept the reference rule (n eigenpairs per iteration): one
+      // more eigensolve of the final projection for the export
+      kk = std::min(keep, m_last);
+      launch_eigh(H, ldh, m_last, kk, c->eigw.as<double>(), c->evals.as<double>(), Y, ldh, c->stream);
+    }
+    if (kk <= n) return;
+    GemmArgs gw = gargs(Y, V, W, kk, L, m_last, plain(ldh), plain(1), plain(L), plain(1), plain(L), plain(1));
     run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, gw);
-    *kk_out = n_eig_last;
+    *kk_out = kk;
   };
   for (int it = 0; it < max_iter; ++it) {
     res.iters = it + 1;
@@ -862,15 +874,15 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
     const int roots_lft = n - roots_cvd;
     if (roots_lft == 0 && (it > 0 || first_can_converge)) {  // davidson.rs:121
       res.converged = true;
-      finish();
+      finish(true);
       return res;
     }
-    if (stop_stall > 0) {
-      if (max_err < 0.5 * best) {
+    if (stop_stall > 0 && max_err < stall_floor) {
+      if (max_err < stall_factor * best) {
         best = max_err;
         stall = 0;
       } else if (++stall >= stop_stall) {
-        finish();
+        finish(false);
         return res;
       }
     }
@@ -920,7 +932,7 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       n_done = 0;
     }
   }
-  finish();
+  finish(false);
   return res;
 }
 
@@ -1515,14 +1527,21 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
         else launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
         start = kk;
       }
+      // The FP32 phase keeps the reference rule; if it stops unconverged it
+      // exports its lp_keep lowest Ritz vectors so that the FP64 phase does
+      // not restart from a thin basis.
+      int start64 = n;
       if (!done) {
+        int kk32 = n;
         DavResult r32 = davidson(c, n, start, std::max(1, max_iter - used), tol, PREC_FP32, used > 0, 8, h_evals,
-                                 h_err);
+                                 h_err, 0, c->lp_keep(n), &kk32, 0.9, 10.0 * tol, false);
         used += r32.iters;
         res.iters = used;
+        if (!r32.converged && kk32 > n) start64 = kk32;
       }
-      launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
-      res64 = davidson(c, n, n, std::max(1, max_iter - used), tol, PREC_FP64, true, 0, h_evals, h_err);
+      if (start64 > n) launch_copy(c->W.as<double>(), c->V.as<double>(), (int64_t)start64 * L, c->stream);
+      else launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
+      res64 = davidson(c, n, start64, std::max(1, max_iter - used), tol, PREC_FP64, true, 0, h_evals, h_err);
       res.iters = used;
     }
     // outputs (tda/mod.rs:88-126)
