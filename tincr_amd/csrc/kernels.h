@@ -64,15 +64,18 @@ void launch_small_gemm(const double* A, const double* B, double* C, int M, int N
 
 // ---- AO-factored operator ----
 // out[r][col] (r = s*n_ao + mu, s in {0,1}) = (s == swap ? SC : C)[mu][base + col]
-// for col < P, 0 for P <= col < P_pad.  transpose=1 writes out[col][r].
+// for col < P, 0 for P <= col < P_pad.  transpose=1 writes out[col][r] with row
+// stride ld >= 2 n_ao (zeros in columns r >= 2 n_ao); ld is ignored otherwise.
 void launch_ao_operand(const double* C, const double* SC, int n_ao, int base, int P, int P_pad, int swap,
-                       int transpose, double* out, hipStream_t s);
-// Coulomb from the D-blocks of M (M = -1/4 gamma~ o D, 2n_ao x 2n_ao per vector,
-// element type dtype): u[A*k + c] = 1/2 sum_{mu in A} (D[(0,mu),(1,mu)] + D[(1,mu),(0,mu)])
-void launch_ao_charges(const void* M, int dtype, int k, int n_ao, const int* ao_off, int n_atoms, const double* gamma_lr,
-                       double* u, hipStream_t s);
+                       int transpose, int64_t ld, double* out, hipStream_t s);
+// Coulomb from the D-blocks of M (M = -1/4 gamma~ o D, 2n_ao x 2n_ao per vector
+// with row stride ld, element type dtype):
+// u[A*k + c] = 1/2 sum_{mu in A} (D[(0,mu),(1,mu)] + D[(1,mu),(0,mu)])
+void launch_ao_charges(const void* M, int dtype, int k, int n_ao, int64_t ld, const int* ao_off, int n_atoms,
+                       const double* gamma_lr, double* u, hipStream_t s);
 // M[c][(0,mu),(1,mu)] += w[atom(mu)*k + c] and M[c][(1,mu),(0,mu)] += same
-void launch_ao_add_coulomb(void* M, int dtype, int k, int n_ao, const int* atom, const double* w, hipStream_t s);
+void launch_ao_add_coulomb(void* M, int dtype, int k, int n_ao, int64_t ld, const int* atom, const double* w,
+                           hipStream_t s);
 
 // ---- subspace eigensolver (eigh, davidson.rs:93) ----
 // H: m x m symmetric (upper triangle read), leading dim ldh; work >= eigh_work_doubles(m)

@@ -498,72 +498,77 @@ void launch_zero_runs(void* base, int64_t rows, int64_t ld, int64_t c0, int64_t 
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) ao_operand_kernel(const double* __restrict__ C, const double* __restrict__ SC,
                                                          int n_ao, int base, int P, int P_pad, int swap, int transpose,
-                                                         double* __restrict__ out) {
+                                                         int64_t ld, double* __restrict__ out) {
   const int64_t n2 = 2 * (int64_t)n_ao;
-  const int64_t total = n2 * P_pad;
+  // transposed: [P_pad][ld] with the ld - n2 pad columns zeroed; else [n2][P_pad]
+  const int64_t rows = transpose ? ld : n2;
+  const int64_t total = rows * P_pad;
   for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
     const int64_t r = t / P_pad;
     const int col = (int)(t % P_pad);
-    const int sidx = (int)(r / n_ao);
-    const int64_t mu = r % n_ao;
-    const double* src = (sidx == swap) ? SC : C;
-    const double v = (col < P) ? src[mu * n_ao + base + col] : 0.0;
-    if (transpose) out[(int64_t)col * n2 + r] = v;
+    double v = 0.0;
+    if (r < n2 && col < P) {
+      const int sidx = (int)(r / n_ao);
+      const int64_t mu = r % n_ao;
+      const double* src = (sidx == swap) ? SC : C;
+      v = src[mu * n_ao + base + col];
+    }
+    if (transpose) out[(int64_t)col * ld + r] = v;
     else out[t] = v;
   }
 }
 
 void launch_ao_operand(const double* C, const double* SC, int n_ao, int base, int P, int P_pad, int swap,
-                       int transpose, double* out, hipStream_t s) {
-  const int64_t total = 2 * (int64_t)n_ao * P_pad;
+                       int transpose, int64_t ld, double* out, hipStream_t s) {
+  const int64_t total = (transpose ? ld : 2 * (int64_t)n_ao) * P_pad;
   if (total == 0) return;
-  ao_operand_kernel<<<grid_for(total), 256, 0, s>>>(C, SC, n_ao, base, P, P_pad, swap, transpose, out);
+  ao_operand_kernel<<<grid_for(total), 256, 0, s>>>(C, SC, n_ao, base, P, P_pad, swap, transpose, ld, out);
   TDA_LAUNCH_CHECK();
 }
 
 template <typename T>
-__global__ void __launch_bounds__(64) ao_charges_kernel(const T* __restrict__ M, int k, int n_ao,
+__global__ void __launch_bounds__(64) ao_charges_kernel(const T* __restrict__ M, int k, int n_ao, int64_t ld,
                                                         const int* __restrict__ ao_off,
                                                         const double* __restrict__ gamma_lr, int n_atoms,
                                                         double* __restrict__ u) {
   const int A = blockIdx.x;
   const int c = blockIdx.y;
   const int64_t n2 = 2 * (int64_t)n_ao;
-  const T* Mc = M + (int64_t)c * n2 * n2;
+  const T* Mc = M + (int64_t)c * n2 * ld;
   // M = -1/4 gamma~ o D, gamma~_mu,mu = gamma_lr[A][A] for mu on atom A  =>  D = M / (-1/4 gamma_lr[A][A])
   const double inv = -4.0 / gamma_lr[(int64_t)A * n_atoms + A];
   double acc = 0.0;
   for (int mu = ao_off[A] + threadIdx.x; mu < ao_off[A + 1]; mu += 64) {
-    const double d2 = to_double<T>(Mc[(int64_t)mu * n2 + n_ao + mu]);
-    const double d3 = to_double<T>(Mc[(int64_t)(n_ao + mu) * n2 + mu]);
+    const double d2 = to_double<T>(Mc[(int64_t)mu * ld + n_ao + mu]);
+    const double d3 = to_double<T>(Mc[(int64_t)(n_ao + mu) * ld + mu]);
     acc += 0.5 * (d2 + d3) * inv;
   }
   acc = wave_sum(acc);
   if (threadIdx.x == 0) u[(int64_t)A * k + c] = acc;
 }
 
-void launch_ao_charges(const void* M, int dtype, int k, int n_ao, const int* ao_off, int n_atoms,
+void launch_ao_charges(const void* M, int dtype, int k, int n_ao, int64_t ld, const int* ao_off, int n_atoms,
                        const double* gamma_lr, double* u, hipStream_t s) {
   dim3 grid(n_atoms, k);
   switch (dtype) {
     case PREC_FP64:
-      ao_charges_kernel<double><<<grid, 64, 0, s>>>((const double*)M, k, n_ao, ao_off, gamma_lr, n_atoms, u);
+      ao_charges_kernel<double><<<grid, 64, 0, s>>>((const double*)M, k, n_ao, ld, ao_off, gamma_lr, n_atoms, u);
       break;
     case PREC_FP32:
-      ao_charges_kernel<float><<<grid, 64, 0, s>>>((const float*)M, k, n_ao, ao_off, gamma_lr, n_atoms, u);
+      ao_charges_kernel<float><<<grid, 64, 0, s>>>((const float*)M, k, n_ao, ld, ao_off, gamma_lr, n_atoms, u);
       break;
     case PREC_BF16:
-      ao_charges_kernel<__bf16><<<grid, 64, 0, s>>>((const __bf16*)M, k, n_ao, ao_off, gamma_lr, n_atoms, u);
+      ao_charges_kernel<__bf16><<<grid, 64, 0, s>>>((const __bf16*)M, k, n_ao, ld, ao_off, gamma_lr, n_atoms, u);
       break;
     default:
-      ao_charges_kernel<_Float16><<<grid, 64, 0, s>>>((const _Float16*)M, k, n_ao, ao_off, gamma_lr, n_atoms, u);
+      ao_charges_kernel<_Float16><<<grid, 64, 0, s>>>((const _Float16*)M, k, n_ao, ld, ao_off, gamma_lr, n_atoms, u);
       break;
   }
   TDA_LAUNCH_CHECK();
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) ao_add_coulomb_kernel(T* __restrict__ M, int k, int n_ao,
+__global__ void __launch_bounds__(256) ao_add_coulomb_kernel(T* __restrict__ M, int k, int n_ao, int64_t ld,
                                                              const int* __restrict__ atom,
                                                              const double* __restrict__ w) {
   const int64_t n2 = 2 * (int64_t)n_ao;
@@ -572,21 +577,22 @@ __global__ void __launch_bounds__(256) ao_add_coulomb_kernel(T* __restrict__ M, 
     const int c = (int)(t / n_ao);
     const int64_t mu = t % n_ao;
     const double add = w[(int64_t)atom[mu] * k + c];
-    T* Mc = M + (int64_t)c * n2 * n2;
-    T* p1 = Mc + mu * n2 + n_ao + mu;
-    T* p2 = Mc + (n_ao + mu) * n2 + mu;
+    T* Mc = M + (int64_t)c * n2 * ld;
+    T* p1 = Mc + mu * ld + n_ao + mu;
+    T* p2 = Mc + (n_ao + mu) * ld + mu;
     *p1 = to_type<T>(to_double<T>(*p1) + add);
     *p2 = to_type<T>(to_double<T>(*p2) + add);
   }
 }
 
-void launch_ao_add_coulomb(void* M, int dtype, int k, int n_ao, const int* atom, const double* w, hipStream_t s) {
+void launch_ao_add_coulomb(void* M, int dtype, int k, int n_ao, int64_t ld, const int* atom, const double* w,
+                           hipStream_t s) {
   const int g = grid_for((int64_t)k * n_ao);
   switch (dtype) {
-    case PREC_FP64: ao_add_coulomb_kernel<double><<<g, 256, 0, s>>>((double*)M, k, n_ao, atom, w); break;
-    case PREC_FP32: ao_add_coulomb_kernel<float><<<g, 256, 0, s>>>((float*)M, k, n_ao, atom, w); break;
-    case PREC_BF16: ao_add_coulomb_kernel<__bf16><<<g, 256, 0, s>>>((__bf16*)M, k, n_ao, atom, w); break;
-    default: ao_add_coulomb_kernel<_Float16><<<g, 256, 0, s>>>((_Float16*)M, k, n_ao, atom, w); break;
+    case PREC_FP64: ao_add_coulomb_kernel<double><<<g, 256, 0, s>>>((double*)M, k, n_ao, ld, atom, w); break;
+    case PREC_FP32: ao_add_coulomb_kernel<float><<<g, 256, 0, s>>>((float*)M, k, n_ao, ld, atom, w); break;
+    case PREC_BF16: ao_add_coulomb_kernel<__bf16><<<g, 256, 0, s>>>((__bf16*)M, k, n_ao, ld, atom, w); break;
+    default: ao_add_coulomb_kernel<_Float16><<<g, 256, 0, s>>>((_Float16*)M, k, n_ao, ld, atom, w); break;
   }
   TDA_LAUNCH_CHECK();
 }

@@ -237,6 +237,12 @@ void build_charges(tda_ctx* c, const double* dC, const double* dSC, const int* d
   if (qvv) launch_qblock(dC, dSC, nao, dao, n_atoms, vb, v, vp, vb, v, vp, qvv, c->stream);
 }
 
+// Row stride of the AO arrays indexed by (s, mu) along their contiguous
+// dimension (M, F, R'^T, L'^T): 2 n_ao rounded up to 64 elements, so that the
+// rows start 128-byte aligned and the GEMM takes the 16-byte load path for
+// every element type.
+int64_t ao_ld(const tda_ctx* c) { return round_up(2 * (int64_t)c->nao, 64); }
+
 void gemm_sc(tda_ctx* c, const double* dS, const double* dC, double* dSC, int nao) {
   // SC = S C  (transition_charges.rs:35)
   GemmArgs g = gargs(dS, dC, dSC, nao, nao, nao, plain(nao), plain(1), plain(nao), plain(1), plain(nao), plain(1));
@@ -257,10 +263,10 @@ void prepare_lowprec(tda_ctx* c, int prec) {
   const int N = c->N;
   const size_t esz = prec_bytes(prec);
   if (c->has_lc && c->op_kind == 0 && !c->ct) {
-    const int64_t n2 = 2 * (int64_t)c->nao;
+    const int64_t n2 = 2 * (int64_t)c->nao, ld = ao_ld(c);
     struct Pair { DevBuf* src; DevBuf* dst; int64_t len; };
     Pair pairs[4] = {{&c->aoRt, &ls.Rt, n2 * c->vp}, {&c->aoLm, &ls.Lm, n2 * c->op},
-                     {&c->aoRpt, &ls.Rpt, n2 * c->vp}, {&c->aoLpt, &ls.Lpt, n2 * c->op}};
+                     {&c->aoRpt, &ls.Rpt, ld * c->vp}, {&c->aoLpt, &ls.Lpt, ld * c->op}};
     for (Pair& pr : pairs) {
       pr.dst->ensure((size_t)pr.len * esz);
       launch_convert(pr.src->as<double>(), pr.len, 1, nullptr, pr.dst->p, prec, c->stream);
@@ -367,19 +373,19 @@ void build_ct_qov(tda_ctx* c) {
 
 void build_ao_operands(tda_ctx* c) {
   const int nao = c->nao;
-  const int64_t n2 = 2 * (int64_t)nao;
+  const int64_t n2 = 2 * (int64_t)nao, ld = ao_ld(c);
   const double* C = c->C.as<double>();
   const double* SC = c->SC.as<double>();
   c->aoRt.ensure((size_t)n2 * c->vp * 8);
   c->aoLm.ensure((size_t)n2 * c->op * 8);
-  c->aoRpt.ensure((size_t)n2 * c->vp * 8);
-  c->aoLpt.ensure((size_t)n2 * c->op * 8);
+  c->aoRpt.ensure((size_t)ld * c->vp * 8);
+  c->aoLpt.ensure((size_t)ld * c->op * 8);
   // Rt rows (SC_v; C_v), Lm rows (SC_o; C_o): they build D = L X R^T
-  launch_ao_operand(C, SC, nao, c->vb, c->v, c->vp, 0, 0, c->aoRt.as<double>(), c->stream);
-  launch_ao_operand(C, SC, nao, c->ob, c->o, c->op, 0, 0, c->aoLm.as<double>(), c->stream);
+  launch_ao_operand(C, SC, nao, c->vb, c->v, c->vp, 0, 0, 0, c->aoRt.as<double>(), c->stream);
+  launch_ao_operand(C, SC, nao, c->ob, c->o, c->op, 0, 0, 0, c->aoLm.as<double>(), c->stream);
   // the back-contraction pairs each D block with the other operator: (C; SC), stored transposed
-  launch_ao_operand(C, SC, nao, c->vb, c->v, c->vp, 1, 1, c->aoRpt.as<double>(), c->stream);
-  launch_ao_operand(C, SC, nao, c->ob, c->o, c->op, 1, 1, c->aoLpt.as<double>(), c->stream);
+  launch_ao_operand(C, SC, nao, c->vb, c->v, c->vp, 1, 1, ld, c->aoRpt.as<double>(), c->stream);
+  launch_ao_operand(C, SC, nao, c->ob, c->o, c->op, 1, 1, ld, c->aoLpt.as<double>(), c->stream);
 }
 
 void prepare(tda_ctx* c) {
@@ -532,10 +538,11 @@ void apply_A(tda_ctx* c, const double* X, int k, double* AX, int prec) {
 //   G2  M_c  = -1/4 gamma~ o (L XR_c^T)          [2n_ao][2n_ao]  (Hadamard epilogue)
 //   G3  F_c  = R' M_c^T ...      [v_p][2n_ao]
 //   G4  AX_c = de o x_c + L'^T F_c^T             [o][v_p]
+// M, F, R'^T and L'^T store their (s, mu) rows with the padded stride ao_ld().
 // ---------------------------------------------------------------------------
 void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   const int o = c->o, v = c->v, op = c->op, vp = c->vp, nao = c->nao, N = c->N;
-  const int64_t ovp = c->ovp, n2 = 2 * (int64_t)nao;
+  const int64_t ovp = c->ovp, n2 = 2 * (int64_t)nao, ld = ao_ld(c);
   const bool lp = prec != PREC_FP64;
   const DType dt = lp ? dtype_of(prec) : DT_F64;
   const size_t esz = lp ? prec_bytes(prec) : 8;
@@ -551,8 +558,8 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   const void* Rpt = lp ? c->lps[prec].Rpt.p : c->aoRpt.p;
   const void* Lpt = lp ? c->lps[prec].Lpt.p : c->aoLpt.p;
   c->aoXR.ensure((size_t)k * n2 * op * esz);
-  c->aoM.ensure((size_t)k * n2 * n2 * esz);
-  c->aoF.ensure((size_t)k * vp * n2 * esz);
+  c->aoM.ensure((size_t)k * n2 * ld * esz);
+  c->aoF.ensure((size_t)k * vp * ld * esz);
   const int pclass = lp ? 1 : 0;
   hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
   if (c->timing)
@@ -566,11 +573,11 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   // G2: M[c] = -1/4 gamma~ o (Lm XR[c]^T)
   if (c->timing) TDA_HIP(hipEventRecord(e[0], c->stream));
   {
-    GemmArgs g = gargs(Lm, c->aoXR.p, c->aoM.p, n2, n2, o, plain(op), plain(1), plain(1), plain(op), plain(n2),
+    GemmArgs g = gargs(Lm, c->aoXR.p, c->aoM.p, n2, n2, o, plain(op), plain(1), plain(1), plain(op), plain(ld),
                        plain(1));
     g.batch = k;
     g.b_batch = n2 * op;
-    g.c_batch = n2 * n2;
+    g.c_batch = n2 * ld;
     g.alpha = -0.25;
     // gamma~_{(s,mu),(t,nu)} = gamma_lr[atom(mu)][atom(nu)], gathered per row / column
     g.hsrc = c->gamma_lr.as<double>();
@@ -583,26 +590,26 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   // Coulomb: u = q_ov x from the D diagonals, w = gamma u, added to M
   c->u.ensure((size_t)N * k * 8);
   c->uu.ensure((size_t)N * k * 8);
-  launch_ao_charges(c->aoM.p, lp ? prec : PREC_FP64, k, nao, c->aooff.as<int>(), N, c->gamma_lr.as<double>(),
+  launch_ao_charges(c->aoM.p, lp ? prec : PREC_FP64, k, nao, ld, c->aooff.as<int>(), N, c->gamma_lr.as<double>(),
                     c->u.as<double>(), c->stream);
   gamma_times(c, N, k);
-  launch_ao_add_coulomb(c->aoM.p, lp ? prec : PREC_FP64, k, nao, c->ao_atom.as<int>(), c->uu.as<double>(),
+  launch_ao_add_coulomb(c->aoM.p, lp ? prec : PREC_FP64, k, nao, ld, c->ao_atom.as<int>(), c->uu.as<double>(),
                         c->stream);
   // G3: F[c][a][(t,nu)] = sum_(s,mu) Rpt[a][(s,mu)] M[c][(t,nu)][(s,mu)]
   if (c->timing) TDA_HIP(hipEventRecord(e[2], c->stream));
   {
-    GemmArgs g = gargs(Rpt, c->aoM.p, c->aoF.p, vp, n2, n2, plain(n2), plain(1), plain(1), plain(n2), plain(n2),
+    GemmArgs g = gargs(Rpt, c->aoM.p, c->aoF.p, vp, n2, n2, plain(ld), plain(1), plain(1), plain(ld), plain(ld),
                        plain(1));
     g.batch = k;
-    g.b_batch = n2 * n2;
-    g.c_batch = vp * n2;
+    g.b_batch = n2 * ld;
+    g.c_batch = vp * ld;
     run_gemm(c, dt, dt, EPI_STORE, true, true, g, 0, e[3]);
   }
   // G4: AX[c][i][a] = de[i][a] X[c][i][a] + sum_(t,nu) Lpt[i][(t,nu)] F[c][a][(t,nu)]
   {
-    GemmArgs g = gargs(Lpt, c->aoF.p, AX, o, vp, n2, plain(n2), plain(1), plain(1), plain(n2), plain(vp), plain(1));
+    GemmArgs g = gargs(Lpt, c->aoF.p, AX, o, vp, n2, plain(ld), plain(1), plain(1), plain(ld), plain(vp), plain(1));
     g.batch = k;
-    g.b_batch = vp * n2;
+    g.b_batch = vp * ld;
     g.c_batch = ovp;
     g.diag = c->de.as<double>();
     g.dm = plain(vp);
@@ -633,8 +640,8 @@ void products(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   int chunk = 64;
   if (ao) {
     // per-vector AO workspace: XR + M + F
-    const double n2 = 2.0 * c->nao;
-    const double per = (n2 * c->op + n2 * n2 + c->vp * n2) * (prec == PREC_FP64 ? 8.0 : prec_bytes(prec));
+    const double n2 = 2.0 * c->nao, ld = (double)ao_ld(c);
+    const double per = (n2 * c->op + n2 * ld + c->vp * ld) * (prec == PREC_FP64 ? 8.0 : prec_bytes(prec));
     chunk = (int)std::max(1.0, std::min(64.0, std::min((double)c->budget, 8.0 * (1ull << 30)) / per));
   }
   for (int c0 = 0; c0 < k; c0 += chunk) {
