@@ -55,6 +55,26 @@ def measured_traffic(key):
     return None
 
 
+def class_prec(pclass, prec):
+    """Precision of a kernel-timing class (0 FP64, 1 FP32, 2 the 16-bit type)."""
+    return ["fp64", "fp32", prec if prec in ("fp16", "bf16") else "fp16"][pclass]
+
+
+def kernel_table(tot_ms, tot_fl, tot_n, operator, prec):
+    """Per exchange stage and precision: launches, ms, achieved TFLOP/s."""
+    rows = []
+    for si in range(tot_ms.shape[0]):
+        for pi in range(tot_ms.shape[1]):
+            if tot_n[si, pi] == 0:
+                continue
+            kp = class_prec(pi, prec)
+            tf = tot_fl[si, pi] / (tot_ms[si, pi] * 1e-3) / 1e12 if tot_ms[si, pi] > 0 else 0.0
+            rows.append({"kernel": KERNEL_NAMES[operator][si], "prec": kp, "launches": int(tot_n[si, pi]),
+                         "ms": round(float(tot_ms[si, pi]), 3), "tflops": round(tf, 2),
+                         "frac": round(tf / PEAK_TFLOPS[kp], 4)})
+    return rows
+
+
 def build_inputs(system, seed):
     from tincr_amd import systems as sy
     if system.startswith("polyene"):
@@ -230,13 +250,13 @@ def main_fmo(args):
     value = n_states / (elapsed / args.steps)
     dw = max(float(np.abs(a["ci_eigenvalues"] - c["ci_eigenvalues"]).max()) for a, c in zip(res, r64))
     df = max(float(np.abs(a["oscillator_strengths"] - c["oscillator_strengths"]).max()) for a, c in zip(res, r64))
-    tot_ms, tot_fl, tot_n = np.zeros((2, 2)), np.zeros((2, 2)), np.zeros((2, 2))
+    tot_ms, tot_fl, tot_n = np.zeros((2, 3)), np.zeros((2, 3)), np.zeros((2, 3))
     for r in res:
         tot_ms += np.array(r["stats"]["exch_ms"])
         tot_fl += np.array(r["stats"]["exch_flops"])
         tot_n += np.array(r["stats"]["exch_launches"])
     si, pi = np.unravel_index(np.argmax(tot_ms), tot_ms.shape)
-    kprec = "fp64" if pi == 0 else prec
+    kprec = class_prec(pi, prec)
     achieved = tot_fl[si, pi] / (tot_ms[si, pi] * 1e-3) / 1e12 if tot_ms[si, pi] > 0 else 0.0
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "states/s", "n_gpus": 1, "steps": args.steps,
@@ -323,15 +343,15 @@ def main():
     df = float(np.abs(f - f64).max())
 
     # roofline of the dominant kernel (largest total time in the timed steps)
-    tot_ms = np.zeros((2, 2))
-    tot_fl = np.zeros((2, 2))
-    tot_n = np.zeros((2, 2))
+    tot_ms = np.zeros((2, 3))
+    tot_fl = np.zeros((2, 3))
+    tot_n = np.zeros((2, 3))
     for st in stats:
         tot_ms += np.array(st["exch_ms"])
         tot_fl += np.array(st["exch_flops"])
         tot_n += np.array(st["exch_launches"])
     si, pi = np.unravel_index(np.argmax(tot_ms), tot_ms.shape)
-    kprec = "fp64" if pi == 0 else args.prec
+    kprec = class_prec(pi, args.prec)
     achieved = tot_fl[si, pi] / (tot_ms[si, pi] * 1e-3) / 1e12 if tot_ms[si, pi] > 0 else 0.0
     traffic_key = f"{args.system}/{kprec}/{args.operator}/stage{si}"
     roofline = {
@@ -361,11 +381,13 @@ def main():
                  "t_products_ms": round(st64["t_products_ms"], 2)},
         "speedup_vs_fp64": round(value / fp64_value, 3),
         "max_abs_domega_meV": dw_mev, "max_abs_df": df,
-        "mixed": {"n_iter": last["n_iter"], "n_iter_fp64": last["n_iter_fp64"],
-                  "n_products": last["n_products"], "n_products_fp64": last["n_products_fp64"],
+        "mixed": {"n_iter": last["n_iter"], "n_iter_fp32": last["n_iter_fp32"], "n_iter_fp64": last["n_iter_fp64"],
+                  "n_products": last["n_products"], "n_products_fp32": last["n_products_fp32"],
+                  "n_products_fp64": last["n_products_fp64"],
                   "max_residual": last["max_residual"], "t_qbuild_ms": last["t_qbuild_ms"],
                   "t_products_ms": last["t_products_ms"]},
         "roofline": roofline,
+        "kernels": kernel_table(tot_ms, tot_fl, tot_n, args.operator, args.prec),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(inp, n, st64["n_iter"], st64["n_products"])

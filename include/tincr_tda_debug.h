@@ -17,6 +17,18 @@ extern "C" {
 int tda_debug_gemm(int in_prec, int a_kc, int b_kc, int64_t M, int64_t N, int64_t K, const double* A,
                    const double* B, double* C, double alpha, double beta, int ksplit);
 
+/* The K-contiguous 16-bit GEMM path for batch entries z < batch:
+ *   C[z][m][n] = epi(alpha * sum_k A[m][k] B[z][n][k])
+ * A [M][lda], B [batch][N][ldb] (row strides >= K; elements past K are never
+ * read), inputs converted on the device to in_prec (TDA_PREC_BF16 / FP16).
+ * epi 0: C in in_prec; 1: C FP64 = alpha acc + diag[m][n] X[z][m][n];
+ * 2: C in in_prec = alpha acc * H[r(m)][r(n)], H [nh][nh], r(x) = hidx[x % hmod]
+ * (hidx NULL: r(x) = x).  C is returned as FP64 [batch][M][N]; *used = 1
+ * when the dedicated kernel ran (0: the generic kernel). */
+int tda_debug_gemm16(int in_prec, int epi, int64_t M, int64_t N, int64_t K, int batch, int64_t lda, int64_t ldb,
+                     const double* A, const double* B, double alpha, const double* H, int nh, const int* hidx,
+                     int hmod, const double* diag, const double* X, double* C, int* used);
+
 /* Lowest n eigenpairs of the symmetric m x m matrix H (upper triangle read):
  * evals [n] ascending, Y [n][m] (row r = eigenvector r). */
 int tda_debug_eigh(int m, int n, const double* H, double* evals, double* Y);

@@ -104,3 +104,95 @@ def test_eigh_degenerate_clusters():
     assert np.abs(w - w_ref).max() < 1e-12
     assert np.abs(y @ y.T - np.eye(8)).max() < 1e-10
     assert np.abs(h @ y.T - y.T * w[None, :]).max() < 1e-10
+
+
+# --- the K-contiguous 16-bit GEMM (AO exchange products) -------------------
+
+def _r16(x, prec):
+    return _round(np.asarray(x, dtype=float), prec)
+
+
+def _ref16(a, b, prec):
+    """sum_k a[m, k] b[z, n, k] of the rounded inputs, FP64."""
+    return np.einsum("mk,znk->zmn", _r16(a, prec), _r16(b, prec), optimize=True)
+
+
+def _inputs16(rng, prec, *shapes):
+    """Uniform(-1, 1) values already representable in the 16-bit type, so the
+    device conversion is exact and only accumulation / output rounding remain."""
+    return [_r16(rng.uniform(-1, 1, sh), prec) for sh in shapes]
+
+
+# FP32 accumulation of up to ~500 exact products, relative to sum |a_k b_k|
+ACC16 = 4e-5
+
+
+# (M, N, K, batch): edge tiles in M / N, K not a multiple of 8 or of 64, K
+# below one slice, and a batch large enough for the 256x256 tile (>= 512 tiles)
+SHAPES16 = [(16, 16, 64, 1), (37, 53, 70, 2), (130, 129, 5, 3), (200, 333, 501, 2), (257, 255, 130, 1),
+            (520, 520, 72, 60)]
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("shape", SHAPES16)
+def test_gemm16_store(prec, shape):
+    nat = _nat()
+    m, n, k, bt = shape
+    print(f"gemm16 store {prec} {shape}", flush=True)
+    rng = np.random.default_rng(m + 7 * n + 13 * k + bt)
+    a, b = _inputs16(rng, prec, (m, k), (bt, n, k))
+    out, used = nat.debug_gemm16(a, b, prec, "store", alpha=-0.25, pad=3 + (k % 5))
+    assert used
+    ref = -0.25 * _ref16(a, b, prec)
+    # output rounded to the 16-bit type (half an ulp: 2^-11 / 2^-8 of |value|) plus the accumulation
+    ulp = 2.0 ** (-11 if prec == "fp16" else -8)
+    scale = 0.25 * np.einsum("mk,znk->zmn", np.abs(a), np.abs(b), optimize=True)
+    assert np.all(np.abs(out - ref) <= ulp * (np.abs(ref) + ACC16 * scale) + ACC16 * scale + 1e-30)
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("shape", [(37, 53, 70, 2), (200, 333, 501, 2), (520, 520, 72, 60)])
+def test_gemm16_diag(prec, shape):
+    nat = _nat()
+    m, n, k, bt = shape
+    rng = np.random.default_rng(k)
+    a, b = _inputs16(rng, prec, (m, k), (bt, n, k))
+    dg = rng.uniform(0.1, 1.5, (m, n))
+    x = rng.uniform(-1, 1, (bt, m, n))
+    out, used = nat.debug_gemm16(a, b, prec, "diag", alpha=1.0, diag=dg, x=x, pad=8)
+    assert used
+    ref = _ref16(a, b, prec) + dg[None] * x
+    scale = np.einsum("mk,znk->zmn", np.abs(a), np.abs(b), optimize=True) + 1.0
+    assert np.all(np.abs(out - ref) <= ACC16 * scale)
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("case", ["ao", "distinct", "identity"])
+@pytest.mark.parametrize("shape", [(200, 333, 501, 2), (520, 520, 72, 60)])
+def test_gemm16_hadamard(prec, case, shape):
+    """H[r(m)][r(n)] with r = an AO->atom table applied modulo its length
+    (runs of equal keys, a wrap inside tiles: LDS-staged table), distinct
+    keys (table larger than the LDS: factors read from H) and no table."""
+    nat = _nat()
+    m, n, k, bt = shape
+    rng = np.random.default_rng(11 * m + n)
+    a, b = _inputs16(rng, prec, (m, k), (bt, n, k))
+    if case == "ao":
+        per_atom = rng.integers(1, 5, size=400)
+        hidx = np.repeat(np.arange(400), per_atom)[:150]
+        nh = int(hidx.max()) + 1
+    elif case == "distinct":
+        hidx = rng.permutation(max(m, n))
+        nh = len(hidx)
+    else:
+        hidx, nh = None, max(m, n)
+    h = rng.uniform(0.1, 0.5, (nh, nh))
+    out, used = nat.debug_gemm16(a, b, prec, "hadamard", alpha=-0.25, h=h, hidx=hidx, pad=5)
+    assert used
+    r = (lambda x: hidx[x % len(hidx)]) if hidx is not None else (lambda x: x)
+    hf = h[np.ix_(r(np.arange(m)), r(np.arange(n)))]
+    ref = -0.25 * _ref16(a, b, prec) * hf[None]
+    ulp = 2.0 ** (-11 if prec == "fp16" else -8)
+    # H is applied in FP32 (one more rounding of the factor)
+    scale = 0.25 * np.einsum("mk,znk->zmn", np.abs(a), np.abs(b), optimize=True) * hf[None]
+    assert np.all(np.abs(out - ref) <= ulp * (np.abs(ref) + ACC16 * scale) + ACC16 * scale + 1e-30)

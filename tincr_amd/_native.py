@@ -40,13 +40,15 @@ class TdaStats(ctypes.Structure):
         ("n_products_fp64", ctypes.c_int32),
         ("converged", ctypes.c_int32),
         ("n_atom_blocks", ctypes.c_int32),
+        ("n_iter_fp32", ctypes.c_int32),
+        ("n_products_fp32", ctypes.c_int32),
         ("max_residual", ctypes.c_double),
         ("t_total_ms", ctypes.c_double),
         ("t_qbuild_ms", ctypes.c_double),
         ("t_products_ms", ctypes.c_double),
-        ("exch_ms", (ctypes.c_double * 2) * 2),
-        ("exch_launches", (ctypes.c_int32 * 2) * 2),
-        ("exch_flops", (ctypes.c_double * 2) * 2),
+        ("exch_ms", (ctypes.c_double * 3) * 2),
+        ("exch_launches", (ctypes.c_int32 * 3) * 2),
+        ("exch_flops", (ctypes.c_double * 3) * 2),
     ]
 
     def as_dict(self):
@@ -54,7 +56,7 @@ class TdaStats(ctypes.Structure):
         for name, _ in self._fields_:
             v = getattr(self, name)
             if name.startswith("exch_"):
-                v = [[v[i][j] for j in range(2)] for i in range(2)]
+                v = [[v[i][j] for j in range(3)] for i in range(2)]
             out[name] = v
         return out
 
@@ -178,6 +180,9 @@ def load_debug():
     i, i64 = ctypes.c_int, ctypes.c_int64
     lib.tda_debug_gemm.argtypes = [i, i, i, i64, i64, i64, D, D, D, ctypes.c_double, ctypes.c_double, i]
     lib.tda_debug_gemm.restype = ctypes.c_int
+    I = ctypes.POINTER(ctypes.c_int)
+    lib.tda_debug_gemm16.argtypes = [i, i, i64, i64, i64, i, i64, i64, D, D, ctypes.c_double, D, i, I, i, D, D, D, I]
+    lib.tda_debug_gemm16.restype = ctypes.c_int
     lib.tda_debug_eigh.argtypes = [i, i, D, D, D]
     lib.tda_debug_eigh.restype = ctypes.c_int
     lib.tda_debug_eigh_timed.argtypes = [i, i, D, D, D, D]
@@ -200,6 +205,36 @@ def debug_gemm(a, b, precision="fp64", a_kc=True, b_kc=False, alpha=1.0, beta=0.
     if rc != TDA_OK:
         raise TdaError(rc, "tda_debug_gemm failed")
     return out
+
+
+def debug_gemm16(a, b, precision="fp16", epi="store", alpha=1.0, h=None, hidx=None, diag=None, x=None, pad=0):
+    """The K-contiguous 16-bit GEMM: a [M, K], b [batch, N, K] -> [batch, M, N]
+    (FP64 copy of the device result) and whether the dedicated kernel ran.
+    Rows are stored with at least `pad` NaN elements past K (never read), the
+    row stride rounded up to 8 elements (16 bytes)."""
+    lib = load_debug()
+    m, k = a.shape
+    bt, n, _ = b.shape
+    lda = ldb = (k + pad + 7) // 8 * 8
+    a_st = np.full((m, lda), np.nan)
+    a_st[:, :k] = a
+    b_st = np.full((bt, n, ldb), np.nan)
+    b_st[:, :, :k] = b
+    out = np.zeros((bt, m, n))
+    used = ctypes.c_int(-1)
+    e = {"store": 0, "diag": 1, "hadamard": 2}[epi]
+    hh = f64(h) if h is not None else None
+    hi = np.ascontiguousarray(hidx, dtype=np.int32) if hidx is not None else None
+    dg = f64(diag) if diag is not None else None
+    xx = f64(x) if x is not None else None
+    rc = lib.tda_debug_gemm16(PREC[precision], e, m, n, k, bt, lda, ldb, dptr(f64(a_st)), dptr(f64(b_st)), alpha,
+                              dptr(hh) if hh is not None else None, hh.shape[0] if hh is not None else 0,
+                              hi.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) if hi is not None else None,
+                              len(hi) if hi is not None else 0, dptr(dg) if dg is not None else None,
+                              dptr(xx) if xx is not None else None, dptr(out), ctypes.byref(used))
+    if rc != TDA_OK:
+        raise TdaError(rc, "tda_debug_gemm16 failed")
+    return out, used.value == 1
 
 
 def debug_eigh(h, n, timed=False):

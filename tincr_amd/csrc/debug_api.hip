@@ -5,6 +5,7 @@
 #include "../../include/tincr_tda.h"
 #include "../../include/tincr_tda_debug.h"
 #include "gemm.h"
+#include "gemm16_impl.h"
 #include "kernels.h"
 
 using namespace tda;
@@ -59,6 +60,76 @@ extern "C" int tda_debug_gemm(int in_prec, int a_kc, int b_kc, int64_t M, int64_
     gemm_launch(dt, DT_F64, EPI_STORE, a_kc, b_kc, g, plan, 0);
     TDA_HIP(hipDeviceSynchronize());
     TDA_HIP(hipMemcpy(C, c.p, M * N * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+extern "C" int tda_debug_gemm16(int in_prec, int epi, int64_t M, int64_t N, int64_t K, int batch, int64_t lda,
+                                int64_t ldb, const double* A, const double* B, double alpha, const double* H, int nh,
+                                const int* hidx, int hmod, const double* diag, const double* X, double* C,
+                                int* used) {
+  return guard([&] {
+    TDA_CHECK(in_prec == 2 || in_prec == 3, TDA_ERR_BAD_ARG, "16-bit precision expected");
+    TDA_CHECK(epi >= 0 && epi <= 2 && M > 0 && N > 0 && K > 0 && batch > 0 && lda >= K && ldb >= K,
+              TDA_ERR_BAD_ARG, "bad gemm16 arguments");
+    const DType dt = in_prec == 2 ? DT_BF16 : DT_F16;
+    const size_t na = (size_t)M * lda, nb = (size_t)batch * N * ldb, nc = (size_t)batch * M * N;
+    Buf a64(na * 8), b64(nb * 8), a(na * 2), b(nb * 2), c(nc * 8);
+    TDA_HIP(hipMemcpy(a64.p, A, na * 8, hipMemcpyHostToDevice));
+    TDA_HIP(hipMemcpy(b64.p, B, nb * 8, hipMemcpyHostToDevice));
+    launch_convert((const double*)a64.p, na, 1, nullptr, a.p, in_prec, 0);
+    launch_convert((const double*)b64.p, nb, 1, nullptr, b.p, in_prec, 0);
+    Buf h(epi == EPI_HADAMARD ? (size_t)nh * nh * 8 : 8), hi(epi == EPI_HADAMARD && hidx ? (size_t)hmod * 4 : 8);
+    Buf dg(epi == EPI_DIAG ? (size_t)M * N * 8 : 8), xs(epi == EPI_DIAG ? nc * 8 : 8);
+    GemmArgs g;
+    std::memset(&g, 0, sizeof(g));
+    g.A = a.p; g.B = b.p; g.C = c.p;
+    g.M = M; g.N = N; g.K = K;
+    g.am = plain(lda); g.ak = plain(1); g.bk = plain(1); g.bn = plain(ldb);
+    g.cm = plain(N); g.cn = plain(1);
+    g.batch = batch; g.a_batch = 0; g.b_batch = N * ldb; g.c_batch = M * N;
+    g.alpha = alpha;
+    g.dm = plain(0); g.dmap = plain(1); g.xm = plain(0); g.xn = plain(0);
+    g.hm = plain(0); g.hn = plain(0);
+    DType out = dt;
+    if (epi == EPI_HADAMARD) {
+      TDA_HIP(hipMemcpy(h.p, H, (size_t)nh * nh * 8, hipMemcpyHostToDevice));
+      if (hidx) TDA_HIP(hipMemcpy(hi.p, hidx, (size_t)hmod * 4, hipMemcpyHostToDevice));
+      g.hsrc = (const double*)h.p;
+      g.hidx = hidx ? (const int*)hi.p : nullptr;
+      g.hmod = hmod;
+      g.hm = plain(nh); g.hn = plain(1);
+    } else if (epi == EPI_DIAG) {
+      TDA_HIP(hipMemcpy(dg.p, diag, (size_t)M * N * 8, hipMemcpyHostToDevice));
+      TDA_HIP(hipMemcpy(xs.p, X, nc * 8, hipMemcpyHostToDevice));
+      g.diag = (const double*)dg.p; g.dm = plain(N); g.dmap = plain(1);
+      g.xsrc = (const double*)xs.p; g.xm = plain(N); g.xn = plain(1); g.x_batch = M * N;
+      out = DT_F64;
+    }
+    const Epi e = (Epi)epi;
+    *used = gemm16_ok(dt, out, e, true, true, g, 1) ? 1 : 0;
+    GemmPlan plan = gemm_plan(dt, M, N, K, batch, true, true, 0);
+    Buf ws(gemm_workspace_bytes(dt, plan, M, N, batch));
+    g.ws = ws.p;
+    gemm_launch(dt, out, e, true, true, g, plan, 0);
+    TDA_HIP(hipDeviceSynchronize());
+    if (out == DT_F64) {
+      TDA_HIP(hipMemcpy(C, c.p, nc * 8, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<uint16_t> raw(nc);
+      TDA_HIP(hipMemcpy(raw.data(), c.p, nc * 2, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < nc; ++i) {
+        if (dt == DT_F16) {
+          _Float16 v;
+          std::memcpy(&v, &raw[i], 2);
+          C[i] = (double)(float)v;
+        } else {
+          const uint32_t bits = (uint32_t)raw[i] << 16;
+          float v;
+          std::memcpy(&v, &bits, 4);
+          C[i] = (double)v;
+        }
+      }
+    }
   });
 }
 

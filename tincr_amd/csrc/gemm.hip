@@ -1,5 +1,5 @@
 // MFMA GEMM: planning, dispatch and the split-K reduction.
-#include "gemm_impl.h"
+#include "gemm16_impl.h"
 
 #ifndef TDA_BIG_K
 #define TDA_BIG_K 0
@@ -65,6 +65,15 @@ void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g,
   TDA_CHECK(g.M >= 0 && g.N >= 0 && g.K >= 0, 2, "gemm: negative extent");
   if (g.M == 0 || g.N == 0) return;
   if (g.batch < 1) g.batch = 1;
+  // 16-bit inputs with both operands K-contiguous: the dedicated kernel (no split-K)
+  if (gemm16_ok(in, out, epi, a_kc, b_kc, g, 1)) {
+    g.ksplit = 1;
+    g.k_chunk = g.K;
+    if (in == DT_F16) gemm16_dispatch_f16(epi, g, stream);
+    else gemm16_dispatch_bf16(epi, g, stream);
+    if (after_main) TDA_HIP(hipEventRecord(after_main, stream));
+    return;
+  }
   GemmPlan pl = plan;
   // 16-bit inputs on the big tile: 128-deep K slices when the K maps allow
   // (TDA_BIG_K; off by default: it spills registers at two workgroups per CU)

@@ -1,0 +1,410 @@
+// 16-bit-input GEMM for operands stored with K contiguous ("NT"), gfx950.
+//
+//   C[z][cm(m) + cn(n)] = epilogue( alpha * sum_k A[z][m*lda + k] * B[z][n*ldb + k] )
+//
+// This is the shape of every product of the AO-factored exchange (L XR^T,
+// R' M^T, R X^T, L'^T F^T): both operands row-major along K.  Compared with
+// the generic index-map kernel (gemm_impl.h) it
+//   * stages K slices of 64 in two LDS buffers (one barrier per slice; the
+//     next slice is fetched into registers while the current one feeds the
+//     matrix cores),
+//   * stores each operand row in LDS as eight 16-byte chunks permuted by
+//     chunk ^ (row & 7), so the ds_read_b128 fragment reads of 16 rows hit 16
+//     different bank groups,
+//   * uses 256x256 workgroup tiles (8 waves of 128x64) for large products and
+//     128x128 tiles (4 waves of 64x64) for small ones,
+//   * computes the transposed 16x16 block (B fragment as the MFMA's first
+//     operand), so a lane ends with 4 consecutive columns of one row and
+//     stores them as one 8-byte (16-bit types) or two 16-byte (FP64) vectors.
+// MFMA: v_mfma_f32_16x16x32_{f16,bf16}, FP32 accumulation.  Lane l supplies
+// rows l%16, K elements 8*(l/16)..+7 of both operands; the result block
+// holds D[4*(l/16)+r][l%16], r < 4.
+#pragma once
+#include <type_traits>
+
+#include "gemm_impl.h"
+
+namespace tda {
+
+// PF: K slices fetched ahead into registers (1: the next slice while the
+// current one is multiplied; 2: two slices ahead, two register sets).
+template <int BM_, int BN_, int WGM_, int WGN_, int MINB_, int PF_ = 1>
+struct Cfg16 {
+  static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_, NT = WGM_ * WGN_ * 64, MINB = MINB_, PF = PF_;
+};
+using Cfg16Big = Cfg16<256, 256, 2, 4, 1, 1>;  // 8 waves, 128x64 each; 128 KiB LDS
+using Cfg16Mid = Cfg16<128, 128, 2, 2, 2, 1>;  // 4 waves, 64x64 each; 64 KiB LDS
+
+template <typename TI>
+struct Mfma16;
+template <>
+struct Mfma16<_Float16> {
+  using Frag = v8h;
+  static __device__ inline v4f op(Frag a, Frag b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma16<__bf16> {
+  using Frag = v8bf;
+  static __device__ inline v4f op(Frag a, Frag b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T>
+struct alignas(8) Vec4 {
+  T v[4];
+};
+
+template <typename TI, typename TO, typename Cfg, int EPI>
+__global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs g) {
+  using MF = Mfma16<TI>;
+  using Frag = typename MF::Frag;
+  constexpr int BM = Cfg::BM, BN = Cfg::BN, NT = Cfg::NT;
+  constexpr int WTM = BM / Cfg::WGM, WTN = BN / Cfg::WGN;
+  constexpr int MI = WTM / 16, NI = WTN / 16;
+  constexpr int BK = 64;                  // K elements per slice = 8 chunks of 16 bytes
+  constexpr int A_CH = BM * 8, B_CH = BN * 8;  // 16-byte chunks per slice
+  constexpr int A_PER = A_CH / NT, B_PER = B_CH / NT;
+  static_assert(A_CH % NT == 0 && B_CH % NT == 0, "tile / thread count mismatch");
+  constexpr int STAGE = (BM + BN) * BK;   // elements per LDS buffer
+  __shared__ __align__(16) TI lds[2 * STAGE];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm0 = (wave / Cfg::WGN) * WTM;
+  const int wn0 = (wave % Cfg::WGN) * WTN;
+  const int zb = blockIdx.z;
+  const int64_t m0 = (int64_t)blockIdx.y * BM;
+  const int64_t n0 = (int64_t)blockIdx.x * BN;
+  const int64_t lda = g.am.s0, ldb = g.bn.s0;
+  const TI* A = (const TI*)g.A + zb * g.a_batch;
+  const TI* B = (const TI*)g.B + zb * g.b_batch;
+  const int64_t K = g.K;
+  const int nkt = (int)((K + BK - 1) / BK);
+
+  // Loader: chunk c = tid + i*NT of a slice is row c/8 (= tid/8 + i*NT/8),
+  // K chunk c%8 (= tid%8).  Rows past M / N read row M-1 / N-1 (results
+  // discarded at the store).  32-bit element offsets from the batch base.
+  const int lr = tid >> 3, lkc = tid & 7;
+  uint32_t a_off[A_PER], b_off[B_PER];
+#pragma unroll
+  for (int i = 0; i < A_PER; ++i)
+    a_off[i] = (uint32_t)(min(m0 + lr + i * (NT / 8), g.M - 1) * lda) + lkc * 8;
+#pragma unroll
+  for (int i = 0; i < B_PER; ++i)
+    b_off[i] = (uint32_t)(min(n0 + lr + i * (NT / 8), g.N - 1) * ldb) + lkc * 8;
+  VecN<TI, 8> ra[Cfg::PF][A_PER], rb[Cfg::PF][B_PER];
+
+  auto load = [&](int kt, auto set_c) {
+    constexpr int P = decltype(set_c)::value;
+    const uint32_t k0 = (uint32_t)kt * BK;
+    if ((int64_t)k0 + BK <= K) {
+#pragma unroll
+      for (int i = 0; i < A_PER; ++i) ra[P][i] = *(const VecN<TI, 8>*)(A + (a_off[i] + k0));
+#pragma unroll
+      for (int i = 0; i < B_PER; ++i) rb[P][i] = *(const VecN<TI, 8>*)(B + (b_off[i] + k0));
+    } else {
+      // last slice: elements past K read as zero
+      const int64_t kk = (int64_t)k0 + lkc * 8;
+#pragma unroll
+      for (int i = 0; i < A_PER; ++i) {
+        if (kk + 8 <= K) {
+          ra[P][i] = *(const VecN<TI, 8>*)(A + (a_off[i] + k0));
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) ra[P][i].v[e] = (kk + e < K) ? A[a_off[i] + k0 + e] : (TI)0;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < B_PER; ++i) {
+        if (kk + 8 <= K) {
+          rb[P][i] = *(const VecN<TI, 8>*)(B + (b_off[i] + k0));
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) rb[P][i].v[e] = (kk + e < K) ? B[b_off[i] + k0 + e] : (TI)0;
+        }
+      }
+    }
+  };
+  // LDS position of this thread's chunks: row lr + i*NT/8, chunk lkc ^ (row & 7)
+  // (NT/8 is a multiple of 8, so row & 7 == lr & 7)
+  const int lds_a0 = lr * BK + ((lkc ^ (lr & 7)) << 3);
+  auto store = [&](int buf, auto set_c) {
+    constexpr int P = decltype(set_c)::value;
+    TI* s = lds + buf * STAGE;
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) *(VecN<TI, 8>*)(s + lds_a0 + i * (NT / 8) * BK) = ra[P][i];
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) *(VecN<TI, 8>*)(s + BM * BK + lds_a0 + i * (NT / 8) * BK) = rb[P][i];
+  };
+
+  v4f acc[MI][NI];
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < NI; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  // fragment row (within the wave tile) and its swizzle key
+  const int fr = lane & 15, fq = lane >> 4;
+  auto compute = [&](int buf) {
+    const TI* s = lds + buf * STAGE;
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const int kc = ks * 4 + fq;  // K chunk of this lane's 8 elements
+      Frag fb[NI];
+#pragma unroll
+      for (int j = 0; j < NI; ++j) {
+        const int r = wn0 + j * 16 + fr;
+        fb[j] = *(const Frag*)(s + BM * BK + r * BK + ((kc ^ (r & 7)) << 3));
+      }
+#pragma unroll
+      for (int i = 0; i < MI; ++i) {
+        const int r = wm0 + i * 16 + fr;
+        const Frag fa = *(const Frag*)(s + r * BK + ((kc ^ (r & 7)) << 3));
+#pragma unroll
+        for (int j = 0; j < NI; ++j) acc[i][j] = MF::op(fb[j], fa, acc[i][j]);
+      }
+    }
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  if constexpr (Cfg::PF == 1) {
+    if (nkt > 0) {
+      load(0, I0{});
+      store(0, I0{});
+      __syncthreads();
+    }
+    for (int kt = 0; kt < nkt; ++kt) {
+      if (kt + 1 < nkt) load(kt + 1, I0{});
+      compute(kt & 1);
+      if (kt + 1 < nkt) store((kt + 1) & 1, I0{});
+      __syncthreads();
+    }
+  } else {
+    // slice kt+2 is fetched into register set kt&1 (free: slice kt went to
+    // the LDS in step kt-1) before slice kt is multiplied; slice kt+1 (set
+    // 1-(kt&1), fetched in step kt-1) goes to the LDS after it.
+    if (nkt > 0) load(0, I0{});
+    if (nkt > 1) load(1, I1{});
+    if (nkt > 0) {
+      store(0, I0{});
+      __syncthreads();
+    }
+    auto step = [&](int kt, auto p_c) {
+      constexpr int P = decltype(p_c)::value;
+      using IP = std::integral_constant<int, P>;
+      using IQ = std::integral_constant<int, 1 - P>;
+      if (kt + 2 < nkt) load(kt + 2, IP{});
+      compute(P);
+      if (kt + 1 < nkt) store(1 - P, IQ{});
+      __syncthreads();
+    };
+    for (int kt = 0; kt < nkt; kt += 2) {
+      step(kt, I0{});
+      if (kt + 1 < nkt) step(kt + 1, I1{});
+    }
+  }
+
+  // ---- epilogue: lane holds C[m][n..n+3], m = row l%16 of block i, n = 4*(l/16) of block j ----
+  TO* C = (TO*)g.C + zb * g.c_batch;
+  const float alpha = (float)(g.batch_scale ? g.alpha * (double)g.batch_scale[zb] : g.alpha);
+  // column offsets per block j (4 consecutive columns)
+  int64_t cn_off[NI];
+  bool cn_vec[NI];
+  int64_t n_first[NI];
+#pragma unroll
+  for (int j = 0; j < NI; ++j) {
+    const int64_t n = n0 + wn0 + j * 16 + 4 * fq;
+    n_first[j] = n;
+    const bool all_in = n + 4 <= g.N;
+    cn_off[j] = n < g.N ? g.cn(n) : 0;
+    // the 4 columns map to 4 consecutive, vector-aligned elements of C
+    bool contig = all_in && g.cn.s0 == 1 && (g.cn.div >= (INT64_C(1) << 62) || (n % g.cn.div) + 4 <= g.cn.div);
+    cn_vec[j] = contig && (cn_off[j] % 4 == 0);
+  }
+  // EPI_HADAMARD: the factors H[hm(r(m)) + hn(r(n))] of the tile.  Runs of
+  // equal keys r(.) along the tile's rows / columns share a slot; when the
+  // slot table (rows x columns, FP32) fits the LDS it is gathered once per
+  // tile, else every factor is read from H.  hrow / hcol: per-lane table
+  // offsets (or H offsets when not staged).
+  int hrow[MI], hcol[NI][4];
+  bool staged = false;
+  float* ht = nullptr;
+  if constexpr (EPI == EPI_HADAMARD) {
+    constexpr int NW = NT / 64;
+    int* rslot = (int*)lds;   // [BM] slot of each tile row
+    int* cslot = rslot + BM;  // [BN]
+    int* rkey = cslot + BN;   // [BM] key of each row slot
+    int* ckey = rkey + BM;    // [BN]
+    int* tot = ckey + BN;     // [NW] slots opened per wave
+    ht = (float*)(tot + 16);
+    constexpr int CAP = (int)((sizeof(TI) * 2 * STAGE) / 4) - 2 * (BM + BN) - 16;
+    auto key_of = [&](int64_t x) -> int {
+      return g.hidx ? g.hidx[(uint32_t)x % (uint32_t)g.hmod] : (int)x;
+    };
+    // thread t < BM: row t; BM <= t < BM + BN: column t - BM (BM is a multiple of 64)
+    const bool is_row = tid < BM, is_col = !is_row && tid < BM + BN;
+    int key = 0;
+    bool open = false;
+    if (is_row || is_col) {
+      const int t = is_row ? tid : tid - BM;
+      const int64_t base = is_row ? m0 : n0, last = is_row ? g.M - 1 : g.N - 1;
+      key = key_of(min(base + t, last));
+      open = t == 0 || key != key_of(min(base + t - 1, last));
+    }
+    const uint64_t bal = __ballot(open);
+    const int below = __popcll(bal & ((UINT64_C(1) << lane) - 1));
+    if (lane == 0) tot[wave] = __popcll(bal);
+    __syncthreads();
+    int R = 0, Cn = 0, off = 0;
+    for (int w = 0; w < NW; ++w) {
+      if (w < BM / 64) R += tot[w];
+      else if (w < (BM + BN) / 64) Cn += tot[w];
+      if (w < wave && (is_row ? true : w >= BM / 64)) off += tot[w];
+    }
+    const int slot = off + below + (open ? 1 : 0) - 1;
+    if (is_row) {
+      rslot[tid] = slot;
+      if (open) rkey[slot] = key;
+    } else if (is_col) {
+      cslot[tid - BM] = slot;
+      if (open) ckey[slot] = key;
+    }
+    __syncthreads();
+    staged = R * Cn <= CAP;
+    if (staged) {
+      for (int e = tid; e < R * Cn; e += NT) {
+        const int r = (uint32_t)e / (uint32_t)Cn, c = e - r * Cn;
+        ht[e] = (float)g.hsrc[g.hm(rkey[r]) + g.hn(ckey[c])];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+      const int rs = rslot[wm0 + i * 16 + fr];
+      hrow[i] = staged ? rs * Cn : (int)g.hm(rkey[rs]);
+    }
+#pragma unroll
+    for (int j = 0; j < NI; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int cs = cslot[wn0 + j * 16 + 4 * fq + r];
+        hcol[j][r] = staged ? cs : (int)g.hn(ckey[cs]);
+      }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < MI; ++i) {
+    const int64_t m = m0 + wm0 + i * 16 + fr;
+    if (m >= g.M) continue;
+    const int64_t cmo = g.cm(m);
+    const bool c_al = (cmo % 4 == 0) && ((g.c_batch * zb) % 4 == 0);
+    if constexpr (EPI == EPI_HADAMARD) {
+#pragma unroll
+      for (int j = 0; j < NI; ++j) {
+        if (n_first[j] >= g.N) continue;
+        float v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int h = hrow[i] + hcol[j][r];
+          v[r] = alpha * acc[i][j][r] * (staged ? ht[h] : (float)g.hsrc[h]);
+        }
+        if (cn_vec[j] && c_al) {
+          Vec4<TO> w;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) w.v[r] = (TO)v[r];
+          *(Vec4<TO>*)(C + cmo + cn_off[j]) = w;
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (n_first[j] + r < g.N) C[cmo + g.cn(n_first[j] + r)] = (TO)v[r];
+        }
+      }
+    } else if constexpr (EPI == EPI_DIAG) {
+      // FP64 output: C = alpha acc + diag[dm(m) + dmap(n)] * X[z][xm(m) + xn(n)]
+      const double* xs = g.xsrc + zb * g.x_batch;
+      const int64_t dmo = g.dm(m), xmo = g.xm(m);
+#pragma unroll
+      for (int j = 0; j < NI; ++j) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t n = n_first[j] + r;
+          if (n >= g.N) continue;
+          double v = (double)alpha * (double)acc[i][j][r];
+          v += g.diag[dmo + g.dmap(n)] * xs[xmo + g.xn(n)];
+          C[cmo + g.cn(n)] = to_type<TO>(v);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NI; ++j) {
+        if (n_first[j] >= g.N) continue;
+        if (cn_vec[j] && c_al) {
+          Vec4<TO> w;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) w.v[r] = (TO)(alpha * acc[i][j][r]);
+          *(Vec4<TO>*)(C + cmo + cn_off[j]) = w;
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (n_first[j] + r < g.N) C[cmo + g.cn(n_first[j] + r)] = (TO)(alpha * acc[i][j][r]);
+        }
+      }
+    }
+  }
+}
+
+// Whether the fast kernel applies: 16-bit inputs, both operands K-contiguous
+// with plain row maps, 16-byte aligned rows, no split-K, and an epilogue it
+// implements (no beta / batch scale).
+inline bool gemm16_ok(DType in, DType out, Epi epi, bool a_kc, bool b_kc, const GemmArgs& g, int ksplit) {
+  constexpr int64_t PLAIN = INT64_C(1) << 62;
+  if (!(in == DT_F16 || in == DT_BF16) || !a_kc || !b_kc || ksplit > 1) return false;
+  if (g.am.div < PLAIN || g.bn.div < PLAIN || g.ak.div < PLAIN || g.bk.div < PLAIN) return false;
+  if (g.ak.s0 != 1 || g.bk.s0 != 1 || g.am.s0 % 8 || g.bn.s0 % 8 || g.a_batch % 8 || g.b_batch % 8) return false;
+  if ((((uintptr_t)g.A) & 15) || (((uintptr_t)g.B) & 15)) return false;
+  if (g.beta != 0.0) return false;
+  // 32-bit element offsets within one batch entry of each operand
+  if (g.M < 1 || g.N < 1 || (g.M - 1) * g.am.s0 + g.K >= (INT64_C(1) << 32) ||
+      (g.N - 1) * g.bn.s0 + g.K >= (INT64_C(1) << 32))
+    return false;
+  if (epi == EPI_DIAG) return out == DT_F64;
+  // EPI_HADAMARD: plain H maps with 32-bit offsets
+  if (epi == EPI_HADAMARD &&
+      (g.hm.div < PLAIN || g.hn.div < PLAIN || g.hn.s0 != 1 || g.hm.s0 < 0 || g.hm.s0 > 46340 ||
+       (g.hidx && (g.hmod < 1 || g.hmod > INT32_MAX))))
+    return false;
+  return (in == DT_F16 && out == DT_F16) || (in == DT_BF16 && out == DT_BF16);
+}
+
+// Tile choice: the big tile when it gives at least ~2 workgroups per CU
+// (256 CUs), else the 128x128 tile.
+inline bool gemm16_big(const GemmArgs& g) {
+  const int64_t tiles = ceil_div(g.M, 256) * ceil_div(g.N, 256) * (int64_t)g.batch;
+  return tiles >= 512;
+}
+
+template <typename TI, typename TO, int EPI, typename Cfg>
+void launch16_cfg(GemmArgs& g, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(g.N, Cfg::BN), (unsigned)ceil_div(g.M, Cfg::BM), (unsigned)g.batch);
+  gemm16_nt_kernel<TI, TO, Cfg, EPI><<<grid, Cfg::NT, 0, s>>>(g);
+  TDA_LAUNCH_CHECK();
+}
+
+template <typename TI, typename TO, int EPI>
+void launch16_t(GemmArgs& g, hipStream_t s) {
+  if (gemm16_big(g)) launch16_cfg<TI, TO, EPI, Cfg16Big>(g, s);
+  else launch16_cfg<TI, TO, EPI, Cfg16Mid>(g, s);
+}
+
+// Epilogue / output dispatch for one input type (gemm16_ok has been checked).
+template <typename TI>
+void launch16_in(Epi epi, GemmArgs& g, hipStream_t s) {
+  if (epi == EPI_HADAMARD) launch16_t<TI, TI, EPI_HADAMARD>(g, s);
+  else if (epi == EPI_DIAG) launch16_t<TI, double, EPI_DIAG>(g, s);
+  else launch16_t<TI, TI, EPI_STORE>(g, s);
+}
+
+}  // namespace tda

@@ -1,5 +1,5 @@
 // BF16-input instantiations of the MFMA GEMM.
-#include "gemm_impl.h"
+#include "gemm16_impl.h"
 
 namespace tda {
 
@@ -16,5 +16,7 @@ void gemm_dispatch_bf16(int epi, DType out, int cfg, bool a_kc, bool b_kc, GemmA
     throw Error{2, "gemm bf16: unsupported output"};
   }
 }
+
+void gemm16_dispatch_bf16(Epi epi, GemmArgs& g, hipStream_t s) { launch16_in<__bf16>(epi, g, s); }
 
 }  // namespace tda

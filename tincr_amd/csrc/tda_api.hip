@@ -99,14 +99,19 @@ struct tda_ctx {
   bool dense = false;
   DevBuf Hd, dxyz, post;
   DevBuf adjP, adjQ, adjX, adjD;  // adjoint work buffers
+  DevBuf one;                     // a device 1.0
   int last_roots = 0;             // CI vectors of the last tda_run held in RZ
   // thick restart of the FP16 / BF16 Davidson phase, in multiples of n_roots
   // (environment TINCR_TDA_LP_CAP / TINCR_TDA_LP_KEEP); the FP32 and FP64
   // phases keep the reference rule (cap 15 n, collapse to n)
-  int lp_cap_mult = 8, lp_keep_mult = 2;
+  int lp_cap_mult = 15, lp_keep_mult = 2;
   int lp_keep(int n) const { return std::min(64, std::max(n, lp_keep_mult * n)); }
   int lp_cap(int n) const { return std::min(15 * n, std::max(lp_cap_mult * n, lp_keep(n) + n)); }
   double pre_thr = 1e-4;
+  // precision ladder: floor factor of the low-precision phases, stall window
+  double floor_k = 2.0;
+  int stall_window = 10;
+  bool fp32_thick = true;
   int64_t ovp = 0;
   std::vector<double> xyz, h_de;
   DevBuf C, S, SC, orbe, aooff, gamma, gamma_lr;
@@ -133,11 +138,11 @@ struct tda_ctx {
   bool timing = true;
   bool verbose = false;
   double t_products_ms = 0;
-  int n_products = 0, n_products_fp64 = 0, n_blocks = 1;
+  int n_products = 0, n_products_fp64 = 0, n_products_fp32 = 0, n_blocks = 1;
   // per exchange stage s (0: stage 1, 1: stage 3), per precision class p
   // (0: FP64, 1: low precision): main-kernel time, launches, flops
-  double st_ms[2][2] = {{0, 0}, {0, 0}}, st_flops[2][2] = {{0, 0}, {0, 0}};
-  int st_n[2][2] = {{0, 0}, {0, 0}};
+  double st_ms[2][3] = {}, st_flops[2][3] = {};
+  int st_n[2][3] = {};
   struct Rec {
     hipEvent_t a, b;
     int stage, pclass;
@@ -155,7 +160,7 @@ struct tda_ctx {
                       &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &P, &W, &norms, &npart, &idx, &count,
                       &gs_scratch, &eigw, &ao_atom, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
                       &Cl, &Sl, &SCl, &Shl, &aooff_l, &ctW, &ctZ, &Hd, &dxyz, &post,
-                      &adjP, &adjQ, &adjX, &adjD})
+                      &adjP, &adjQ, &adjX, &adjD, &one})
       b->release();
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -185,9 +190,9 @@ struct tda_ctx {
   }
   void reset_counters() {
     t_products_ms = 0;
-    n_products = n_products_fp64 = 0;
+    n_products = n_products_fp64 = n_products_fp32 = 0;
     for (int i = 0; i < 2; ++i)
-      for (int j = 0; j < 2; ++j) st_ms[i][j] = st_flops[i][j] = 0, st_n[i][j] = 0;
+      for (int j = 0; j < 3; ++j) st_ms[i][j] = st_flops[i][j] = 0, st_n[i][j] = 0;
   }
 };
 
@@ -486,7 +491,7 @@ void apply_A(tda_ctx* c, const double* X, int k, double* AX, int prec) {
     // pads of G; they are never written by stage 1 and the buffer is shared by
     // all precisions, so clear them (0 * NaN-pattern would poison the sum).
     if (op > o) launch_zero_runs(c->T.p, (int64_t)k * vp * nb, op, o, op - o, (int)esz, c->stream);
-    const int pclass = lp ? 1 : 0;
+    const int pclass = prec == PREC_FP64 ? 0 : prec == PREC_FP32 ? 1 : 2;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
     if (c->timing) {
       e0 = c->get_event();
@@ -560,7 +565,7 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   c->aoXR.ensure((size_t)k * n2 * op * esz);
   c->aoM.ensure((size_t)k * n2 * ld * esz);
   c->aoF.ensure((size_t)k * vp * ld * esz);
-  const int pclass = lp ? 1 : 0;
+  const int pclass = prec == PREC_FP64 ? 0 : prec == PREC_FP32 ? 1 : 2;
   hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
   if (c->timing)
     for (auto& ev : e) ev = c->get_event();
@@ -651,6 +656,7 @@ void products(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   }
   c->n_products += k;
   if (prec == PREC_FP64) c->n_products_fp64 += k;
+  if (prec == PREC_FP32) c->n_products_fp32 += k;
 }
 
 // ---------------------------------------------------------------------------
@@ -753,6 +759,7 @@ struct DavResult {
   bool converged = false;
   int iters = 0;
   double max_err = 0;
+  double delta = 0;  // measured product error of the phase's precision (0: not measured)
 };
 
 // Runs the block Davidson with products in `prec` starting from the m0
@@ -771,10 +778,15 @@ struct DavResult {
 // are rotated onto the keep lowest Ritz vectors instead (V <- Y V,
 // AV <- Y AV), so no product is recomputed and the projection restarts as
 // diag(theta).
+// floor_k > 0 (low-precision phases): after the first iteration the phase
+// measures delta = |A_prec z - A_fp64 z| for its lowest Ritz vector z (unit
+// norm), the size of the product error every residual of the phase carries,
+// and leaves once max residual < floor_k * delta: below that the computed
+// residuals no longer say how far the roots are from convergence.
 DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec, bool first_can_converge,
                    int stop_stall, std::vector<double>& h_evals, std::vector<double>& h_err, int cap_in = 0,
                    int keep = 0, int* kk_out = nullptr, double stall_factor = 0.5,
-                   double stall_floor = 1e300, bool thick_restart = true) {
+                   double stall_floor = 1e300, bool thick_restart = true, double floor_k = 0.0) {
   const int64_t L = c->ovp;
   const int cap = cap_in > 0 ? cap_in : 15 * n;  // max_space (davidson.rs:74)
   const int ldh = 15 * n;                         // leading dimension of H and Y
@@ -882,6 +894,22 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
     if (roots_lft == 0 && (it > 0 || first_can_converge)) {  // davidson.rs:121
       res.converged = true;
       finish(true);
+      return res;
+    }
+    if (floor_k > 0 && prec != PREC_FP64 && !c->dense && it == 0) {
+      // product error of this precision on the lowest Ritz vector
+      products(c, RZ, 1, P, prec);
+      products(c, RZ, 1, W, PREC_FP64);
+      c->one.ensure(8);
+      launch_fill(c->one.as<double>(), 1.0, 1, c->stream);
+      launch_residual(W, P, c->one.as<double>(), 1, L, c->stream);  // W <- W - P
+      launch_norms(W, 1, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
+      TDA_HIP(hipMemcpyAsync(&res.delta, c->norms.p, 8, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipStreamSynchronize(c->stream));
+      if (c->verbose) fprintf(stderr, "tincr_tda davidson prec=%d product error delta=%.3e\n", prec, res.delta);
+    }
+    if (res.delta > 0 && max_err < floor_k * res.delta) {
+      finish(false);
       return res;
     }
     if (stop_stall > 0 && max_err < stall_floor) {
@@ -1033,6 +1061,12 @@ int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx**
     const char* ek = getenv("TINCR_TDA_LP_KEEP");
     if (ec && atoi(ec) >= 2) c->lp_cap_mult = atoi(ec);
     if (ek && atoi(ek) >= 1) c->lp_keep_mult = atoi(ek);
+    const char* ef = getenv("TINCR_TDA_FLOOR_K");
+    const char* es = getenv("TINCR_TDA_STALL");
+    if (ef && atof(ef) >= 0) c->floor_k = atof(ef);
+    if (es && atoi(es) >= 1) c->stall_window = atoi(es);
+    const char* et = getenv("TINCR_TDA_FP32_THICK");
+    if (et) c->fp32_thick = atoi(et) != 0;
   }
   if (hbm_budget_bytes) c->budget = hbm_budget_bytes;
   int rc = guarded(c, [&] {
@@ -1512,14 +1546,17 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     }
     TDA_HIP(hipStreamSynchronize(c->stream));
     std::vector<double> h_evals, h_err;
+    int lp16_iters = 0, fp32_iters = 0;
     if (c->prec == PREC_FP64 || c->dense) {
       res = davidson(c, n, n, max_iter, tol, PREC_FP64, false, 0, h_evals, h_err);
       res64 = res;
     } else {
-      // Precision ladder.  FP16/BF16 products stop at their accuracy floor
-      // (stagnation); FP32 products then drive the residuals to `tol`; FP64
-      // products of the n Ritz vectors verify (and, if needed, continue).
-      // Each phase restarts from the previous phase's Ritz vectors.
+      // Precision ladder.  Each low-precision phase measures the error of
+      // its products after its first iteration and runs until the residuals
+      // reach floor_k times that error (or stop improving); FP16/BF16 first,
+      // then FP32 (whose floor lies far below `tol`, so it normally converges),
+      // then FP64 products of the n Ritz vectors verify (and, if needed,
+      // continue).  Each phase restarts from the previous phase's Ritz vectors.
       // The FP16 / BF16 phase uses a thick restart (basis cap lp_cap, keep
       // lp_keep Ritz vectors) and hands its lp_keep lowest Ritz vectors to the
       // FP32 phase, which follows the reference rule.
@@ -1527,7 +1564,8 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       bool done = false;
       int start = n, kk = n;
       if (c->prec == PREC_FP16 || c->prec == PREC_BF16) {
-        res = davidson(c, n, n, max_iter, tol, c->prec, false, 3, h_evals, h_err, c->lp_cap(n), c->lp_keep(n), &kk);
+        res = davidson(c, n, n, max_iter, tol, c->prec, false, c->stall_window, h_evals, h_err, c->lp_cap(n),
+                       c->lp_keep(n), &kk, 0.8, 1e300, true, c->floor_k);
         used += res.iters;
         done = res.converged;
         if (kk > n) launch_copy(c->W.as<double>(), c->V.as<double>(), (int64_t)kk * L, c->stream);
@@ -1538,11 +1576,14 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       // exports its lp_keep lowest Ritz vectors so that the FP64 phase does
       // not restart from a thin basis.
       int start64 = n;
+      lp16_iters = used;
       if (!done) {
         int kk32 = n;
-        DavResult r32 = davidson(c, n, start, std::max(1, max_iter - used), tol, PREC_FP32, used > 0, 8, h_evals,
-                                 h_err, 0, c->lp_keep(n), &kk32, 0.9, 10.0 * tol, false);
+        DavResult r32 = davidson(c, n, start, std::max(1, max_iter - used), tol, PREC_FP32, used > 0,
+                                 c->stall_window, h_evals, h_err, 0, c->lp_keep(n), &kk32, 0.8, 1e300,
+                                 c->fp32_thick, c->floor_k);
         used += r32.iters;
+        fp32_iters = r32.iters;
         res.iters = used;
         if (!r32.converged && kk32 > n) start64 = kk32;
       }
@@ -1587,13 +1628,15 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       stats->n_iter_fp64 = fp64_only ? res.iters : res64.iters;
       stats->n_products = c->n_products;
       stats->n_products_fp64 = c->n_products_fp64;
+      stats->n_iter_fp32 = fp32_iters;
+      stats->n_products_fp32 = c->n_products_fp32;
       stats->converged = res64.converged ? 1 : 0;
       stats->n_atom_blocks = c->n_blocks;
       stats->max_residual = res64.max_err;
       stats->t_qbuild_ms = tq;
       stats->t_products_ms = c->t_products_ms;
       for (int st = 0; st < 2; ++st)
-        for (int p = 0; p < 2; ++p) {
+        for (int p = 0; p < 3; ++p) {
           stats->exch_ms[st][p] = c->st_ms[st][p];
           stats->exch_launches[st][p] = c->st_n[st][p];
           stats->exch_flops[st][p] = c->st_flops[st][p];
