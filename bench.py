@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "LC-TDA states/sec: mixed-prec batched vs plain FP64, same GPU, ≤1 meV ω err"
 HARTREE_TO_EV = 27.211386245988
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0, "fp16x3": 2500.0}
 KERNEL_NAMES = {
     "ao": ["AO transition-density GEMM L X R^T with gamma_lr Hadamard epilogue",
            "AO back-contraction GEMM R' (gamma_lr o D)^T"],
@@ -55,19 +55,20 @@ def measured_traffic(key):
     return None
 
 
-def class_prec(pclass, prec):
-    """Precision of a kernel-timing class (0 FP64, 1 FP32, 2 the 16-bit type)."""
-    return ["fp64", "fp32", prec if prec in ("fp16", "bf16") else "fp16"][pclass]
+def class_prec(pclass, prec, split=False):
+    """Precision of a kernel-timing class (0 FP64, 1 FP32 accuracy: FP32 or
+    FP16 hi/lo split, 2 the 16-bit type)."""
+    return ["fp64", "fp16x3" if split else "fp32", prec if prec in ("fp16", "bf16") else "fp16"][pclass]
 
 
-def kernel_table(tot_ms, tot_fl, tot_n, operator, prec):
+def kernel_table(tot_ms, tot_fl, tot_n, operator, prec, split=False):
     """Per exchange stage and precision: launches, ms, achieved TFLOP/s."""
     rows = []
     for si in range(tot_ms.shape[0]):
         for pi in range(tot_ms.shape[1]):
             if tot_n[si, pi] == 0:
                 continue
-            kp = class_prec(pi, prec)
+            kp = class_prec(pi, prec, split)
             tf = tot_fl[si, pi] / (tot_ms[si, pi] * 1e-3) / 1e12 if tot_ms[si, pi] > 0 else 0.0
             rows.append({"kernel": KERNEL_NAMES[operator][si], "prec": kp, "launches": int(tot_n[si, pi]),
                          "ms": round(float(tot_ms[si, pi]), 3), "tflops": round(tf, 2),
@@ -256,7 +257,8 @@ def main_fmo(args):
         tot_fl += np.array(r["stats"]["exch_flops"])
         tot_n += np.array(r["stats"]["exch_launches"])
     si, pi = np.unravel_index(np.argmax(tot_ms), tot_ms.shape)
-    kprec = class_prec(pi, prec)
+    split = any(r["stats"].get("split_fp32", 0) for r in res)
+    kprec = class_prec(pi, prec, split)
     achieved = tot_fl[si, pi] / (tot_ms[si, pi] * 1e-3) / 1e12 if tot_ms[si, pi] > 0 else 0.0
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "states/s", "n_gpus": 1, "steps": args.steps,
@@ -351,7 +353,8 @@ def main():
         tot_fl += np.array(st["exch_flops"])
         tot_n += np.array(st["exch_launches"])
     si, pi = np.unravel_index(np.argmax(tot_ms), tot_ms.shape)
-    kprec = class_prec(pi, args.prec)
+    split = any(st.get("split_fp32", 0) for st in stats)
+    kprec = class_prec(pi, args.prec, split)
     achieved = tot_fl[si, pi] / (tot_ms[si, pi] * 1e-3) / 1e12 if tot_ms[si, pi] > 0 else 0.0
     traffic_key = f"{args.system}/{kprec}/{args.operator}/stage{si}"
     roofline = {
@@ -387,7 +390,7 @@ def main():
                   "max_residual": last["max_residual"], "t_qbuild_ms": last["t_qbuild_ms"],
                   "t_products_ms": last["t_products_ms"]},
         "roofline": roofline,
-        "kernels": kernel_table(tot_ms, tot_fl, tot_n, args.operator, args.prec),
+        "kernels": kernel_table(tot_ms, tot_fl, tot_n, args.operator, args.prec, split),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(inp, n, st64["n_iter"], st64["n_products"])

@@ -50,7 +50,8 @@ typedef struct tda_stats {
   int32_t converged;        /* 1 if every root met the tolerance in FP64 */
   int32_t n_atom_blocks;    /* atom blocks per exchange (HBM budget) */
   int32_t n_iter_fp32;      /* iterations of the FP32 phase (mixed precision) */
-  int32_t n_products_fp32;  /* products computed in FP32 */
+  int32_t n_products_fp32;  /* products of the FP32-accuracy phase */
+  int32_t split_fp32;       /* 1: that phase used FP16 hi / lo split products (AO operator) */
   double max_residual;      /* max FP64 residual norm of the returned roots */
   double t_total_ms;        /* tda_run wall time */
   double t_qbuild_ms;       /* q build + gamma_lr.q_oo (inside tda_run if not prepared) */
@@ -58,7 +59,9 @@ typedef struct tda_stats {
   /* exchange contraction kernels, [stage][class].  TDA_OP_CHARGES: stage 0 =
    * X.q_vv, 1 = G^T.T; TDA_OP_AO: stage 0 = transition densities L X R^T
    * (gamma_lr Hadamard epilogue), 1 = their back-contraction with R'.
-   * class 0 = FP64 products, 1 = FP32 products, 2 = FP16 / BF16 products.
+   * class 0 = FP64 products, 1 = FP32-accuracy products (FP32, or FP16 hi / lo
+   * split when split_fp32: flops then count the three MFMA products), 2 =
+   * FP16 / BF16 products.
    * Time of the main MFMA kernel only (device events on the context stream),
    * its launch count and its algorithmic flops. */
   double exch_ms[2][3];

@@ -23,11 +23,14 @@ int tda_debug_gemm(int in_prec, int a_kc, int b_kc, int64_t M, int64_t N, int64_
  * read), inputs converted on the device to in_prec (TDA_PREC_BF16 / FP16).
  * epi 0: C in in_prec; 1: C FP64 = alpha acc + diag[m][n] X[z][m][n];
  * 2: C in in_prec = alpha acc * H[r(m)][r(n)], H [nh][nh], r(x) = hidx[x % hmod]
- * (hidx NULL: r(x) = x).  C is returned as FP64 [batch][M][N]; *used = 1
- * when the dedicated kernel ran (0: the generic kernel). */
+ * (hidx NULL: r(x) = x).  split (FP16 only): A, B and a 16-bit C held as
+ * hi / lo pairs (hi = fp16(v), lo = fp16((v - hi) 2^11)), the product
+ * evaluated as 2^-11 (A_hi B_lo + A_lo B_hi) + A_hi B_hi.  C is returned as
+ * FP64 [batch][M][N] (hi + lo 2^-11 when split); *used = 1 when the
+ * dedicated kernel ran (0: the generic kernel). */
 int tda_debug_gemm16(int in_prec, int epi, int64_t M, int64_t N, int64_t K, int batch, int64_t lda, int64_t ldb,
                      const double* A, const double* B, double alpha, const double* H, int nh, const int* hidx,
-                     int hmod, const double* diag, const double* X, double* C, int* used);
+                     int hmod, const double* diag, const double* X, double* C, int split, int* used);
 
 /* Lowest n eigenpairs of the symmetric m x m matrix H (upper triangle read):
  * evals [n] ascending, Y [n][m] (row r = eigenvector r). */

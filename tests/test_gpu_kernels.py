@@ -196,3 +196,34 @@ def test_gemm16_hadamard(prec, case, shape):
     # H is applied in FP32 (one more rounding of the factor)
     scale = 0.25 * np.einsum("mk,znk->zmn", np.abs(a), np.abs(b), optimize=True) * hf[None]
     assert np.all(np.abs(out - ref) <= ulp * (np.abs(ref) + ACC16 * scale) + ACC16 * scale + 1e-30)
+
+
+@pytest.mark.parametrize("epi", ["store", "diag", "hadamard"])
+@pytest.mark.parametrize("shape", [(37, 53, 70, 2), (200, 333, 501, 2), (520, 520, 72, 60)])
+def test_gemm16_split(epi, shape):
+    """FP16 hi / lo split operands and output: about FP32 accuracy.  Plain
+    FP16 inputs would be off by ~2^-11 per term (~2e-5 of sum |a b| here);
+    the split must stay two orders below that."""
+    nat = _nat()
+    m, n, k, bt = shape
+    print(f"gemm16 split {epi} {shape}", flush=True)
+    rng = np.random.default_rng(3 * m + n + k)
+    a = rng.uniform(-1, 1, (m, k))
+    b = rng.uniform(-1, 1, (bt, n, k))
+    kw = {}
+    ref = np.einsum("mk,znk->zmn", a, b, optimize=True)
+    if epi == "diag":
+        kw = {"diag": rng.uniform(0.1, 1.5, (m, n)), "x": rng.uniform(-1, 1, (bt, m, n))}
+        ref = ref + kw["diag"][None] * kw["x"]
+    elif epi == "hadamard":
+        hidx = np.repeat(np.arange(60), 3)[:170]
+        h = rng.uniform(0.1, 0.5, (60, 60))
+        kw = {"h": h, "hidx": hidx}
+        r = hidx[np.arange(max(m, n)) % len(hidx)]
+        ref = ref * h[np.ix_(r[:m], r[:n])][None]
+    out, used = nat.debug_gemm16(a, b, "fp16", epi, alpha=1.0, pad=8, split=True, **kw)
+    assert used
+    scale = np.einsum("mk,znk->zmn", np.abs(a), np.abs(b), optimize=True)
+    err = np.abs(out - ref)
+    assert err.max() <= 2e-6 * scale.max()
+    assert np.all(err <= 2e-5 * scale + 1e-12)

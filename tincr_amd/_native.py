@@ -42,6 +42,7 @@ class TdaStats(ctypes.Structure):
         ("n_atom_blocks", ctypes.c_int32),
         ("n_iter_fp32", ctypes.c_int32),
         ("n_products_fp32", ctypes.c_int32),
+        ("split_fp32", ctypes.c_int32),
         ("max_residual", ctypes.c_double),
         ("t_total_ms", ctypes.c_double),
         ("t_qbuild_ms", ctypes.c_double),
@@ -181,7 +182,8 @@ def load_debug():
     lib.tda_debug_gemm.argtypes = [i, i, i, i64, i64, i64, D, D, D, ctypes.c_double, ctypes.c_double, i]
     lib.tda_debug_gemm.restype = ctypes.c_int
     I = ctypes.POINTER(ctypes.c_int)
-    lib.tda_debug_gemm16.argtypes = [i, i, i64, i64, i64, i, i64, i64, D, D, ctypes.c_double, D, i, I, i, D, D, D, I]
+    lib.tda_debug_gemm16.argtypes = [i, i, i64, i64, i64, i, i64, i64, D, D, ctypes.c_double, D, i, I, i, D, D, D, i,
+                                     I]
     lib.tda_debug_gemm16.restype = ctypes.c_int
     lib.tda_debug_eigh.argtypes = [i, i, D, D, D]
     lib.tda_debug_eigh.restype = ctypes.c_int
@@ -207,7 +209,8 @@ def debug_gemm(a, b, precision="fp64", a_kc=True, b_kc=False, alpha=1.0, beta=0.
     return out
 
 
-def debug_gemm16(a, b, precision="fp16", epi="store", alpha=1.0, h=None, hidx=None, diag=None, x=None, pad=0):
+def debug_gemm16(a, b, precision="fp16", epi="store", alpha=1.0, h=None, hidx=None, diag=None, x=None, pad=0,
+                 split=False):
     """The K-contiguous 16-bit GEMM: a [M, K], b [batch, N, K] -> [batch, M, N]
     (FP64 copy of the device result) and whether the dedicated kernel ran.
     Rows are stored with at least `pad` NaN elements past K (never read), the
@@ -231,7 +234,7 @@ def debug_gemm16(a, b, precision="fp16", epi="store", alpha=1.0, h=None, hidx=No
                               dptr(hh) if hh is not None else None, hh.shape[0] if hh is not None else 0,
                               hi.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) if hi is not None else None,
                               len(hi) if hi is not None else 0, dptr(dg) if dg is not None else None,
-                              dptr(xx) if xx is not None else None, dptr(out), ctypes.byref(used))
+                              dptr(xx) if xx is not None else None, dptr(out), int(split), ctypes.byref(used))
     if rc != TDA_OK:
         raise TdaError(rc, "tda_debug_gemm16 failed")
     return out, used.value == 1

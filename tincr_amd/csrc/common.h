@@ -16,8 +16,18 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 // Storage / arithmetic precisions of the bulk products.
 enum Prec : int { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16 = 2, PREC_FP16 = 3 };
+// Internal: FP16 hi / lo split operands (about FP32 accuracy from FP16 MFMA),
+// the FP32-accuracy phase of the precision ladder for the AO operator.
+constexpr int PREC_F16X = 4;
 
 inline size_t prec_bytes(int p) { return p == PREC_FP64 ? 8 : p == PREC_FP32 ? 4 : 2; }
+
+// FP16 hi / lo split of a value: hi = fp16(v), lo = fp16((v - hi) 2^11).
+__device__ inline void split_f16(float v, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)v;
+  lo = (_Float16)((v - (float)hi) * 2048.f);
+}
+__device__ inline float join_f16(_Float16 hi, _Float16 lo) { return (float)hi + (float)lo * 4.8828125e-4f; }
 
 // Thrown by launchers / the driver; the C ABI turns it into a status code.
 struct Error {

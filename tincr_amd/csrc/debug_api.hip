@@ -66,27 +66,38 @@ extern "C" int tda_debug_gemm(int in_prec, int a_kc, int b_kc, int64_t M, int64_
 extern "C" int tda_debug_gemm16(int in_prec, int epi, int64_t M, int64_t N, int64_t K, int batch, int64_t lda,
                                 int64_t ldb, const double* A, const double* B, double alpha, const double* H, int nh,
                                 const int* hidx, int hmod, const double* diag, const double* X, double* C,
-                                int* used) {
+                                int split, int* used) {
   return guard([&] {
     TDA_CHECK(in_prec == 2 || in_prec == 3, TDA_ERR_BAD_ARG, "16-bit precision expected");
     TDA_CHECK(epi >= 0 && epi <= 2 && M > 0 && N > 0 && K > 0 && batch > 0 && lda >= K && ldb >= K,
               TDA_ERR_BAD_ARG, "bad gemm16 arguments");
+    TDA_CHECK(!split || in_prec == 3, TDA_ERR_BAD_ARG, "split operands are FP16");
     const DType dt = in_prec == 2 ? DT_BF16 : DT_F16;
+    const int64_t w = split ? 2 : 1;  // split: rows [hi | lo] of width 2 ld
     const size_t na = (size_t)M * lda, nb = (size_t)batch * N * ldb, nc = (size_t)batch * M * N;
-    Buf a64(na * 8), b64(nb * 8), a(na * 2), b(nb * 2), c(nc * 8);
+    Buf a64(na * 8), b64(nb * 8), a(na * 2 * w), b(nb * 2 * w), c(nc * 8 * w);
     TDA_HIP(hipMemcpy(a64.p, A, na * 8, hipMemcpyHostToDevice));
     TDA_HIP(hipMemcpy(b64.p, B, nb * 8, hipMemcpyHostToDevice));
-    launch_convert((const double*)a64.p, na, 1, nullptr, a.p, in_prec, 0);
-    launch_convert((const double*)b64.p, nb, 1, nullptr, b.p, in_prec, 0);
+    if (split) {
+      launch_convert_split((const double*)a64.p, M, lda, lda, a.p, 2 * lda, lda, 0);
+      launch_convert_split((const double*)b64.p, (int64_t)batch * N, ldb, ldb, b.p, 2 * ldb, ldb, 0);
+    } else {
+      launch_convert((const double*)a64.p, na, 1, nullptr, a.p, in_prec, 0);
+      launch_convert((const double*)b64.p, nb, 1, nullptr, b.p, in_prec, 0);
+    }
     Buf h(epi == EPI_HADAMARD ? (size_t)nh * nh * 8 : 8), hi(epi == EPI_HADAMARD && hidx ? (size_t)hmod * 4 : 8);
     Buf dg(epi == EPI_DIAG ? (size_t)M * N * 8 : 8), xs(epi == EPI_DIAG ? nc * 8 : 8);
     GemmArgs g;
     std::memset(&g, 0, sizeof(g));
     g.A = a.p; g.B = b.p; g.C = c.p;
     g.M = M; g.N = N; g.K = K;
-    g.am = plain(lda); g.ak = plain(1); g.bk = plain(1); g.bn = plain(ldb);
-    g.cm = plain(N); g.cn = plain(1);
-    g.batch = batch; g.a_batch = 0; g.b_batch = N * ldb; g.c_batch = M * N;
+    g.am = plain(w * lda); g.ak = plain(1); g.bk = plain(1); g.bn = plain(w * ldb);
+    g.cm = plain(w * N); g.cn = plain(1);
+    g.batch = batch; g.a_batch = 0; g.b_batch = N * w * ldb; g.c_batch = M * w * N;
+    if (split) {
+      g.a_lo = lda;
+      g.b_lo = ldb;
+    }
     g.alpha = alpha;
     g.dm = plain(0); g.dmap = plain(1); g.xm = plain(0); g.xn = plain(0);
     g.hm = plain(0); g.hn = plain(0);
@@ -104,7 +115,9 @@ extern "C" int tda_debug_gemm16(int in_prec, int epi, int64_t M, int64_t N, int6
       g.diag = (const double*)dg.p; g.dm = plain(N); g.dmap = plain(1);
       g.xsrc = (const double*)xs.p; g.xm = plain(N); g.xn = plain(1); g.x_batch = M * N;
       out = DT_F64;
+      g.cm = plain(N); g.c_batch = M * N;
     }
+    if (split && epi != EPI_DIAG) g.c_lo = N;
     const Epi e = (Epi)epi;
     *used = gemm16_ok(dt, out, e, true, true, g, 1) ? 1 : 0;
     GemmPlan plan = gemm_plan(dt, M, N, K, batch, true, true, 0);
@@ -114,6 +127,13 @@ extern "C" int tda_debug_gemm16(int in_prec, int epi, int64_t M, int64_t N, int6
     TDA_HIP(hipDeviceSynchronize());
     if (out == DT_F64) {
       TDA_HIP(hipMemcpy(C, c.p, nc * 8, hipMemcpyDeviceToHost));
+    } else if (split) {
+      // rows [hi | lo]: value = hi + lo 2^-11
+      std::vector<_Float16> raw(nc * 2);
+      TDA_HIP(hipMemcpy(raw.data(), c.p, nc * 4, hipMemcpyDeviceToHost));
+      for (size_t r = 0; r < (size_t)batch * M; ++r)
+        for (int64_t n = 0; n < N; ++n)
+          C[r * N + n] = (double)(float)raw[r * 2 * N + n] + (double)(float)raw[r * 2 * N + N + n] / 2048.0;
     } else {
       std::vector<uint16_t> raw(nc);
       TDA_HIP(hipMemcpy(raw.data(), c.p, nc * 2, hipMemcpyDeviceToHost));

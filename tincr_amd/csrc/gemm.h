@@ -42,6 +42,13 @@ struct GemmArgs {
   IdxMap hm, hn;
   const int* hidx;  // optional row / column gather table (e.g. AO -> atom)
   int64_t hmod;     // its length; indices are reduced modulo hmod first
+  // Split 16-bit operands (16-bit kernel only): a value v is stored as
+  // hi = fp16(v) and lo = fp16((v - hi) * 2^11), lo at element offset a_lo /
+  // b_lo from hi in the same row.  Then
+  //   sum_k A B = 2^-11 sum_k (A_hi B_lo + A_lo B_hi) + sum_k A_hi B_hi
+  // (the lo x lo term, 2^-22 of the value, is dropped).  0 = not split.
+  // c_lo != 0: the result is written split the same way, lo at offset c_lo.
+  int64_t a_lo, b_lo, c_lo;
 };
 
 // Row / column index into H of the Hadamard epilogue.

@@ -74,6 +74,7 @@ void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g,
     if (after_main) TDA_HIP(hipEventRecord(after_main, stream));
     return;
   }
+  TDA_CHECK(!g.a_lo && !g.b_lo && !g.c_lo, 2, "gemm: split operands need the 16-bit K-contiguous kernel");
   GemmPlan pl = plan;
   // 16-bit inputs on the big tile: 128-deep K slices when the K maps allow
   // (TDA_BIG_K; off by default: it spills registers at two workgroups per CU)

@@ -16,6 +16,11 @@
 //   * computes the transposed 16x16 block (B fragment as the MFMA's first
 //     operand), so a lane ends with 4 consecutive columns of one row and
 //     stores them as one 8-byte (16-bit types) or two 16-byte (FP64) vectors.
+// Split operands (a_lo / b_lo, FP16): the K loop runs three segments,
+// A_hi B_lo, A_lo B_hi, then -- after the accumulators are scaled by 2^-11 --
+// A_hi B_hi, which recovers about FP32 accuracy from FP16 MFMA (the hi / lo
+// splitting of Ootomo and Yokota, "Recovering single precision accuracy from
+// Tensor Cores while surpassing the FP32 theoretical peak performance", 2022).
 // MFMA: v_mfma_f32_16x16x32_{f16,bf16}, FP32 accumulation.  Lane l supplies
 // rows l%16, K elements 8*(l/16)..+7 of both operands; the result block
 // holds D[4*(l/16)+r][l%16], r < 4.
@@ -83,7 +88,10 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
   const TI* A = (const TI*)g.A + zb * g.a_batch;
   const TI* B = (const TI*)g.B + zb * g.b_batch;
   const int64_t K = g.K;
-  const int nkt = (int)((K + BK - 1) / BK);
+  const int nks = (int)((K + BK - 1) / BK);  // K slices per segment
+  const bool split_in = g.a_lo != 0;
+  const int nkt = split_in ? 3 * nks : nks;
+  const uint32_t a_lo = (uint32_t)g.a_lo, b_lo = (uint32_t)g.b_lo;
 
   // Loader: chunk c = tid + i*NT of a slice is row c/8 (= tid/8 + i*NT/8),
   // K chunk c%8 (= tid%8).  Rows past M / N read row M-1 / N-1 (results
@@ -97,36 +105,61 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
   for (int i = 0; i < B_PER; ++i)
     b_off[i] = (uint32_t)(min(n0 + lr + i * (NT / 8), g.N - 1) * ldb) + lkc * 8;
   VecN<TI, 8> ra[Cfg::PF][A_PER], rb[Cfg::PF][B_PER];
+  v4f acc[MI][NI];
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < NI; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
 
   auto load = [&](int kt, auto set_c) {
     constexpr int P = decltype(set_c)::value;
-    const uint32_t k0 = (uint32_t)kt * BK;
+    // segment (split operands): 0 = A_hi B_lo, 1 = A_lo B_hi, 2 = A_hi B_hi
+    int ks = kt;
+    uint32_t ao = 0, bo = 0;
+    if (split_in) {
+      const int seg = kt / nks;
+      ks = kt - seg * nks;
+      ao = seg == 1 ? a_lo : 0u;
+      bo = seg == 0 ? b_lo : 0u;
+    }
+    const uint32_t k0 = (uint32_t)ks * BK;
+    const uint32_t ka = k0 + ao, kb = k0 + bo;
     if ((int64_t)k0 + BK <= K) {
 #pragma unroll
-      for (int i = 0; i < A_PER; ++i) ra[P][i] = *(const VecN<TI, 8>*)(A + (a_off[i] + k0));
+      for (int i = 0; i < A_PER; ++i) ra[P][i] = *(const VecN<TI, 8>*)(A + (a_off[i] + ka));
 #pragma unroll
-      for (int i = 0; i < B_PER; ++i) rb[P][i] = *(const VecN<TI, 8>*)(B + (b_off[i] + k0));
+      for (int i = 0; i < B_PER; ++i) rb[P][i] = *(const VecN<TI, 8>*)(B + (b_off[i] + kb));
     } else {
-      // last slice: elements past K read as zero
+      // last slice of a segment: elements past K read as zero
       const int64_t kk = (int64_t)k0 + lkc * 8;
 #pragma unroll
       for (int i = 0; i < A_PER; ++i) {
         if (kk + 8 <= K) {
-          ra[P][i] = *(const VecN<TI, 8>*)(A + (a_off[i] + k0));
+          ra[P][i] = *(const VecN<TI, 8>*)(A + (a_off[i] + ka));
         } else {
 #pragma unroll
-          for (int e = 0; e < 8; ++e) ra[P][i].v[e] = (kk + e < K) ? A[a_off[i] + k0 + e] : (TI)0;
+          for (int e = 0; e < 8; ++e) ra[P][i].v[e] = (kk + e < K) ? A[a_off[i] + ka + e] : (TI)0;
         }
       }
 #pragma unroll
       for (int i = 0; i < B_PER; ++i) {
         if (kk + 8 <= K) {
-          rb[P][i] = *(const VecN<TI, 8>*)(B + (b_off[i] + k0));
+          rb[P][i] = *(const VecN<TI, 8>*)(B + (b_off[i] + kb));
         } else {
 #pragma unroll
-          for (int e = 0; e < 8; ++e) rb[P][i].v[e] = (kk + e < K) ? B[b_off[i] + k0 + e] : (TI)0;
+          for (int e = 0; e < 8; ++e) rb[P][i].v[e] = (kk + e < K) ? B[b_off[i] + kb + e] : (TI)0;
         }
       }
+    }
+  };
+  // split operands: the correction segments are done once 2 nks slices are in
+  auto rescale = [&](int kt) {
+    if (split_in && kt == 2 * nks) {
+#pragma unroll
+      for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NI; ++j) acc[i][j] *= 4.8828125e-4f;  // 2^-11
     }
   };
   // LDS position of this thread's chunks: row lr + i*NT/8, chunk lkc ^ (row & 7)
@@ -140,12 +173,6 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) *(VecN<TI, 8>*)(s + BM * BK + lds_a0 + i * (NT / 8) * BK) = rb[P][i];
   };
-
-  v4f acc[MI][NI];
-#pragma unroll
-  for (int i = 0; i < MI; ++i)
-#pragma unroll
-    for (int j = 0; j < NI; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
   // fragment row (within the wave tile) and its swizzle key
   const int fr = lane & 15, fq = lane >> 4;
@@ -179,6 +206,7 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
     }
     for (int kt = 0; kt < nkt; ++kt) {
       if (kt + 1 < nkt) load(kt + 1, I0{});
+      rescale(kt);
       compute(kt & 1);
       if (kt + 1 < nkt) store((kt + 1) & 1, I0{});
       __syncthreads();
@@ -198,6 +226,7 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
       using IP = std::integral_constant<int, P>;
       using IQ = std::integral_constant<int, 1 - P>;
       if (kt + 2 < nkt) load(kt + 2, IP{});
+      rescale(kt);
       compute(P);
       if (kt + 1 < nkt) store(1 - P, IQ{});
       __syncthreads();
@@ -295,6 +324,36 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
       }
     __syncthreads();
   }
+  // 16-bit outputs: 4 consecutive columns of row offset cmo, as one 8-byte
+  // vector when contiguous and aligned; with c_lo also the lo part
+  // fp16((v - hi) * 2^11) at offset c_lo.
+  auto put4 = [&](int64_t cmo, bool c_al, int j, const float* v) {
+    TO hi[4], lo[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      hi[r] = (TO)v[r];
+      lo[r] = (TO)((v[r] - (float)hi[r]) * 2048.f);
+    }
+    if (cn_vec[j] && c_al && (g.c_lo % 4 == 0)) {
+      Vec4<TO> w;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) w.v[r] = hi[r];
+      *(Vec4<TO>*)(C + cmo + cn_off[j]) = w;
+      if (g.c_lo) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) w.v[r] = lo[r];
+        *(Vec4<TO>*)(C + cmo + cn_off[j] + g.c_lo) = w;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (n_first[j] + r >= g.N) continue;
+        const int64_t o = cmo + g.cn(n_first[j] + r);
+        C[o] = hi[r];
+        if (g.c_lo) C[o + g.c_lo] = lo[r];
+      }
+    }
+  };
 #pragma unroll
   for (int i = 0; i < MI; ++i) {
     const int64_t m = m0 + wm0 + i * 16 + fr;
@@ -311,16 +370,7 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
           const int h = hrow[i] + hcol[j][r];
           v[r] = alpha * acc[i][j][r] * (staged ? ht[h] : (float)g.hsrc[h]);
         }
-        if (cn_vec[j] && c_al) {
-          Vec4<TO> w;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) w.v[r] = (TO)v[r];
-          *(Vec4<TO>*)(C + cmo + cn_off[j]) = w;
-        } else {
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (n_first[j] + r < g.N) C[cmo + g.cn(n_first[j] + r)] = (TO)v[r];
-        }
+        put4(cmo, c_al, j, v);
       }
     } else if constexpr (EPI == EPI_DIAG) {
       // FP64 output: C = alpha acc + diag[dm(m) + dmap(n)] * X[z][xm(m) + xn(n)]
@@ -341,16 +391,10 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
 #pragma unroll
       for (int j = 0; j < NI; ++j) {
         if (n_first[j] >= g.N) continue;
-        if (cn_vec[j] && c_al) {
-          Vec4<TO> w;
+        float v[4];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) w.v[r] = (TO)(alpha * acc[i][j][r]);
-          *(Vec4<TO>*)(C + cmo + cn_off[j]) = w;
-        } else {
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (n_first[j] + r < g.N) C[cmo + g.cn(n_first[j] + r)] = (TO)(alpha * acc[i][j][r]);
-        }
+        for (int r = 0; r < 4; ++r) v[r] = alpha * acc[i][j][r];
+        put4(cmo, c_al, j, v);
       }
     }
   }
@@ -367,9 +411,14 @@ inline bool gemm16_ok(DType in, DType out, Epi epi, bool a_kc, bool b_kc, const 
   if ((((uintptr_t)g.A) & 15) || (((uintptr_t)g.B) & 15)) return false;
   if (g.beta != 0.0) return false;
   // 32-bit element offsets within one batch entry of each operand
-  if (g.M < 1 || g.N < 1 || (g.M - 1) * g.am.s0 + g.K >= (INT64_C(1) << 32) ||
-      (g.N - 1) * g.bn.s0 + g.K >= (INT64_C(1) << 32))
+  if (g.M < 1 || g.N < 1 || (g.M - 1) * g.am.s0 + g.a_lo + g.K >= (INT64_C(1) << 32) ||
+      (g.N - 1) * g.bn.s0 + g.b_lo + g.K >= (INT64_C(1) << 32))
     return false;
+  // split operands: FP16, both split (lo rows 16-byte aligned), lo past hi
+  if ((g.a_lo != 0) != (g.b_lo != 0) || g.a_lo < 0 || g.b_lo < 0 || g.c_lo < 0) return false;
+  if ((g.a_lo || g.c_lo) && in != DT_F16) return false;
+  if (g.a_lo && (g.a_lo % 8 || g.b_lo % 8 || g.a_lo < g.K || g.b_lo < g.K)) return false;
+  if (g.c_lo && epi == EPI_DIAG) return false;
   if (epi == EPI_DIAG) return out == DT_F64;
   // EPI_HADAMARD: plain H maps with 32-bit offsets
   if (epi == EPI_HADAMARD &&

@@ -21,6 +21,9 @@ void launch_block_amax(const double* x, int64_t len, int n, float* amax, hipStre
 // dst[b][e] = (T) (src[b][e] * inv_scale[b])   (inv_scale may be null => 1)
 void launch_convert(const double* src, int64_t len, int n, const float* inv_scale, void* dst, int dtype,
                     hipStream_t s);
+// FP16 hi / lo split copy: dst[r][c] = hi(src[r][c]), dst[r][lo_off + c] = lo
+void launch_convert_split(const double* src, int64_t rows, int64_t cols, int64_t ld_src, void* dst, int64_t ld_dst,
+                          int64_t lo_off, hipStream_t s);
 void launch_convert_f32_to(const float* src, int64_t n, void* dst, int dtype, hipStream_t s);
 
 // ---- layout conversions ----
@@ -72,10 +75,12 @@ void launch_ao_operand(const double* C, const double* SC, int n_ao, int base, in
 // with row stride ld, element type dtype):
 // u[A*k + c] = 1/2 sum_{mu in A} (D[(0,mu),(1,mu)] + D[(1,mu),(0,mu)])
 void launch_ao_charges(const void* M, int dtype, int k, int n_ao, int64_t ld, const int* ao_off, int n_atoms,
-                       const double* gamma_lr, double* u, hipStream_t s);
+                       const double* gamma_lr, double* u, hipStream_t s,
+                       int64_t lo_off = 0);
 // M[c][(0,mu),(1,mu)] += w[atom(mu)*k + c] and M[c][(1,mu),(0,mu)] += same
 void launch_ao_add_coulomb(void* M, int dtype, int k, int n_ao, int64_t ld, const int* atom, const double* w,
-                           hipStream_t s);
+                           hipStream_t s,
+                           int64_t lo_off = 0);
 
 // ---- subspace eigensolver (eigh, davidson.rs:93) ----
 // H: m x m symmetric (upper triangle read), leading dim ldh; work >= eigh_work_doubles(m)

@@ -1,5 +1,7 @@
 // Streaming (HBM-bound) kernels of the TDA path: transition-charge build,
 // layout conversions and the Davidson vector updates.
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace tda {
@@ -144,6 +146,28 @@ void launch_convert(const double* src, int64_t len, int n, const float* inv_scal
     case PREC_FP16: convert_kernel<_Float16><<<g, 256, 0, s>>>(src, len, n, inv_scale, (_Float16*)dst); break;
     default: convert_kernel<double><<<g, 256, 0, s>>>(src, len, n, inv_scale, (double*)dst); break;
   }
+  TDA_LAUNCH_CHECK();
+}
+
+// dst[r][c] = hi, dst[r][lo_off + c] = lo of src[r][c] (rows of `cols`, row strides ld_src / ld_dst)
+__global__ void __launch_bounds__(256) convert_split_kernel(const double* __restrict__ src, int64_t rows,
+                                                            int64_t cols, int64_t ld_src, _Float16* __restrict__ dst,
+                                                            int64_t ld_dst, int64_t lo_off) {
+  const int64_t total = rows * cols;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    const int64_t r = t / cols, c = t - r * cols;
+    _Float16 hi, lo;
+    split_f16((float)src[r * ld_src + c], hi, lo);
+    dst[r * ld_dst + c] = hi;
+    dst[r * ld_dst + lo_off + c] = lo;
+  }
+}
+
+void launch_convert_split(const double* src, int64_t rows, int64_t cols, int64_t ld_src, void* dst, int64_t ld_dst,
+                          int64_t lo_off, hipStream_t s) {
+  const int64_t total = rows * cols;
+  if (total == 0) return;
+  convert_split_kernel<<<grid_for(total), 256, 0, s>>>(src, rows, cols, ld_src, (_Float16*)dst, ld_dst, lo_off);
   TDA_LAUNCH_CHECK();
 }
 
@@ -530,7 +554,7 @@ template <typename T>
 __global__ void __launch_bounds__(64) ao_charges_kernel(const T* __restrict__ M, int k, int n_ao, int64_t ld,
                                                         const int* __restrict__ ao_off,
                                                         const double* __restrict__ gamma_lr, int n_atoms,
-                                                        double* __restrict__ u) {
+                                                        double* __restrict__ u, int64_t lo_off) {
   const int A = blockIdx.x;
   const int c = blockIdx.y;
   const int64_t n2 = 2 * (int64_t)n_ao;
@@ -539,8 +563,12 @@ __global__ void __launch_bounds__(64) ao_charges_kernel(const T* __restrict__ M,
   const double inv = -4.0 / gamma_lr[(int64_t)A * n_atoms + A];
   double acc = 0.0;
   for (int mu = ao_off[A] + threadIdx.x; mu < ao_off[A + 1]; mu += 64) {
-    const double d2 = to_double<T>(Mc[(int64_t)mu * ld + n_ao + mu]);
-    const double d3 = to_double<T>(Mc[(int64_t)(n_ao + mu) * ld + mu]);
+    const int64_t i2 = (int64_t)mu * ld + n_ao + mu, i3 = (int64_t)(n_ao + mu) * ld + mu;
+    double d2 = to_double<T>(Mc[i2]), d3 = to_double<T>(Mc[i3]);
+    if (lo_off) {  // FP16 hi / lo split storage
+      d2 += to_double<T>(Mc[i2 + lo_off]) * 4.8828125e-4;
+      d3 += to_double<T>(Mc[i3 + lo_off]) * 4.8828125e-4;
+    }
     acc += 0.5 * (d2 + d3) * inv;
   }
   acc = wave_sum(acc);
@@ -548,20 +576,24 @@ __global__ void __launch_bounds__(64) ao_charges_kernel(const T* __restrict__ M,
 }
 
 void launch_ao_charges(const void* M, int dtype, int k, int n_ao, int64_t ld, const int* ao_off, int n_atoms,
-                       const double* gamma_lr, double* u, hipStream_t s) {
+                       const double* gamma_lr, double* u, hipStream_t s, int64_t lo_off) {
   dim3 grid(n_atoms, k);
   switch (dtype) {
     case PREC_FP64:
-      ao_charges_kernel<double><<<grid, 64, 0, s>>>((const double*)M, k, n_ao, ld, ao_off, gamma_lr, n_atoms, u);
+      ao_charges_kernel<double><<<grid, 64, 0, s>>>((const double*)M, k, n_ao, ld, ao_off, gamma_lr, n_atoms, u,
+                                                    lo_off);
       break;
     case PREC_FP32:
-      ao_charges_kernel<float><<<grid, 64, 0, s>>>((const float*)M, k, n_ao, ld, ao_off, gamma_lr, n_atoms, u);
+      ao_charges_kernel<float><<<grid, 64, 0, s>>>((const float*)M, k, n_ao, ld, ao_off, gamma_lr, n_atoms, u,
+                                                    lo_off);
       break;
     case PREC_BF16:
-      ao_charges_kernel<__bf16><<<grid, 64, 0, s>>>((const __bf16*)M, k, n_ao, ld, ao_off, gamma_lr, n_atoms, u);
+      ao_charges_kernel<__bf16><<<grid, 64, 0, s>>>((const __bf16*)M, k, n_ao, ld, ao_off, gamma_lr, n_atoms, u,
+                                                    lo_off);
       break;
     default:
-      ao_charges_kernel<_Float16><<<grid, 64, 0, s>>>((const _Float16*)M, k, n_ao, ld, ao_off, gamma_lr, n_atoms, u);
+      ao_charges_kernel<_Float16><<<grid, 64, 0, s>>>((const _Float16*)M, k, n_ao, ld, ao_off, gamma_lr, n_atoms, u,
+                                                    lo_off);
       break;
   }
   TDA_LAUNCH_CHECK();
@@ -570,7 +602,7 @@ void launch_ao_charges(const void* M, int dtype, int k, int n_ao, int64_t ld, co
 template <typename T>
 __global__ void __launch_bounds__(256) ao_add_coulomb_kernel(T* __restrict__ M, int k, int n_ao, int64_t ld,
                                                              const int* __restrict__ atom,
-                                                             const double* __restrict__ w) {
+                                                             const double* __restrict__ w, int64_t lo_off) {
   const int64_t n2 = 2 * (int64_t)n_ao;
   const int64_t total = (int64_t)k * n_ao;
   for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
@@ -580,19 +612,28 @@ __global__ void __launch_bounds__(256) ao_add_coulomb_kernel(T* __restrict__ M, 
     T* Mc = M + (int64_t)c * n2 * ld;
     T* p1 = Mc + mu * ld + n_ao + mu;
     T* p2 = Mc + (n_ao + mu) * ld + mu;
+    if constexpr (std::is_same<T, _Float16>::value) {
+      if (lo_off) {  // FP16 hi / lo split storage: add to the joined value, split again
+        const float v1 = (float)((double)join_f16(p1[0], p1[lo_off]) + add);
+        split_f16(v1, p1[0], p1[lo_off]);
+        const float v2 = (float)((double)join_f16(p2[0], p2[lo_off]) + add);
+        split_f16(v2, p2[0], p2[lo_off]);
+        continue;
+      }
+    }
     *p1 = to_type<T>(to_double<T>(*p1) + add);
     *p2 = to_type<T>(to_double<T>(*p2) + add);
   }
 }
 
 void launch_ao_add_coulomb(void* M, int dtype, int k, int n_ao, int64_t ld, const int* atom, const double* w,
-                           hipStream_t s) {
+                           hipStream_t s, int64_t lo_off) {
   const int g = grid_for((int64_t)k * n_ao);
   switch (dtype) {
-    case PREC_FP64: ao_add_coulomb_kernel<double><<<g, 256, 0, s>>>((double*)M, k, n_ao, ld, atom, w); break;
-    case PREC_FP32: ao_add_coulomb_kernel<float><<<g, 256, 0, s>>>((float*)M, k, n_ao, ld, atom, w); break;
-    case PREC_BF16: ao_add_coulomb_kernel<__bf16><<<g, 256, 0, s>>>((__bf16*)M, k, n_ao, ld, atom, w); break;
-    default: ao_add_coulomb_kernel<_Float16><<<g, 256, 0, s>>>((_Float16*)M, k, n_ao, ld, atom, w); break;
+    case PREC_FP64: ao_add_coulomb_kernel<double><<<g, 256, 0, s>>>((double*)M, k, n_ao, ld, atom, w, lo_off); break;
+    case PREC_FP32: ao_add_coulomb_kernel<float><<<g, 256, 0, s>>>((float*)M, k, n_ao, ld, atom, w, lo_off); break;
+    case PREC_BF16: ao_add_coulomb_kernel<__bf16><<<g, 256, 0, s>>>((__bf16*)M, k, n_ao, ld, atom, w, lo_off); break;
+    default: ao_add_coulomb_kernel<_Float16><<<g, 256, 0, s>>>((_Float16*)M, k, n_ao, ld, atom, w, lo_off); break;
   }
   TDA_LAUNCH_CHECK();
 }

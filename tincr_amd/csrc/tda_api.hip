@@ -112,6 +112,7 @@ struct tda_ctx {
   double floor_k = 2.0;
   int stall_window = 10;
   bool fp32_thick = true;
+  bool split32 = true;  // FP32-accuracy phase on FP16 hi / lo split products (AO operator)
   int64_t ovp = 0;
   std::vector<double> xyz, h_de;
   DevBuf C, S, SC, orbe, aooff, gamma, gamma_lr;
@@ -128,7 +129,7 @@ struct tda_ctx {
   struct LpSet {
     bool ready = false;
     DevBuf qvv, G, Rt, Lm, Rpt, Lpt, scale;  // scale: [s_B*g_B | 1/s_B | 1/g_B] per atom
-  } lps[4];
+  } lps[5];  // per PREC_* (index PREC_F16X: FP16 hi / lo split)
   DevBuf qov32, qvv_amax, G_amax;
   // workspaces
   DevBuf ws_gemm, T, xlp, u, uu, vv32, ax_tmp, stage, stage2;
@@ -272,9 +273,18 @@ void prepare_lowprec(tda_ctx* c, int prec) {
     struct Pair { DevBuf* src; DevBuf* dst; int64_t len; };
     Pair pairs[4] = {{&c->aoRt, &ls.Rt, n2 * c->vp}, {&c->aoLm, &ls.Lm, n2 * c->op},
                      {&c->aoRpt, &ls.Rpt, ld * c->vp}, {&c->aoLpt, &ls.Lpt, ld * c->op}};
-    for (Pair& pr : pairs) {
-      pr.dst->ensure((size_t)pr.len * esz);
-      launch_convert(pr.src->as<double>(), pr.len, 1, nullptr, pr.dst->p, prec, c->stream);
+    // row widths of the four operands (rows x width = len)
+    const int64_t width[4] = {c->vp, c->op, ld, ld};
+    for (int i = 0; i < 4; ++i) {
+      Pair& pr = pairs[i];
+      if (prec == PREC_F16X) {
+        pr.dst->ensure((size_t)pr.len * 2 * esz);
+        launch_convert_split(pr.src->as<double>(), pr.len / width[i], width[i], width[i], pr.dst->p, 2 * width[i],
+                             width[i], c->stream);
+      } else {
+        pr.dst->ensure((size_t)pr.len * esz);
+        launch_convert(pr.src->as<double>(), pr.len, 1, nullptr, pr.dst->p, prec, c->stream);
+      }
     }
     TDA_HIP(hipStreamSynchronize(c->stream));
     ls.ready = true;
@@ -549,10 +559,19 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   const int o = c->o, v = c->v, op = c->op, vp = c->vp, nao = c->nao, N = c->N;
   const int64_t ovp = c->ovp, n2 = 2 * (int64_t)nao, ld = ao_ld(c);
   const bool lp = prec != PREC_FP64;
+  // PREC_F16X: every 16-bit operand row holds hi then lo (row width w = 2
+  // rows of the plain layout; lo at +vp / +op / +ld)
+  const bool sp = prec == PREC_F16X;
+  const int64_t w = sp ? 2 : 1;
   const DType dt = lp ? dtype_of(prec) : DT_F64;
   const size_t esz = lp ? prec_bytes(prec) : 8;
+  const int mprec = lp ? (sp ? PREC_FP16 : prec) : PREC_FP64;  // element type of M
   const void* Xin = X;
-  if (lp) {
+  if (sp) {
+    c->xlp.ensure((size_t)k * ovp * 2 * esz);
+    launch_convert_split(X, (int64_t)k * o, vp, vp, c->xlp.p, 2 * vp, vp, c->stream);
+    Xin = c->xlp.p;
+  } else if (lp) {
     c->xlp.ensure((size_t)k * ovp * esz);
     launch_cast_vectors(X, (int64_t)k * ovp, c->xlp.p, prec, c->stream);
     Xin = c->xlp.p;
@@ -562,28 +581,30 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   const void* Lm = lp ? c->lps[prec].Lm.p : c->aoLm.p;
   const void* Rpt = lp ? c->lps[prec].Rpt.p : c->aoRpt.p;
   const void* Lpt = lp ? c->lps[prec].Lpt.p : c->aoLpt.p;
-  c->aoXR.ensure((size_t)k * n2 * op * esz);
-  c->aoM.ensure((size_t)k * n2 * ld * esz);
-  c->aoF.ensure((size_t)k * vp * ld * esz);
-  const int pclass = prec == PREC_FP64 ? 0 : prec == PREC_FP32 ? 1 : 2;
+  c->aoXR.ensure((size_t)k * n2 * w * op * esz);
+  c->aoM.ensure((size_t)k * n2 * w * ld * esz);
+  c->aoF.ensure((size_t)k * vp * w * ld * esz);
+  const int pclass = prec == PREC_FP64 ? 0 : (prec == PREC_FP32 || sp) ? 1 : 2;
   hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
   if (c->timing)
     for (auto& ev : e) ev = c->get_event();
   // G1: XR[c][(s,mu)][j] = sum_b Rt[(s,mu)][b] X[c][j][b]
   {
-    GemmArgs g = gargs(Rt, Xin, c->aoXR.p, n2, (int64_t)k * o, vp, plain(vp), plain(1), plain(1), plain(vp),
-                       plain(op), split(o, n2 * op, 1));
+    GemmArgs g = gargs(Rt, Xin, c->aoXR.p, n2, (int64_t)k * o, vp, plain(w * vp), plain(1), plain(1), plain(w * vp),
+                       plain(w * op), split(o, n2 * w * op, 1));
+    if (sp) g.a_lo = g.b_lo = vp, g.c_lo = op;
     run_gemm(c, dt, dt, EPI_STORE, true, true, g);
   }
   // G2: M[c] = -1/4 gamma~ o (Lm XR[c]^T)
   if (c->timing) TDA_HIP(hipEventRecord(e[0], c->stream));
   {
-    GemmArgs g = gargs(Lm, c->aoXR.p, c->aoM.p, n2, n2, o, plain(op), plain(1), plain(1), plain(op), plain(ld),
-                       plain(1));
+    GemmArgs g = gargs(Lm, c->aoXR.p, c->aoM.p, n2, n2, o, plain(w * op), plain(1), plain(1), plain(w * op),
+                       plain(w * ld), plain(1));
     g.batch = k;
-    g.b_batch = n2 * op;
-    g.c_batch = n2 * ld;
+    g.b_batch = n2 * w * op;
+    g.c_batch = n2 * w * ld;
     g.alpha = -0.25;
+    if (sp) g.a_lo = g.b_lo = op, g.c_lo = ld;
     // gamma~_{(s,mu),(t,nu)} = gamma_lr[atom(mu)][atom(nu)], gathered per row / column
     g.hsrc = c->gamma_lr.as<double>();
     g.hidx = c->ao_atom.as<int>();
@@ -595,27 +616,30 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   // Coulomb: u = q_ov x from the D diagonals, w = gamma u, added to M
   c->u.ensure((size_t)N * k * 8);
   c->uu.ensure((size_t)N * k * 8);
-  launch_ao_charges(c->aoM.p, lp ? prec : PREC_FP64, k, nao, ld, c->aooff.as<int>(), N, c->gamma_lr.as<double>(),
-                    c->u.as<double>(), c->stream);
+  launch_ao_charges(c->aoM.p, mprec, k, nao, w * ld, c->aooff.as<int>(), N, c->gamma_lr.as<double>(),
+                    c->u.as<double>(), c->stream, sp ? ld : 0);
   gamma_times(c, N, k);
-  launch_ao_add_coulomb(c->aoM.p, lp ? prec : PREC_FP64, k, nao, ld, c->ao_atom.as<int>(), c->uu.as<double>(),
-                        c->stream);
+  launch_ao_add_coulomb(c->aoM.p, mprec, k, nao, w * ld, c->ao_atom.as<int>(), c->uu.as<double>(), c->stream,
+                        sp ? ld : 0);
   // G3: F[c][a][(t,nu)] = sum_(s,mu) Rpt[a][(s,mu)] M[c][(t,nu)][(s,mu)]
   if (c->timing) TDA_HIP(hipEventRecord(e[2], c->stream));
   {
-    GemmArgs g = gargs(Rpt, c->aoM.p, c->aoF.p, vp, n2, n2, plain(ld), plain(1), plain(1), plain(ld), plain(ld),
-                       plain(1));
+    GemmArgs g = gargs(Rpt, c->aoM.p, c->aoF.p, vp, n2, n2, plain(w * ld), plain(1), plain(1), plain(w * ld),
+                       plain(w * ld), plain(1));
     g.batch = k;
-    g.b_batch = n2 * ld;
-    g.c_batch = vp * ld;
+    g.b_batch = n2 * w * ld;
+    g.c_batch = vp * w * ld;
+    if (sp) g.a_lo = g.b_lo = ld, g.c_lo = ld;
     run_gemm(c, dt, dt, EPI_STORE, true, true, g, 0, e[3]);
   }
   // G4: AX[c][i][a] = de[i][a] X[c][i][a] + sum_(t,nu) Lpt[i][(t,nu)] F[c][a][(t,nu)]
   {
-    GemmArgs g = gargs(Lpt, c->aoF.p, AX, o, vp, n2, plain(ld), plain(1), plain(1), plain(ld), plain(vp), plain(1));
+    GemmArgs g = gargs(Lpt, c->aoF.p, AX, o, vp, n2, plain(w * ld), plain(1), plain(1), plain(w * ld), plain(vp),
+                       plain(1));
     g.batch = k;
-    g.b_batch = vp * ld;
+    g.b_batch = vp * w * ld;
     g.c_batch = ovp;
+    if (sp) g.a_lo = g.b_lo = ld;
     g.diag = c->de.as<double>();
     g.dm = plain(vp);
     g.dmap = plain(1);
@@ -626,8 +650,10 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
     run_gemm(c, dt, DT_F64, EPI_DIAG, true, true, g);
   }
   if (c->timing) {
-    c->recs.push_back({e[0], e[1], 0, pclass, 2.0 * k * (double)n2 * n2 * o});
-    c->recs.push_back({e[2], e[3], 1, pclass, 2.0 * k * (double)n2 * n2 * vp});
+    // MFMA flops of the launches (three products per split GEMM)
+    const double f = sp ? 3.0 : 1.0;
+    c->recs.push_back({e[0], e[1], 0, pclass, f * 2.0 * k * (double)n2 * n2 * o});
+    c->recs.push_back({e[2], e[3], 1, pclass, f * 2.0 * k * (double)n2 * n2 * vp});
   }
 }
 
@@ -642,11 +668,13 @@ void products(tda_ctx* c, const double* X, int k, double* AX, int prec) {
     return;
   }
   const bool ao = c->has_lc && c->op_kind == 0 && !c->ct;
+  if (prec == PREC_F16X && !ao) prec = PREC_FP32;  // split products exist for the AO operator only
   int chunk = 64;
   if (ao) {
     // per-vector AO workspace: XR + M + F
     const double n2 = 2.0 * c->nao, ld = (double)ao_ld(c);
-    const double per = (n2 * c->op + n2 * ld + c->vp * ld) * (prec == PREC_FP64 ? 8.0 : prec_bytes(prec));
+    const double per = (n2 * c->op + n2 * ld + c->vp * ld) * (prec == PREC_FP64 ? 8.0 : prec_bytes(prec)) *
+                       (prec == PREC_F16X ? 2.0 : 1.0);
     chunk = (int)std::max(1.0, std::min(64.0, std::min((double)c->budget, 8.0 * (1ull << 30)) / per));
   }
   for (int c0 = 0; c0 < k; c0 += chunk) {
@@ -656,7 +684,7 @@ void products(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   }
   c->n_products += k;
   if (prec == PREC_FP64) c->n_products_fp64 += k;
-  if (prec == PREC_FP32) c->n_products_fp32 += k;
+  if (prec == PREC_FP32 || prec == PREC_F16X) c->n_products_fp32 += k;
 }
 
 // ---------------------------------------------------------------------------
@@ -783,10 +811,13 @@ struct DavResult {
 // norm), the size of the product error every residual of the phase carries,
 // and leaves once max residual < floor_k * delta: below that the computed
 // residuals no longer say how far the roots are from convergence.
+// reorth: a second Gram-Schmidt pass for the new basis vectors (the phases
+// of the precision ladder; the FP64 path keeps the reference's single pass).
 DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec, bool first_can_converge,
                    int stop_stall, std::vector<double>& h_evals, std::vector<double>& h_err, int cap_in = 0,
                    int keep = 0, int* kk_out = nullptr, double stall_factor = 0.5,
-                   double stall_floor = 1e300, bool thick_restart = true, double floor_k = 0.0) {
+                   double stall_floor = 1e300, bool thick_restart = true, double floor_k = 0.0,
+                   bool reorth = false) {
   const int64_t L = c->ovp;
   const int cap = cap_in > 0 ? cap_in : 15 * n;  // max_space (davidson.rs:74)
   const int ldh = 15 * n;                         // leading dimension of H and Y
@@ -945,6 +976,26 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
         int h_count = 0;
         TDA_HIP(hipMemcpyAsync(&h_count, d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         TDA_HIP(hipStreamSynchronize(c->stream));
+        if (reorth && h_count > 0) {
+          // second Gram-Schmidt pass for the accepted vectors Q ("twice is
+          // enough"): a single classical pass loses orthogonality when the
+          // preconditioned residuals lie nearly in the basis, and a nearly
+          // dependent basis shows up as spurious Ritz values near 0
+          double* Q = V + (size_t)m * L;
+          GemmArgs r0 = gargs(V, Q, C0, m, h_count, L, plain(L), plain(1), plain(1), plain(L), plain(h_count),
+                              plain(1));
+          run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, r0);
+          GemmArgs r1 = gargs(C0, V, Q, h_count, L, m, plain(1), plain(h_count), plain(L), plain(1), plain(L),
+                              plain(1));
+          r1.alpha = -1.0;
+          r1.beta = 1.0;
+          run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, r1);
+          launch_copy(Q, P, (int64_t)h_count * L, c->stream);
+          launch_copy(Q, W, (int64_t)h_count * L, c->stream);
+          launch_gs_block(P, W, Q, L, h_count, c->gs_scratch.as<double>(), d_count, c->stream);
+          TDA_HIP(hipMemcpyAsync(&h_count, d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+          TDA_HIP(hipStreamSynchronize(c->stream));
+        }
         m += h_count;
       }
     } else if (thick) {
@@ -1067,6 +1118,8 @@ int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx**
     if (es && atoi(es) >= 1) c->stall_window = atoi(es);
     const char* et = getenv("TINCR_TDA_FP32_THICK");
     if (et) c->fp32_thick = atoi(et) != 0;
+    const char* ep = getenv("TINCR_TDA_SPLIT32");
+    if (ep) c->split32 = atoi(ep) != 0;
   }
   if (hbm_budget_bytes) c->budget = hbm_budget_bytes;
   int rc = guarded(c, [&] {
@@ -1546,7 +1599,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     }
     TDA_HIP(hipStreamSynchronize(c->stream));
     std::vector<double> h_evals, h_err;
-    int lp16_iters = 0, fp32_iters = 0;
+    int lp16_iters = 0, fp32_iters = 0, p32 = PREC_FP32;
     if (c->prec == PREC_FP64 || c->dense) {
       res = davidson(c, n, n, max_iter, tol, PREC_FP64, false, 0, h_evals, h_err);
       res64 = res;
@@ -1565,7 +1618,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       int start = n, kk = n;
       if (c->prec == PREC_FP16 || c->prec == PREC_BF16) {
         res = davidson(c, n, n, max_iter, tol, c->prec, false, c->stall_window, h_evals, h_err, c->lp_cap(n),
-                       c->lp_keep(n), &kk, 0.8, 1e300, true, c->floor_k);
+                       c->lp_keep(n), &kk, 0.8, 1e300, true, c->floor_k, true);
         used += res.iters;
         done = res.converged;
         if (kk > n) launch_copy(c->W.as<double>(), c->V.as<double>(), (int64_t)kk * L, c->stream);
@@ -1579,9 +1632,13 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       lp16_iters = used;
       if (!done) {
         int kk32 = n;
-        DavResult r32 = davidson(c, n, start, std::max(1, max_iter - used), tol, PREC_FP32, used > 0,
+        // the FP32-accuracy phase: FP16 hi / lo split products when the
+        // context runs FP16 / BF16 on the AO operator, else FP32
+        const bool ao_op = c->has_lc && c->op_kind == 0 && !c->ct;
+        p32 = (c->split32 && ao_op && (c->prec == PREC_FP16 || c->prec == PREC_BF16)) ? PREC_F16X : PREC_FP32;
+        DavResult r32 = davidson(c, n, start, std::max(1, max_iter - used), tol, p32, used > 0,
                                  c->stall_window, h_evals, h_err, 0, c->lp_keep(n), &kk32, 0.8, 1e300,
-                                 c->fp32_thick, c->floor_k);
+                                 c->fp32_thick, c->floor_k, true);
         used += r32.iters;
         fp32_iters = r32.iters;
         res.iters = used;
@@ -1589,7 +1646,8 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       }
       if (start64 > n) launch_copy(c->W.as<double>(), c->V.as<double>(), (int64_t)start64 * L, c->stream);
       else launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
-      res64 = davidson(c, n, start64, std::max(1, max_iter - used), tol, PREC_FP64, true, 0, h_evals, h_err);
+      res64 = davidson(c, n, start64, std::max(1, max_iter - used), tol, PREC_FP64, true, 0, h_evals, h_err, 0, 0,
+                       nullptr, 0.5, 1e300, false, 0.0, true);
       res.iters = used;
     }
     // outputs (tda/mod.rs:88-126)
@@ -1630,6 +1688,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       stats->n_products_fp64 = c->n_products_fp64;
       stats->n_iter_fp32 = fp32_iters;
       stats->n_products_fp32 = c->n_products_fp32;
+      stats->split_fp32 = (!fp64_only && fp32_iters > 0 && p32 == PREC_F16X) ? 1 : 0;
       stats->converged = res64.converged ? 1 : 0;
       stats->n_atom_blocks = c->n_blocks;
       stats->max_residual = res64.max_err;
