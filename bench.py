@@ -1,12 +1,16 @@
 """LC-TDA benchmark: states/s of the mixed-precision path vs plain FP64 on one MI355X.
 
-Workload: all-trans polyene C200H202 (402 atoms, 1002 AO, 501 occ x 501 virt,
-ov = 251,001), lowest 20 singlets, residual tolerance 1e-5.  One step = one
-full solve from device-resident orbitals: transition-charge build, operator
-set-up, Davidson (low-precision products, then the FP64 continuation) and the
-outputs (omega, X, q_trans, dipoles, f) back on the host.
+Workload (default): the (H2O)512 cluster (1536 atoms, 3072 AO, 2048 occ x
+1024 virt, ov = 2,097,152), lowest 16 singlets, residual tolerance 1e-5 --
+the mixed-precision configuration.  --system polyene200 --states 20: the
+all-trans polyene C200H202 (402 atoms, 1002 AO, 501 x 501, ov = 251,001).
+One step = one full solve from device-resident orbitals: transition-charge
+build, operator set-up, Davidson (FP16 products, then FP16 hi/lo split
+products of FP32 accuracy, then the FP64 verification) and the outputs
+(omega, X, q_trans, dipoles, f) back on the host.
 
     python bench.py [--steps K] [--warmup W] [--prec fp16|fp32|bf16|fp64]
+                    [--system water512|polyene200|...] [--states n]
                     [--operator ao|charges] [--workload fmo]
 
 --workload fmo: the batched FMO case instead -- 1024 seeded fragments (H2O,
@@ -111,25 +115,43 @@ def solve(sysg, n_states, max_iter, tol):
     return t, w, x, dict(sysg.last_stats), sysg.properties["oscillator_strengths"].copy()
 
 
-def cpu_baseline(inp, n_states, n_iter, n_products):
+def cpu_baseline(inp, n_states, n_iter, n_products, q_budget_bytes=2 << 30):
     """Time the oracle (reference-faithful FP64 numpy restatement) on a bounded
     sample: trans_charges, one gamma_lr.q_oo (recomputed per product call in
     the reference), the Coulomb and per-vector exchange for 2 vectors; then
-    scale to a solve with the FP64 GPU solve's iteration / product counts."""
+    scale to a solve with the FP64 GPU solve's iteration / product counts.
+
+    When the q arrays of the whole system exceed q_budget_bytes the sample
+    is the first N' atoms (same orbitals, o and v; needs S = I, which the
+    synthetic inputs have): trans_charges, the Coulomb and the exchange scale
+    as N, gamma_lr.q_oo as N^2, so the sample times are scaled by N/N' and
+    (N/N')^2."""
     from oracle import tda_oracle as orc
     try:
         from threadpoolctl import threadpool_info
         cores = max([d.get("num_threads", 1) for d in threadpool_info()] or [1])
     except Exception:
         cores = os.cpu_count() or 1
-    t0 = time.perf_counter()
-    q_ov, q_oo, q_vv = orc.trans_charges(inp["n_atoms"], inp["orbs_per_atom"], inp["orbs"], inp["s"],
-                                         inp["occ_indices"], inp["virt_indices"])
-    t_q = time.perf_counter() - t0
     o, v = len(inp["occ_indices"]), len(inp["virt_indices"])
+    n_at = inp["n_atoms"]
+    per_atom = 8.0 * (o * v + o * o + v * v)
+    ns = n_at if n_at * per_atom <= q_budget_bytes else max(1, int(q_budget_bytes // per_atom))
+    opa = np.asarray(inp["orbs_per_atom"])
+    if ns < n_at:
+        if not np.array_equal(inp["s"], np.eye(inp["s"].shape[0])):
+            raise SystemExit("cpu_baseline: an atom-sliced sample needs S = I")
+        nao_s = int(opa[:ns].sum())
+        orbs, s_mat = inp["orbs"][:nao_s], np.eye(nao_s)
+        gam, glr = inp["gamma"][:ns, :ns], inp["gamma_lr"][:ns, :ns]
+    else:
+        orbs, s_mat, gam, glr = inp["orbs"], inp["s"], inp["gamma"], inp["gamma_lr"]
+    r = n_at / ns
+    t0 = time.perf_counter()
+    q_ov, q_oo, q_vv = orc.trans_charges(ns, opa[:ns], orbs, s_mat, inp["occ_indices"], inp["virt_indices"])
+    t_q = time.perf_counter() - t0
     orbe = inp["orbe"]
     omega = orc.orbe_differences(orbe[:o], orbe[o:])
-    eng = orc.SystemEngine(q_ov, q_oo, q_vv, inp["gamma"], inp["gamma_lr"], omega, o, v, faithful=True)
+    eng = orc.SystemEngine(q_ov, q_oo, q_vv, gam, glr, omega, o, v, faithful=True)
     x = np.random.default_rng(0).standard_normal((o * v, 2))
     t0 = time.perf_counter()
     eng._gamma_oo()
@@ -138,13 +160,15 @@ def cpu_baseline(inp, n_states, n_iter, n_products):
     eng.apply(x)
     t_call2 = time.perf_counter() - t0
     t_vec = max(t_call2 - t_g, 0.0) / 2.0
-    t_solve = t_q + n_iter * t_g + n_products * t_vec
+    t_solve = r * t_q + n_iter * r * r * t_g + n_products * r * t_vec
+    what = "the whole system" if ns == n_at else f"the first {ns} of {n_at} atoms (scaled by N/N' = {r:.1f}, " \
+                                                  f"gamma_lr.q_oo by (N/N')^2)"
     return {
         "value": n_states / t_solve, "unit": "states/s", "cores": int(cores), "kind": "port",
         "sample": (f"numpy FP64 oracle (reference-faithful: gamma_lr.q_oo per call, per-vector exchange) on "
-                   f"the same inputs: trans_charges {t_q:.2f} s, gamma_lr.q_oo {t_g:.2f} s, "
-                   f"{t_vec:.2f} s per A.x vector; scaled to the FP64 GPU solve's {n_iter} iterations / "
-                   f"{n_products} products = {t_solve:.1f} s per solve"),
+                   f"{what}: trans_charges {t_q:.2f} s, gamma_lr.q_oo {t_g:.2f} s, {t_vec:.2f} s per A.x "
+                   f"vector; scaled to the FP64 GPU solve's {n_iter} iterations / {n_products} products = "
+                   f"{t_solve:.1f} s per solve"),
     }
 
 
@@ -291,9 +315,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--prec", default="fp16", choices=["fp64", "fp32", "bf16", "fp16"])
-    ap.add_argument("--system", default="polyene200")
+    ap.add_argument("--system", default="water512")
     ap.add_argument("--operator", default="ao", choices=["ao", "charges"])
-    ap.add_argument("--states", type=int, default=20)
+    ap.add_argument("--states", type=int, default=16)
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--max-iter", type=int, default=450)
     ap.add_argument("--seed", type=int, default=0)

@@ -222,7 +222,9 @@ class SystemEngine:
             k_x = np.zeros_like(two_el)
             if self.faithful:
                 for c in range(x.shape[1]):                               # :335-358
-                    xi = x[:, c].reshape(o, v)
+                    # the column as a contiguous o x v block (the reference's
+                    # to_owned().into_shape), so the product runs on BLAS
+                    xi = np.ascontiguousarray(x[:, c]).reshape(o, v)
                     t = (xi @ q_vv_r.T).reshape(o, n_b, v).transpose(1, 0, 2)
                     t = np.ascontiguousarray(t).reshape(n_b * o, v)
                     k_x[:, c] = (gamma_oo.T @ t).reshape(o * v)

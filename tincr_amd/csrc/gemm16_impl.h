@@ -62,7 +62,7 @@ struct alignas(8) Vec4 {
   T v[4];
 };
 
-template <typename TI, typename TO, typename Cfg, int EPI>
+template <typename TI, typename TO, typename Cfg, int EPI, bool SPLIT>
 __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs g) {
   using MF = Mfma16<TI>;
   using Frag = typename MF::Frag;
@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
   const TI* B = (const TI*)g.B + zb * g.b_batch;
   const int64_t K = g.K;
   const int nks = (int)((K + BK - 1) / BK);  // K slices per segment
-  const bool split_in = g.a_lo != 0;
+  constexpr bool split_in = SPLIT;  // a_lo, b_lo != 0 (checked on the host)
   const int nkt = split_in ? 3 * nks : nks;
   const uint32_t a_lo = (uint32_t)g.a_lo, b_lo = (uint32_t)g.b_lo;
 
@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
     // segment (split operands): 0 = A_hi B_lo, 1 = A_lo B_hi, 2 = A_hi B_hi
     int ks = kt;
     uint32_t ao = 0, bo = 0;
-    if (split_in) {
+    if constexpr (split_in) {
       const int seg = kt / nks;
       ks = kt - seg * nks;
       ao = seg == 1 ? a_lo : 0u;
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
   };
   // split operands: the correction segments are done once 2 nks slices are in
   auto rescale = [&](int kt) {
-    if (split_in && kt == 2 * nks) {
+    if (split_in && kt == 2 * nks) {  // compile-time false without SPLIT
 #pragma unroll
       for (int i = 0; i < MI; ++i)
 #pragma unroll
@@ -438,7 +438,14 @@ inline bool gemm16_big(const GemmArgs& g) {
 template <typename TI, typename TO, int EPI, typename Cfg>
 void launch16_cfg(GemmArgs& g, hipStream_t s) {
   dim3 grid((unsigned)ceil_div(g.N, Cfg::BN), (unsigned)ceil_div(g.M, Cfg::BM), (unsigned)g.batch);
-  gemm16_nt_kernel<TI, TO, Cfg, EPI><<<grid, Cfg::NT, 0, s>>>(g);
+  if constexpr (std::is_same<TI, _Float16>::value) {
+    if (g.a_lo) {
+      gemm16_nt_kernel<TI, TO, Cfg, EPI, true><<<grid, Cfg::NT, 0, s>>>(g);
+      TDA_LAUNCH_CHECK();
+      return;
+    }
+  }
+  gemm16_nt_kernel<TI, TO, Cfg, EPI, false><<<grid, Cfg::NT, 0, s>>>(g);
   TDA_LAUNCH_CHECK();
 }
 
