@@ -176,23 +176,53 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
 
   // fragment row (within the wave tile) and its swizzle key
   const int fr = lane & 15, fq = lane >> 4;
+  // One K slice: 2 steps of 32 x (MI x NI) MFMAs.  Fragment reads run one
+  // group ahead of the MFMAs that consume them (the next pair of A fragments,
+  // and the next step's B fragments, are read while the current pair is
+  // multiplied), so the LDS latency overlaps matrix-core work.
+  auto frag_a = [&](const TI* s, int ks, int i) {
+    const int r = wm0 + i * 16 + fr, kc = ks * 4 + fq;
+    return *(const Frag*)(s + r * BK + ((kc ^ (r & 7)) << 3));
+  };
+  auto frag_b = [&](const TI* s, int ks, int j) {
+    const int r = wn0 + j * 16 + fr, kc = ks * 4 + fq;
+    return *(const Frag*)(s + BM * BK + r * BK + ((kc ^ (r & 7)) << 3));
+  };
   auto compute = [&](int buf) {
     const TI* s = lds + buf * STAGE;
+    static_assert(MI % 2 == 0, "A fragments go in pairs");
+    Frag fb[NI], fa0 = frag_a(s, 0, 0), fa1 = frag_a(s, 0, 1);
+#pragma unroll
+    for (int j = 0; j < NI; ++j) fb[j] = frag_b(s, 0, j);
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
-      const int kc = ks * 4 + fq;  // K chunk of this lane's 8 elements
-      Frag fb[NI];
+      Frag nb[NI];
 #pragma unroll
-      for (int j = 0; j < NI; ++j) {
-        const int r = wn0 + j * 16 + fr;
-        fb[j] = *(const Frag*)(s + BM * BK + r * BK + ((kc ^ (r & 7)) << 3));
+      for (int ip = 0; ip < MI / 2; ++ip) {
+        Frag na0, na1;
+        if (ip + 1 < MI / 2) {
+          na0 = frag_a(s, ks, 2 * ip + 2);
+          na1 = frag_a(s, ks, 2 * ip + 3);
+        } else if (ks == 0) {
+          na0 = frag_a(s, 1, 0);
+          na1 = frag_a(s, 1, 1);
+#pragma unroll
+          for (int j = 0; j < NI; ++j) nb[j] = frag_b(s, 1, j);
+        }
+        // keep the reads above this group's MFMAs (the scheduler would
+        // otherwise sink them next to their uses)
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < NI; ++j) acc[2 * ip][j] = MF::op(fb[j], fa0, acc[2 * ip][j]);
+#pragma unroll
+        for (int j = 0; j < NI; ++j) acc[2 * ip + 1][j] = MF::op(fb[j], fa1, acc[2 * ip + 1][j]);
+        __builtin_amdgcn_sched_barrier(0);
+        fa0 = na0;
+        fa1 = na1;
       }
+      if (ks == 0) {
 #pragma unroll
-      for (int i = 0; i < MI; ++i) {
-        const int r = wm0 + i * 16 + fr;
-        const Frag fa = *(const Frag*)(s + r * BK + ((kc ^ (r & 7)) << 3));
-#pragma unroll
-        for (int j = 0; j < NI; ++j) acc[i][j] = MF::op(fb[j], fa, acc[i][j]);
+        for (int j = 0; j < NI; ++j) fb[j] = nb[j];
       }
     }
   };
@@ -451,8 +481,14 @@ void launch16_cfg(GemmArgs& g, hipStream_t s) {
 
 template <typename TI, typename TO, int EPI>
 void launch16_t(GemmArgs& g, hipStream_t s) {
-  if (gemm16_big(g)) launch16_cfg<TI, TO, EPI, Cfg16Big>(g, s);
-  else launch16_cfg<TI, TO, EPI, Cfg16Mid>(g, s);
+  // the FP64-output epilogue keeps the 128x128 tile (the 256x256 one spills)
+  if constexpr (EPI != EPI_DIAG) {
+    if (gemm16_big(g)) {
+      launch16_cfg<TI, TO, EPI, Cfg16Big>(g, s);
+      return;
+    }
+  }
+  launch16_cfg<TI, TO, EPI, Cfg16Mid>(g, s);
 }
 
 // Epilogue / output dispatch for one input type (gemm16_ok has been checked).

@@ -972,11 +972,25 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
         g1.beta = 1.0;
         run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, g1);
         // ... then against the accepted new vectors, in order, 1e-8 drop
+        std::vector<double> pn(nc), wn(nc);
+        if (reorth) {
+          launch_norms(P, nc, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
+          TDA_HIP(hipMemcpyAsync(pn.data(), c->norms.p, nc * 8, hipMemcpyDeviceToHost, c->stream));
+        }
         launch_gs_block(P, W, V + (size_t)m * L, L, nc, c->gs_scratch.as<double>(), d_count, c->stream);
+        if (reorth) {
+          launch_norms(W, nc, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
+          TDA_HIP(hipMemcpyAsync(wn.data(), c->norms.p, nc * 8, hipMemcpyDeviceToHost, c->stream));
+        }
         int h_count = 0;
         TDA_HIP(hipMemcpyAsync(&h_count, d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         TDA_HIP(hipStreamSynchronize(c->stream));
-        if (reorth && h_count > 0) {
+        // a vector that kept less than 1/sqrt(2) of its norm through the
+        // projections (Daniel, Gragg, Kaufman and Stewart's criterion) has
+        // lost orthogonality to rounding: then the block gets a second pass
+        bool need = false;
+        for (int j = 0; j < nc && reorth; ++j) need = need || (wn[j] > 1e-8 && wn[j] < 0.7071 * pn[j]);
+        if (need && h_count > 0) {
           // second Gram-Schmidt pass for the accepted vectors Q ("twice is
           // enough"): a single classical pass loses orthogonality when the
           // preconditioned residuals lie nearly in the basis, and a nearly
