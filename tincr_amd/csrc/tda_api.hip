@@ -110,7 +110,7 @@ struct tda_ctx {
   double pre_thr = 1e-4;
   // precision ladder: floor factor of the low-precision phases, stall window
   double floor_k = 2.0;
-  int stall_window = 10;
+  int stall_window = 0;  // 0: from the product cost (product_flops)
   bool fp32_thick = true;
   bool split32 = true;  // FP32-accuracy phase on FP16 hi / lo split products (AO operator)
   int64_t ovp = 0;
@@ -813,6 +813,23 @@ struct DavResult {
 // residuals no longer say how far the roots are from convergence.
 // reorth: a second Gram-Schmidt pass for the new basis vectors (the phases
 // of the precision ladder; the FP64 path keeps the reference's single pass).
+// Algorithmic flops of one single-vector product (AO form: the four GEMMs;
+// charges form: the three-stage exchange).
+double product_flops(const tda_ctx* c) {
+  const double o = c->o, v = c->v, N = c->N, n2 = 2.0 * c->nao;
+  if (c->has_lc && c->op_kind == 0 && !c->ct) return 2.0 * n2 * n2 * (o + v) + 4.0 * n2 * o * v;
+  return 2.0 * N * o * v * (o + v) + 4.0 * N * o * v;
+}
+
+// Stall window of the low-precision phases: cheap products (small systems,
+// where the subspace work dominates an iteration) hand over after 4
+// iterations without progress; expensive ones ride out longer plateaus,
+// since their iterations cost less than those of the next precision.
+int lp_stall_window(const tda_ctx* c) {
+  if (c->stall_window > 0) return c->stall_window;
+  return product_flops(c) >= 5e10 ? 10 : 4;
+}
+
 DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec, bool first_can_converge,
                    int stop_stall, std::vector<double>& h_evals, std::vector<double>& h_err, int cap_in = 0,
                    int keep = 0, int* kk_out = nullptr, double stall_factor = 0.5,
@@ -1631,7 +1648,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       bool done = false;
       int start = n, kk = n;
       if (c->prec == PREC_FP16 || c->prec == PREC_BF16) {
-        res = davidson(c, n, n, max_iter, tol, c->prec, false, c->stall_window, h_evals, h_err, c->lp_cap(n),
+        res = davidson(c, n, n, max_iter, tol, c->prec, false, lp_stall_window(c), h_evals, h_err, c->lp_cap(n),
                        c->lp_keep(n), &kk, 0.8, 1e300, true, c->floor_k, true);
         used += res.iters;
         done = res.converged;
@@ -1651,7 +1668,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
         const bool ao_op = c->has_lc && c->op_kind == 0 && !c->ct;
         p32 = (c->split32 && ao_op && (c->prec == PREC_FP16 || c->prec == PREC_BF16)) ? PREC_F16X : PREC_FP32;
         DavResult r32 = davidson(c, n, start, std::max(1, max_iter - used), tol, p32, used > 0,
-                                 c->stall_window, h_evals, h_err, 0, c->lp_keep(n), &kk32, 0.8, 1e300,
+                                 lp_stall_window(c), h_evals, h_err, 0, c->lp_keep(n), &kk32, 0.8, 1e300,
                                  c->fp32_thick, c->floor_k, true);
         used += r32.iters;
         fp32_iters = r32.iters;
