@@ -207,6 +207,24 @@ def test_run_tda_mixed_precision(prec, mol, op):
     slp.close()
 
 
+def test_small_system_mixed_context_runs_fp64(monkeypatch):
+    """A mixed-precision context on a system whose products cost less than
+    TINCR_TDA_LADDER_MIN_FLOPS runs the reference FP64 path: same omega as an
+    FP64 context to the last bits, no low-precision phase."""
+    m = molecule("uracil")
+    s64 = _system(m)
+    w64, x64 = s64.run_tda(4, 200, 1e-5)
+    monkeypatch.setenv("TINCR_TDA_LADDER_MIN_FLOPS", "1e12")
+    slp = _system(m, precision="fp16")
+    w, x = slp.run_tda(4, 200, 1e-5)
+    st = slp.last_stats
+    assert st["n_iter_fp64"] == st["n_iter"] == s64.last_stats["n_iter"]
+    assert st["n_products_fp64"] == st["n_products"] and st["split_fp32"] == 0
+    assert np.abs(w - w64).max() < 1e-14
+    s64.close()
+    slp.close()
+
+
 def test_exchange_atom_blocking():
     """A tiny HBM budget forces several atom blocks in the exchange."""
     from tincr_amd import systems as sy

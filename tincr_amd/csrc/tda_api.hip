@@ -113,6 +113,7 @@ struct tda_ctx {
   int stall_window = 0;  // 0: from the product cost (product_flops)
   bool fp32_thick = true;
   bool split32 = true;  // FP32-accuracy phase on FP16 hi / lo split products (AO operator)
+  double ladder_min_flops = 1e9;  // per-product flops below which mixed contexts run FP64
   int64_t ovp = 0;
   std::vector<double> xyz, h_de;
   DevBuf C, S, SC, orbe, aooff, gamma, gamma_lr;
@@ -821,10 +822,13 @@ double product_flops(const tda_ctx* c) {
   return 2.0 * N * o * v * (o + v) + 4.0 * N * o * v;
 }
 
-// Stall window of the low-precision phases: cheap products (small systems,
+// Stall window of the FP16 / BF16 phase: cheap products (small systems,
 // where the subspace work dominates an iteration) hand over after 4
 // iterations without progress; expensive ones ride out longer plateaus,
-// since their iterations cost less than those of the next precision.
+// since their iterations cost less than those of the next precision.  The
+// FP32-accuracy phase, whose accuracy floor lies far below the tolerance,
+// gets three times the window: handing a slow plateau to FP64 products costs
+// far more than riding it out.
 int lp_stall_window(const tda_ctx* c) {
   if (c->stall_window > 0) return c->stall_window;
   return product_flops(c) >= 5e10 ? 10 : 4;
@@ -1151,6 +1155,8 @@ int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx**
     if (et) c->fp32_thick = atoi(et) != 0;
     const char* ep = getenv("TINCR_TDA_SPLIT32");
     if (ep) c->split32 = atoi(ep) != 0;
+    const char* el = getenv("TINCR_TDA_LADDER_MIN_FLOPS");
+    if (el && atof(el) >= 0) c->ladder_min_flops = atof(el);
   }
   if (hbm_budget_bytes) c->budget = hbm_budget_bytes;
   int rc = guarded(c, [&] {
@@ -1613,7 +1619,11 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     const double tq = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tq0).count();
     const int n = n_roots;
     const int64_t L = c->ovp;
-    alloc_davidson(c, n, c->prec == PREC_FP64 || c->dense ? n : c->lp_keep(n));
+    // the reference FP64 path: FP64 contexts, dense engines, and systems whose
+    // products are too cheap for a precision ladder to pay off (the subspace
+    // work of the extra iterations costs more than the products save)
+    const bool fp64_only = c->prec == PREC_FP64 || c->dense || product_flops(c) < c->ladder_min_flops;
+    alloc_davidson(c, n, fp64_only ? n : c->lp_keep(n));
     // initial subspace: unit vectors at the n smallest omega (solvers/utils.rs:17-26)
     std::vector<int64_t> order(ov);
     std::iota(order.begin(), order.end(), 0);
@@ -1631,7 +1641,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     TDA_HIP(hipStreamSynchronize(c->stream));
     std::vector<double> h_evals, h_err;
     int lp16_iters = 0, fp32_iters = 0, p32 = PREC_FP32;
-    if (c->prec == PREC_FP64 || c->dense) {
+    if (fp64_only) {
       res = davidson(c, n, n, max_iter, tol, PREC_FP64, false, 0, h_evals, h_err);
       res64 = res;
     } else {
@@ -1668,7 +1678,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
         const bool ao_op = c->has_lc && c->op_kind == 0 && !c->ct;
         p32 = (c->split32 && ao_op && (c->prec == PREC_FP16 || c->prec == PREC_BF16)) ? PREC_F16X : PREC_FP32;
         DavResult r32 = davidson(c, n, start, std::max(1, max_iter - used), tol, p32, used > 0,
-                                 lp_stall_window(c), h_evals, h_err, 0, c->lp_keep(n), &kk32, 0.8, 1e300,
+                                 3 * lp_stall_window(c), h_evals, h_err, 0, c->lp_keep(n), &kk32, 0.8, 1e300,
                                  c->fp32_thick, c->floor_k, true);
         used += r32.iters;
         fp32_iters = r32.iters;
@@ -1684,7 +1694,6 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     // outputs (tda/mod.rs:88-126)
     const int N = c->N;
     c->last_roots = n;
-    const bool fp64_only = c->prec == PREC_FP64 || c->dense;
     if (omega) std::memcpy(omega, h_evals.data(), (size_t)n * 8);
     if (X) {
       c->stage.ensure((size_t)ov * n * 8);
