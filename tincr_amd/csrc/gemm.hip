@@ -1,4 +1,9 @@
 // MFMA GEMM: planning, dispatch and the split-K reduction.
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "gemm16_impl.h"
 
 #ifndef TDA_BIG_K
@@ -60,6 +65,63 @@ static void dispatch_reduce(DType in, DType out, GemmArgs& g, hipStream_t s) {
   }
 }
 
+// Tile autotuner of the 16-bit kernel.  The first launch of each problem
+// (input type, epilogue, split, M, N, K, batch, output strides) times the
+// 256x256 and the 128x128 tile on the caller's stream (one warm-up and one
+// timed launch each; the launches only write C, so repeating them is
+// harmless, and every tile sums each output element in the same K order,
+// so the choice does not change results) and keeps the faster one for the
+// process.  TINCR_TDA_AUTOTUNE=0: the tile-count rule alone.
+namespace {
+using Tune16Key = std::tuple<int, int, bool, int64_t, int64_t, int64_t, int, int64_t, int64_t>;
+std::mutex tune16_mu;
+std::map<Tune16Key, int> tune16_cache;
+
+bool autotune16_on() {
+  static const bool on = [] {
+    const char* e = getenv("TINCR_TDA_AUTOTUNE");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+void dispatch16(DType in, Epi epi, GemmArgs& g, hipStream_t s, int tile) {
+  if (in == DT_F16) gemm16_dispatch_f16(epi, g, s, tile);
+  else gemm16_dispatch_bf16(epi, g, s, tile);
+}
+
+int tune16(DType in, Epi epi, GemmArgs& g, hipStream_t s) {
+  if (epi == EPI_DIAG || !autotune16_on()) return -1;
+  const Tune16Key key{(int)in, (int)epi, g.a_lo != 0, g.M, g.N, g.K, g.batch, g.cm.s0, g.c_lo};
+  {
+    std::lock_guard<std::mutex> lk(tune16_mu);
+    auto it = tune16_cache.find(key);
+    if (it != tune16_cache.end()) return it->second;
+  }
+  hipEvent_t ev[4];
+  for (auto& e : ev) TDA_HIP(hipEventCreate(&e));
+  float ms[2] = {0.f, 0.f};
+  try {
+    for (int tile = 0; tile < 2; ++tile) {
+      dispatch16(in, epi, g, s, tile);  // warm-up (code object load, caches)
+      TDA_HIP(hipEventRecord(ev[2 * tile], s));
+      dispatch16(in, epi, g, s, tile);
+      TDA_HIP(hipEventRecord(ev[2 * tile + 1], s));
+    }
+    TDA_HIP(hipEventSynchronize(ev[3]));
+    for (int tile = 0; tile < 2; ++tile) TDA_HIP(hipEventElapsedTime(&ms[tile], ev[2 * tile], ev[2 * tile + 1]));
+  } catch (...) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  const int choice = ms[1] < ms[0] ? 1 : 0;
+  std::lock_guard<std::mutex> lk(tune16_mu);
+  tune16_cache[key] = choice;
+  return choice;
+}
+}  // namespace
+
 void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g, const GemmPlan& plan,
                  hipStream_t stream, hipEvent_t after_main) {
   TDA_CHECK(g.M >= 0 && g.N >= 0 && g.K >= 0, 2, "gemm: negative extent");
@@ -69,8 +131,7 @@ void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g,
   if (gemm16_ok(in, out, epi, a_kc, b_kc, g, 1)) {
     g.ksplit = 1;
     g.k_chunk = g.K;
-    if (in == DT_F16) gemm16_dispatch_f16(epi, g, stream);
-    else gemm16_dispatch_bf16(epi, g, stream);
+    dispatch16(in, epi, g, stream, tune16(in, epi, g, stream));
     if (after_main) TDA_HIP(hipEventRecord(after_main, stream));
     return;
   }

@@ -479,11 +479,12 @@ void launch16_cfg(GemmArgs& g, hipStream_t s) {
   TDA_LAUNCH_CHECK();
 }
 
+// tile: 1 = 256x256, 0 = 128x128, -1 = by tile count (gemm16_big).  The
+// FP64-output epilogue always takes the 128x128 tile (the 256x256 one spills).
 template <typename TI, typename TO, int EPI>
-void launch16_t(GemmArgs& g, hipStream_t s) {
-  // the FP64-output epilogue keeps the 128x128 tile (the 256x256 one spills)
+void launch16_t(GemmArgs& g, hipStream_t s, int tile) {
   if constexpr (EPI != EPI_DIAG) {
-    if (gemm16_big(g)) {
+    if (tile > 0 || (tile < 0 && gemm16_big(g))) {
       launch16_cfg<TI, TO, EPI, Cfg16Big>(g, s);
       return;
     }
@@ -493,10 +494,10 @@ void launch16_t(GemmArgs& g, hipStream_t s) {
 
 // Epilogue / output dispatch for one input type (gemm16_ok has been checked).
 template <typename TI>
-void launch16_in(Epi epi, GemmArgs& g, hipStream_t s) {
-  if (epi == EPI_HADAMARD) launch16_t<TI, TI, EPI_HADAMARD>(g, s);
-  else if (epi == EPI_DIAG) launch16_t<TI, double, EPI_DIAG>(g, s);
-  else launch16_t<TI, TI, EPI_STORE>(g, s);
+void launch16_in(Epi epi, GemmArgs& g, hipStream_t s, int tile) {
+  if (epi == EPI_HADAMARD) launch16_t<TI, TI, EPI_HADAMARD>(g, s, tile);
+  else if (epi == EPI_DIAG) launch16_t<TI, double, EPI_DIAG>(g, s, tile);
+  else launch16_t<TI, TI, EPI_STORE>(g, s, tile);
 }
 
 }  // namespace tda

@@ -17,6 +17,6 @@ void gemm_dispatch_bf16(int epi, DType out, int cfg, bool a_kc, bool b_kc, GemmA
   }
 }
 
-void gemm16_dispatch_bf16(Epi epi, GemmArgs& g, hipStream_t s) { launch16_in<__bf16>(epi, g, s); }
+void gemm16_dispatch_bf16(Epi epi, GemmArgs& g, hipStream_t s, int tile) { launch16_in<__bf16>(epi, g, s, tile); }
 
 }  // namespace tda

@@ -17,6 +17,6 @@ void gemm_dispatch_f16(int epi, DType out, int cfg, bool a_kc, bool b_kc, GemmAr
   }
 }
 
-void gemm16_dispatch_f16(Epi epi, GemmArgs& g, hipStream_t s) { launch16_in<_Float16>(epi, g, s); }
+void gemm16_dispatch_f16(Epi epi, GemmArgs& g, hipStream_t s, int tile) { launch16_in<_Float16>(epi, g, s, tile); }
 
 }  // namespace tda

@@ -580,8 +580,9 @@ void gemm_dispatch_f64(int epi, DType out, int cfg, bool a_kc, bool b_kc, GemmAr
 void gemm_dispatch_f32(int epi, DType out, int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s);
 void gemm_dispatch_f16(int epi, DType out, int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s);
 void gemm_dispatch_bf16(int epi, DType out, int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s);
-// The K-contiguous 16-bit kernel (gemm16_impl.h), for arguments that pass gemm16_ok.
-void gemm16_dispatch_f16(Epi epi, GemmArgs& g, hipStream_t s);
-void gemm16_dispatch_bf16(Epi epi, GemmArgs& g, hipStream_t s);
+// The K-contiguous 16-bit kernel (gemm16_impl.h), for arguments that pass
+// gemm16_ok; tile: 1 = 256x256, 0 = 128x128, -1 = by tile count.
+void gemm16_dispatch_f16(Epi epi, GemmArgs& g, hipStream_t s, int tile);
+void gemm16_dispatch_bf16(Epi epi, GemmArgs& g, hipStream_t s, int tile);
 
 }  // namespace tda
