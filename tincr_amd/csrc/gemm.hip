@@ -71,7 +71,9 @@ static void dispatch_reduce(DType in, DType out, GemmArgs& g, hipStream_t s) {
 // timed launch each; the launches only write C, so repeating them is
 // harmless, and every tile sums each output element in the same K order,
 // so the choice does not change results) and keeps the faster one for the
-// process.  TINCR_TDA_AUTOTUNE=0: the tile-count rule alone.
+// process.  Problems under 1e10 MFMA flops (a few microseconds; the many
+// distinct shapes of small fragments) keep the tile-count rule, as does
+// TINCR_TDA_AUTOTUNE=0.
 namespace {
 using Tune16Key = std::tuple<int, int, bool, int64_t, int64_t, int64_t, int, int64_t, int64_t>;
 std::mutex tune16_mu;
@@ -92,6 +94,8 @@ void dispatch16(DType in, Epi epi, GemmArgs& g, hipStream_t s, int tile) {
 
 int tune16(DType in, Epi epi, GemmArgs& g, hipStream_t s) {
   if (epi == EPI_DIAG || !autotune16_on()) return -1;
+  const double flops = 2.0 * (double)g.M * (double)g.N * (double)g.K * g.batch * (g.a_lo ? 3.0 : 1.0);
+  if (flops < 1e10) return -1;
   const Tune16Key key{(int)in, (int)epi, g.a_lo != 0, g.M, g.N, g.K, g.batch, g.cm.s0, g.c_lo};
   {
     std::lock_guard<std::mutex> lk(tune16_mu);
