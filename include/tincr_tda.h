@@ -151,6 +151,17 @@ int tda_get_charges(tda_ctx* ctx, double* q_ov, double* q_oo, double* q_vv, doub
  * columns of x [ov, k], in the context's precision. */
 int tda_compute_products(tda_ctx* ctx, const double* x, int k, double* ax);
 
+/* Z-vector solve of the excited-state gradient, tda_zvector_lc /
+ * tda_zvector_no_lc (src/gradients/helpers.rs:1130-1223, :1053-1128):
+ * conjugate gradients on A z = rhs for each of the k columns of rhs [ov, k]
+ * (layout of X), with A the operator of mult_av_lc (helpers.rs:1384-1437:
+ * the forward singlet A.x, omega + 2 J - K) or, without gamma_lr,
+ * mult_av_nolc (:1351-1382: omega + 4 J), in FP64.  Start z = rhs / omega;
+ * a column stops when r.r <= conv (1e-19 in the reference) or after max_iter
+ * iterations (1e4 in the reference).  z [ov, k]; n_iter [k] (optional):
+ * iterations per column (max_iter if it did not converge). */
+int tda_zvector(tda_ctx* ctx, const double* rhs, int k, int max_iter, double conv, double* z, int32_t* n_iter);
+
 /* DavidsonEngine::precondition (solvers/mod.rs:373-380):
  * out = r / (w - omega), entries with |w - omega| < 1e-4 divided by 1. */
 int tda_precondition(tda_ctx* ctx, const double* r, double w, double* out);

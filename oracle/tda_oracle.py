@@ -22,6 +22,9 @@ It restates, in FP64 numpy, the reference's algorithm (tincr, Rust):
 * ``ct_pair_engine``       src/excited_states/tda/ct_pair.rs:12-76 +
   solvers/mod.rs:165-274
 * ``frozen_charge_gradient`` the gamma terms of d omega / dR
+* ``zvector``              src/gradients/helpers.rs:1130-1223 (LC) and
+  :1053-1128 (no LC): conjugate gradients on A z = rhs with the operator of
+  mult_av_lc :1384-1437 / mult_av_nolc :1351-1382 (``zvector_operator``)
 * ``hf_derivatives``       d omega / d(gamma, gamma_lr, q) by Hellmann-Feynman
   on the dense A of tda/mod.rs:180-215 (no reference function; the gamma and
   charge terms of src/gradients/excited_state.rs:259-487 at frozen orbitals)
@@ -410,6 +413,52 @@ def hf_derivatives(engine, x):
         g = (engine.gamma_lr @ engine.q_oo).reshape(-1, o, o)   # G^B = sum_A gamma_lr[B, A] q_oo^A
         out["d_qvv"] = -np.einsum("ia,Bij,jb->Bab", xm, g, xm).reshape(-1, v * v)
     return out
+
+
+def zvector_operator(engine):
+    """x -> A x of the Z-vector equation: mult_av_lc (helpers.rs:1384-1437,
+    de + 2 J - K, the forward product) with LC; mult_av_nolc (:1351-1382,
+    de + 4 J, Coulomb factor 4 at :1377) without."""
+    if engine.gamma_lr is not None:
+        return engine.apply
+
+    def op(x):
+        x = np.asarray(x, dtype=np.float64)
+        return engine.apply(x) + 2.0 * engine.q_ov.T @ (engine.gamma @ (engine.q_ov @ x))
+    return op
+
+
+def zvector(engine, rhs, max_iter=10000, conv=1.0e-19):
+    """tda_zvector_lc / tda_zvector_no_lc (helpers.rs:1130-1223, :1053-1128)
+    for each column of rhs [ov, k]: z0 = rhs / omega (:1154-1155),
+    r0 = rhs - A z0, p0 = r0; CG until r.r <= conv (:1212-1216).
+    Returns z [ov, k] and the iteration count per column."""
+    op = zvector_operator(engine)
+    rhs = np.asarray(rhs, dtype=np.float64)
+    if rhs.ndim == 1:
+        rhs = rhs[:, None]
+    z = np.zeros_like(rhs)
+    its = []
+    for c in range(rhs.shape[1]):
+        b = rhs[:, c]
+        zc = b / engine.omega
+        r = b - op(zc[:, None])[:, 0]
+        p = r.copy()
+        n_it = max_iter
+        for it in range(max_iter):
+            ap = op(p[:, None])[:, 0]
+            t1 = r @ r
+            t2 = p @ ap
+            zc = zc + (t1 / t2) * p
+            r = r - (t1 / t2) * ap
+            t2 = r @ r
+            if t2 <= conv:
+                n_it = it + 1
+                break
+            p = (t2 / t1) * p + r
+        z[:, c] = zc
+        its.append(n_it)
+    return z, its
 
 
 def frozen_charge_gradient(engine, x, dgamma, dgamma_lr):

@@ -225,6 +225,32 @@ def test_small_system_mixed_context_runs_fp64(monkeypatch):
     slp.close()
 
 
+@pytest.mark.parametrize("mol", ["h2o", "benzene", "uracil"])
+@pytest.mark.parametrize("lc,op", [(True, "ao"), (True, "charges"), (False, "charges")])
+def test_zvector_vs_oracle(mol, lc, op):
+    """tda_zvector (CG on the forward product, helpers.rs:1130-1223 /
+    :1053-1128) against the oracle's CG and a dense solve (without gamma_lr
+    the operator is the charges form)."""
+    m = molecule(mol)
+    print(f"zvector {mol} lc={lc} op={op}", flush=True)
+    eng = _oracle_engine(m, lc=lc)
+    n = eng.get_size()
+    rhs = np.random.default_rng(n + 1).standard_normal((n, 3))
+    z_ref, its_ref = orc.zvector(eng, rhs)
+    s = _system(m, lc=lc, operator=op)
+    z, its = s.zvector(rhs)
+    assert np.abs(z - z_ref).max() < 1e-9 * np.abs(z_ref).max()
+    a = eng.dense()
+    if not lc:
+        a = a + 2.0 * eng.q_ov.T @ eng.gamma @ eng.q_ov
+    assert np.linalg.norm(a @ z - rhs, axis=0).max() < 1e-9
+    assert np.all(np.abs(np.asarray(its) - np.asarray(its_ref)) <= 2)
+    # a single column and a capped iteration count
+    z1, its1 = s.zvector(rhs[:, 0], max_iter=3)
+    assert its1[0] == 3 and np.isfinite(z1).all()
+    s.close()
+
+
 def test_exchange_atom_blocking():
     """A tiny HBM budget forces several atom blocks in the exchange."""
     from tincr_amd import systems as sy

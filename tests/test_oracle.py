@@ -314,3 +314,27 @@ def test_frozen_charge_gradient_vs_finite_differences():
                 xyz[c // 3, c % 3] += sgn * h
                 vals.append(np.linalg.eigh(engine(xyz).dense())[0][k])
             assert abs((vals[0] - vals[1]) / (2 * h) - g[c]) < 1e-9
+
+
+@pytest.mark.parametrize("mol", MOLECULES)
+@pytest.mark.parametrize("lc", [True, False])
+def test_zvector_solves_the_operator_equation(mol, lc):
+    """zvector (helpers.rs:1130-1223 / :1053-1128) against a dense solve of
+    the same operator: de + 2 J - K with LC, de + 4 J without (the factor
+    of mult_av_nolc, helpers.rs:1377).  A dropped Coulomb factor or the
+    wrong operator misses the dense solution by far."""
+    m = molecule(mol)
+    eng = _engine(m, lc=lc)
+    n = eng.get_size()
+    a = eng.dense()
+    if not lc:
+        a = a + 2.0 * eng.q_ov.T @ eng.gamma @ eng.q_ov
+    rhs = np.random.default_rng(n).standard_normal((n, 2))
+    z, its = orc.zvector(eng, rhs)
+    ref = np.linalg.solve(a, rhs)
+    assert np.abs(z - ref).max() < 1e-8 * np.abs(ref).max()
+    assert np.linalg.norm(a @ z - rhs, axis=0).max() < 1e-9
+    assert all(0 < i < 10000 for i in its)
+    if not lc:
+        # the forward A (factor 2) is a different equation
+        assert np.abs(np.linalg.solve(eng.dense(), rhs) - ref).max() > 1e-3 * np.abs(ref).max()

@@ -47,6 +47,15 @@ void launch_precondition_many(const double* R, const double* u, const int* idx, 
 void launch_residual(double* AVY, const double* RZ, const double* u, int n, int64_t L, hipStream_t s);
 // norms[r] = || X[r] ||  (deterministic two-pass), partial: >= n*256 doubles
 void launch_norms(const double* X, int n, int64_t L, double* partial, double* norms, hipStream_t s);
+// out[c] = X[c] . Y[c] for n vectors of length L (partial: n * 256 doubles)
+void launch_dots(const double* X, const double* Y, int n, int64_t L, double* partial, double* out, hipStream_t s);
+// conjugate gradients of the Z-vector solve (helpers.rs:1130-1223), per column c:
+// Z = R / de (de == 0 -> 0); Z += a_c P, R -= a_c AP (a_c == 0: skipped);
+// P = b_c P + R (b_c < 0: skipped)
+void launch_cg_init(const double* R, const double* de, int n, int64_t L, double* Z, hipStream_t s);
+void launch_cg_step(double* Z, double* R, const double* P, const double* AP, const double* a, int n, int64_t L,
+                    hipStream_t s);
+void launch_cg_dir(double* P, const double* R, const double* b, int n, int64_t L, hipStream_t s);
 // Gram-Schmidt of nc new vectors among themselves (davidson.rs:162-168),
 // after their projection against the old basis: W[j] (= P[j] - V V^T P[j])
 // becomes u_j = W[j] - sum_{s<j accepted} q_s (q_s . P[j]); u_j is accepted

@@ -364,6 +364,89 @@ void launch_norms(const double* X, int n, int64_t L, double* partial, double* no
   TDA_LAUNCH_CHECK();
 }
 
+// Column dot products out[c] = X[c] . Y[c] (same two-pass order as the norms).
+__global__ void __launch_bounds__(256) dot_partial_kernel(const double* __restrict__ X, const double* __restrict__ Y,
+                                                          int64_t L, double* __restrict__ partial) {
+  const int r = blockIdx.y;
+  const double* x = X + (int64_t)r * L;
+  const double* y = Y + (int64_t)r * L;
+  double acc = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256) acc += x[t] * y[t];
+  acc = wave_sum(acc);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[(int64_t)r * NB_NORM + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void __launch_bounds__(256) dot_final_kernel(const double* __restrict__ partial, double* __restrict__ out) {
+  const int r = blockIdx.x;
+  double acc = partial[(int64_t)r * NB_NORM + threadIdx.x];
+  acc = wave_sum(acc);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[r] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+void launch_dots(const double* X, const double* Y, int n, int64_t L, double* partial, double* out, hipStream_t s) {
+  if (n == 0) return;
+  dot_partial_kernel<<<dim3(NB_NORM, n), 256, 0, s>>>(X, Y, L, partial);
+  TDA_LAUNCH_CHECK();
+  dot_final_kernel<<<n, 256, 0, s>>>(partial, out);
+  TDA_LAUNCH_CHECK();
+}
+
+// Conjugate-gradient pieces of the Z-vector solve (per column c):
+//   init:  Z = R / de (entries with de == 0, the row pads, stay 0)
+//   step:  Z += a_c P, R -= a_c AP
+//   dir:   P = b_c P + R          (columns with b_c < 0 are left alone)
+__global__ void __launch_bounds__(256) cg_init_kernel(const double* __restrict__ R, const double* __restrict__ de,
+                                                      int64_t L, double* __restrict__ Z) {
+  const int c = blockIdx.y;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256)
+    Z[(int64_t)c * L + t] = de[t] != 0.0 ? R[(int64_t)c * L + t] / de[t] : 0.0;
+}
+__global__ void __launch_bounds__(256) cg_step_kernel(double* __restrict__ Z, double* __restrict__ R,
+                                                      const double* __restrict__ P, const double* __restrict__ AP,
+                                                      const double* __restrict__ a, int64_t L) {
+  const int c = blockIdx.y;
+  const double ac = a[c];
+  if (ac == 0.0) return;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256) {
+    const int64_t i = (int64_t)c * L + t;
+    Z[i] += ac * P[i];
+    R[i] -= ac * AP[i];
+  }
+}
+__global__ void __launch_bounds__(256) cg_dir_kernel(double* __restrict__ P, const double* __restrict__ R,
+                                                     const double* __restrict__ b, int64_t L) {
+  const int c = blockIdx.y;
+  const double bc = b[c];
+  if (bc < 0.0) return;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < L; t += (int64_t)gridDim.x * 256) {
+    const int64_t i = (int64_t)c * L + t;
+    P[i] = bc * P[i] + R[i];
+  }
+}
+
+void launch_cg_init(const double* R, const double* de, int n, int64_t L, double* Z, hipStream_t s) {
+  if (n == 0) return;
+  cg_init_kernel<<<dim3((unsigned)grid_for(L, 256, 4096), (unsigned)n), 256, 0, s>>>(R, de, L, Z);
+  TDA_LAUNCH_CHECK();
+}
+void launch_cg_step(double* Z, double* R, const double* P, const double* AP, const double* a, int n, int64_t L,
+                    hipStream_t s) {
+  if (n == 0) return;
+  cg_step_kernel<<<dim3((unsigned)grid_for(L, 256, 4096), (unsigned)n), 256, 0, s>>>(Z, R, P, AP, a, L);
+  TDA_LAUNCH_CHECK();
+}
+void launch_cg_dir(double* P, const double* R, const double* b, int n, int64_t L, hipStream_t s) {
+  if (n == 0) return;
+  cg_dir_kernel<<<dim3((unsigned)grid_for(L, 256, 4096), (unsigned)n), 256, 0, s>>>(P, R, b, L);
+  TDA_LAUNCH_CHECK();
+}
+
 // ---- Gram-Schmidt of a block of new vectors among themselves ----
 // After the projection against the old basis (W = P - V V^T P), vector j is
 //   u_j = w_j - sum_{s < j accepted} q_s (q_s . p_j),  q_s = u_s / |u_s|,

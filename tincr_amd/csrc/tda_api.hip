@@ -99,7 +99,8 @@ struct tda_ctx {
   bool dense = false;
   DevBuf Hd, dxyz, post;
   DevBuf adjP, adjQ, adjX, adjD;  // adjoint work buffers
-  DevBuf one;                     // a device 1.0
+  DevBuf one;                     // device 1.0 values
+  DevBuf zvw;                     // Z-vector work (R, Z, P, AP)
   int last_roots = 0;             // CI vectors of the last tda_run held in RZ
   // thick restart of the FP16 / BF16 Davidson phase, in multiples of n_roots
   // (environment TINCR_TDA_LP_CAP / TINCR_TDA_LP_KEEP); the FP32 and FP64
@@ -108,6 +109,9 @@ struct tda_ctx {
   int lp_keep(int n) const { return std::min(64, std::max(n, lp_keep_mult * n)); }
   int lp_cap(int n) const { return std::min(15 * n, std::max(lp_cap_mult * n, lp_keep(n) + n)); }
   double pre_thr = 1e-4;
+  // Coulomb factor of the charges-form product: 2 (solvers/mod.rs:311); the
+  // no-LC Z-vector operator uses 4 (gradients/helpers.rs:1377)
+  double coul_factor = 2.0;
   // precision ladder: floor factor of the low-precision phases, stall window
   double floor_k = 2.0;
   int stall_window = 0;  // 0: from the product cost (product_flops)
@@ -162,7 +166,7 @@ struct tda_ctx {
                       &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &P, &W, &norms, &npart, &idx, &count,
                       &gs_scratch, &eigw, &ao_atom, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
                       &Cl, &Sl, &SCl, &Shl, &aooff_l, &ctW, &ctZ, &Hd, &dxyz, &post,
-                      &adjP, &adjQ, &adjX, &adjD, &one})
+                      &adjP, &adjQ, &adjX, &adjD, &one, &zvw})
       b->release();
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -468,7 +472,7 @@ void apply_A(tda_ctx* c, const double* X, int k, double* AX, int prec) {
       g = gargs(c->vv32.p, c->qov32.p, AX, k, ovp, N, plain(1), plain(k), plain(ovp), plain(1), plain(ovp),
                 plain(1));
     }
-    g.alpha = 2.0;  // singlet Coulomb factor (solvers/mod.rs:311)
+    g.alpha = c->coul_factor;  // singlet Coulomb factor 2 (solvers/mod.rs:311)
     g.diag = c->de.as<double>();
     g.dmap = plain(1);
     g.xsrc = X;
@@ -1496,6 +1500,86 @@ int tda_compute_products(tda_ctx* c, const double* x, int k, double* ax) {
     TDA_HIP(hipMemcpyAsync(ax, c->stage.p, (size_t)ov * k * 8, hipMemcpyDeviceToHost, c->stream));
     TDA_HIP(hipStreamSynchronize(c->stream));
     c->harvest_events();
+  });
+}
+
+// Z-vector solve of the excited-state gradient (gradients/helpers.rs:
+// 1130-1223 with LC, :1053-1128 without): conjugate gradients on
+// A z = rhs for each column, A the singlet product of mult_av_lc
+// (helpers.rs:1384-1437, the forward A.x: de + 2 J - K) or, without LC,
+// mult_av_nolc (:1351-1382: de + 4 J); start z = rhs / de, stop when
+// r.r <= conv (1e-19 there) or after max_iter iterations (1e4 there).
+int tda_zvector(tda_ctx* c, const double* rhs, int k, int max_iter, double conv, double* z, int32_t* n_iter) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] {
+    TDA_CHECK(c->has_system && !c->dense, TDA_ERR_STATE, "no TDA system loaded");
+    TDA_CHECK(k >= 0 && (k == 0 || (rhs && z)) && max_iter >= 0, TDA_ERR_BAD_ARG, "bad rhs/z/k/max_iter");
+    prepare(c);
+    if (k == 0) return;
+    const int64_t L = c->ovp, ov = (int64_t)c->o * c->v;
+    c->stage.ensure((size_t)ov * k * 8);
+    c->zvw.ensure((size_t)4 * k * L * 8);
+    c->npart.ensure((size_t)k * 256 * 8);
+    c->one.ensure((size_t)3 * k * 8);
+    double* dR = c->zvw.as<double>();
+    double* dZ = dR + (size_t)k * L;
+    double* dP = dZ + (size_t)k * L;
+    double* dAP = dP + (size_t)k * L;
+    double* dots = c->one.as<double>();  // [k] scalars of one step
+    double* a_d = dots + k;
+    double* b_d = a_d + k;
+    struct Coul {  // the no-LC Z-vector operator has Coulomb factor 4
+      tda_ctx* c;
+      double keep;
+      ~Coul() { c->coul_factor = keep; }
+    } coul{c, c->coul_factor};
+    if (!c->has_lc) c->coul_factor = 4.0;
+    TDA_HIP(hipMemcpyAsync(c->stage.p, rhs, (size_t)ov * k * 8, hipMemcpyHostToDevice, c->stream));
+    launch_pack_x(c->stage.as<double>(), c->o, c->v, c->vp, k, dR, c->stream);
+    // z0 = rhs / de (:1154-1155); r0 = rhs - A z0; p0 = r0 (:1168-1186)
+    launch_cg_init(dR, c->de.as<double>(), k, L, dZ, c->stream);
+    products(c, dZ, k, dAP, PREC_FP64);
+    launch_fill(a_d, 1.0, k, c->stream);
+    launch_residual(dR, dAP, a_d, k, L, c->stream);
+    launch_copy(dR, dP, (int64_t)k * L, c->stream);
+    std::vector<double> rr(k), pap(k), rr_new(k), a(k), b(k);
+    std::vector<int32_t> its(k, max_iter);
+    std::vector<bool> active(k, true);
+    launch_dots(dR, dR, k, L, c->npart.as<double>(), dots, c->stream);
+    TDA_HIP(hipMemcpyAsync(rr.data(), dots, k * 8, hipMemcpyDeviceToHost, c->stream));
+    TDA_HIP(hipStreamSynchronize(c->stream));
+    int left = k;
+    for (int it = 0; it < max_iter && left > 0; ++it) {
+      products(c, dP, k, dAP, PREC_FP64);  // :1189-1203
+      launch_dots(dP, dAP, k, L, c->npart.as<double>(), dots, c->stream);
+      TDA_HIP(hipMemcpyAsync(pap.data(), dots, k * 8, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipStreamSynchronize(c->stream));
+      for (int j = 0; j < k; ++j) a[j] = active[j] ? rr[j] / pap[j] : 0.0;  // :1206-1210
+      TDA_HIP(hipMemcpyAsync(a_d, a.data(), k * 8, hipMemcpyHostToDevice, c->stream));
+      launch_cg_step(dZ, dR, dP, dAP, a_d, k, L, c->stream);
+      launch_dots(dR, dR, k, L, c->npart.as<double>(), dots, c->stream);
+      TDA_HIP(hipMemcpyAsync(rr_new.data(), dots, k * 8, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipStreamSynchronize(c->stream));
+      for (int j = 0; j < k; ++j) {
+        b[j] = -1.0;
+        if (!active[j]) continue;
+        if (rr_new[j] <= conv) {  // :1212-1216
+          active[j] = false;
+          its[j] = it + 1;
+          --left;
+          continue;
+        }
+        b[j] = rr_new[j] / rr[j];  // :1217
+        rr[j] = rr_new[j];
+      }
+      TDA_HIP(hipMemcpyAsync(b_d, b.data(), k * 8, hipMemcpyHostToDevice, c->stream));
+      launch_cg_dir(dP, dR, b_d, k, L, c->stream);
+    }
+    launch_unpack_x(dZ, c->o, c->v, c->vp, k, c->stage.as<double>(), c->stream);
+    TDA_HIP(hipMemcpyAsync(z, c->stage.p, (size_t)ov * k * 8, hipMemcpyDeviceToHost, c->stream));
+    TDA_HIP(hipStreamSynchronize(c->stream));
+    c->harvest_events();
+    if (n_iter) std::memcpy(n_iter, its.data(), (size_t)k * sizeof(int32_t));
   });
 }
 

@@ -18,6 +18,8 @@ Names, argument meaning and error behaviour follow the reference:
 
 All arithmetic runs on the GPU through libtincr_tda.so; there is no CPU path.
 """
+import ctypes
+
 import numpy as np
 
 from . import _native as nat
@@ -120,6 +122,21 @@ class _TdaEngine:
         ax = np.empty_like(x)
         self.ctx.check(self.ctx.lib.tda_compute_products(self.ctx.handle, nat.dptr(x), k, nat.dptr(ax)))
         return ax[:, 0] if squeeze else ax
+
+    def zvector(self, rhs, max_iter=10000, conv=1.0e-19):
+        """Z-vector solve A z = rhs for each column of rhs [ov, k] (or [ov]):
+        tda_zvector_lc / tda_zvector_no_lc (gradients/helpers.rs:1130-1223,
+        :1053-1128) on the device.  Returns z and the iterations per column."""
+        b = nat.f64(rhs)
+        squeeze = b.ndim == 1
+        if squeeze:
+            b = b[:, None].copy()
+        k = b.shape[1]
+        z = np.empty_like(b)
+        its = np.zeros(k, dtype=np.int32)
+        self.ctx.check(self.ctx.lib.tda_zvector(self.ctx.handle, nat.dptr(b), k, int(max_iter), float(conv),
+                                                nat.dptr(z), its.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return (z[:, 0] if squeeze else z), its
 
     def precondition(self, r_k, w_k):
         r = nat.f64(r_k)
