@@ -125,3 +125,42 @@ def test_nuclear_gradient_mixed_precision_solve(prec):
     assert np.abs(glp - g64).max() < 1e-6
     s64.close()
     slp.close()
+
+
+@pytest.mark.parametrize("mol,lc", [("uracil", True), ("benzene", True), ("h2o", False)])
+def test_autograd_function_vs_finite_differences(mol, lc):
+    """tincr_amd.autograd.excitation_energies: torch backward (the device
+    adjoint) against central differences of the solved energies along a
+    random symmetric direction of gamma and gamma_lr."""
+    import torch
+    from tincr_amd.autograd import FrozenOrbitals, excitation_energies
+    m = molecule(mol)
+    print(f"autograd {mol} lc={lc}", flush=True)
+    fr = FrozenOrbitals(m["n_atoms"], m["orbs_per_atom"], m["xyz"], m["occ_indices"], m["virt_indices"],
+                        m["orbs"], m["s"], m["orbe"])
+    n = 3
+    rng = np.random.default_rng(5)
+    c = torch.tensor(rng.standard_normal(n))
+    g0 = torch.tensor(m["gamma"], requires_grad=True)
+    glr0 = torch.tensor(m["gamma_lr"], requires_grad=True) if lc else None
+    g = 0.5 * (g0 + g0.T)
+    glr = 0.5 * (glr0 + glr0.T) if lc else None
+    loss = excitation_energies(fr, g, glr, n) @ c
+    loss.backward()
+    d = rng.standard_normal(g0.shape)
+    d = 0.5 * (d + d.T)
+    dl = rng.standard_normal(g0.shape) if lc else None
+    if lc:
+        dl = 0.5 * (dl + dl.T)
+
+    def f(h):
+        gg = torch.tensor(m["gamma"] + h * d)
+        gl = torch.tensor(m["gamma_lr"] + h * dl) if lc else None
+        return float(excitation_energies(fr, gg, gl, n) @ c)
+
+    h = 1e-5
+    fd = (f(h) - f(-h)) / (2 * h)
+    an = float((g0.grad * torch.tensor(d)).sum())
+    if lc:
+        an += float((glr0.grad * torch.tensor(dl)).sum())
+    assert abs(an - fd) < 1e-7 * max(1.0, abs(fd)), (an, fd)
