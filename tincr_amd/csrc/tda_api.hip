@@ -106,8 +106,13 @@ struct tda_ctx {
   // (environment TINCR_TDA_LP_CAP / TINCR_TDA_LP_KEEP); the FP32 and FP64
   // phases keep the reference rule (cap 15 n, collapse to n)
   int lp_cap_mult = 15, lp_keep_mult = 2;
+  int fp32_cap_mult = 15;  // basis cap of the FP32-accuracy phase, multiples of n
   int lp_keep(int n) const { return std::min(64, std::max(n, lp_keep_mult * n)); }
-  int lp_cap(int n) const { return std::min(15 * n, std::max(lp_cap_mult * n, lp_keep(n) + n)); }
+  // rows of the basis / subspace buffers: the reference's 15 n (davidson.rs:74)
+  // or a larger cap of the low-precision phases, at most the eigensolver's 1024
+  int space(int n) const { return std::min(1024, std::max(15, std::max(lp_cap_mult, fp32_cap_mult)) * n); }
+  int lp_cap(int n) const { return std::min(space(n), std::max(lp_cap_mult * n, lp_keep(n) + n)); }
+  int fp32_cap(int n) const { return std::min(space(n), std::max(fp32_cap_mult * n, lp_keep(n) + n)); }
   double pre_thr = 1e-4;
   // Coulomb factor of the charges-form product: 2 (solvers/mod.rs:311); the
   // no-LC Z-vector operator uses 4 (gradients/helpers.rs:1377)
@@ -845,7 +850,9 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
                    bool reorth = false) {
   const int64_t L = c->ovp;
   const int cap = cap_in > 0 ? cap_in : 15 * n;  // max_space (davidson.rs:74)
-  const int ldh = 15 * n;                         // leading dimension of H and Y
+  const int ldh = c->space(n);                    // leading dimension of H and Y
+  TDA_CHECK(cap <= ldh, TDA_ERR_BAD_ARG,
+            "Davidson: a basis of " + std::to_string(cap) + " vectors exceeds the subspace eigensolver's 1024");
   if (keep < n) keep = n;
   const bool thick = thick_restart && keep > n;
   double* V = c->V.as<double>();
@@ -1063,7 +1070,7 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
 
 void alloc_davidson(tda_ctx* c, int n, int keep) {
   const int64_t L = c->ovp;
-  const int cap = 15 * n;
+  const int cap = c->space(n);
   keep = std::max(keep, n);
   c->V.ensure((size_t)cap * L * 8);
   c->AV.ensure((size_t)cap * L * 8);
@@ -1157,6 +1164,8 @@ int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx**
     if (es && atoi(es) >= 1) c->stall_window = atoi(es);
     const char* et = getenv("TINCR_TDA_FP32_THICK");
     if (et) c->fp32_thick = atoi(et) != 0;
+    const char* e3 = getenv("TINCR_TDA_FP32_CAP");
+    if (e3 && atoi(e3) >= 2) c->fp32_cap_mult = atoi(e3);
     const char* ep = getenv("TINCR_TDA_SPLIT32");
     if (ep) c->split32 = atoi(ep) != 0;
     const char* el = getenv("TINCR_TDA_LADDER_MIN_FLOPS");
@@ -1762,7 +1771,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
         const bool ao_op = c->has_lc && c->op_kind == 0 && !c->ct;
         p32 = (c->split32 && ao_op && (c->prec == PREC_FP16 || c->prec == PREC_BF16)) ? PREC_F16X : PREC_FP32;
         DavResult r32 = davidson(c, n, start, std::max(1, max_iter - used), tol, p32, used > 0,
-                                 3 * lp_stall_window(c), h_evals, h_err, 0, c->lp_keep(n), &kk32, 0.8, 1e300,
+                                 3 * lp_stall_window(c), h_evals, h_err, c->fp32_cap(n), c->lp_keep(n), &kk32, 0.8, 1e300,
                                  c->fp32_thick, c->floor_k, true);
         used += r32.iters;
         fp32_iters = r32.iters;
