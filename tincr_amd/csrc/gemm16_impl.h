@@ -31,14 +31,12 @@
 
 namespace tda {
 
-// PF: K slices fetched ahead into registers (1: the next slice while the
-// current one is multiplied; 2: two slices ahead, two register sets).
-template <int BM_, int BN_, int WGM_, int WGN_, int MINB_, int PF_ = 1>
+template <int BM_, int BN_, int WGM_, int WGN_, int MINB_>
 struct Cfg16 {
-  static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_, NT = WGM_ * WGN_ * 64, MINB = MINB_, PF = PF_;
+  static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_, NT = WGM_ * WGN_ * 64, MINB = MINB_;
 };
-using Cfg16Big = Cfg16<256, 256, 2, 4, 1, 1>;  // 8 waves, 128x64 each; 128 KiB LDS
-using Cfg16Mid = Cfg16<128, 128, 2, 2, 2, 1>;  // 4 waves, 64x64 each; 64 KiB LDS
+using Cfg16Big = Cfg16<256, 256, 2, 4, 1>;  // 8 waves, 128x64 each; 128 KiB LDS
+using Cfg16Mid = Cfg16<128, 128, 2, 2, 2>;  // 4 waves, 64x64 each; 64 KiB LDS
 
 template <typename TI>
 struct Mfma16;
@@ -104,7 +102,7 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
 #pragma unroll
   for (int i = 0; i < B_PER; ++i)
     b_off[i] = (uint32_t)(min(n0 + lr + i * (NT / 8), g.N - 1) * ldb) + lkc * 8;
-  VecN<TI, 8> ra[Cfg::PF][A_PER], rb[Cfg::PF][B_PER];
+  VecN<TI, 8> ra[1][A_PER], rb[1][B_PER];  // the next K slice, in flight
   v4f acc[MI][NI];
 #pragma unroll
   for (int i = 0; i < MI; ++i)
@@ -227,44 +225,17 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
     }
   };
   using I0 = std::integral_constant<int, 0>;
-  using I1 = std::integral_constant<int, 1>;
-  if constexpr (Cfg::PF == 1) {
-    if (nkt > 0) {
-      load(0, I0{});
-      store(0, I0{});
-      __syncthreads();
-    }
-    for (int kt = 0; kt < nkt; ++kt) {
-      if (kt + 1 < nkt) load(kt + 1, I0{});
-      rescale(kt);
-      compute(kt & 1);
-      if (kt + 1 < nkt) store((kt + 1) & 1, I0{});
-      __syncthreads();
-    }
-  } else {
-    // slice kt+2 is fetched into register set kt&1 (free: slice kt went to
-    // the LDS in step kt-1) before slice kt is multiplied; slice kt+1 (set
-    // 1-(kt&1), fetched in step kt-1) goes to the LDS after it.
-    if (nkt > 0) load(0, I0{});
-    if (nkt > 1) load(1, I1{});
-    if (nkt > 0) {
-      store(0, I0{});
-      __syncthreads();
-    }
-    auto step = [&](int kt, auto p_c) {
-      constexpr int P = decltype(p_c)::value;
-      using IP = std::integral_constant<int, P>;
-      using IQ = std::integral_constant<int, 1 - P>;
-      if (kt + 2 < nkt) load(kt + 2, IP{});
-      rescale(kt);
-      compute(P);
-      if (kt + 1 < nkt) store(1 - P, IQ{});
-      __syncthreads();
-    };
-    for (int kt = 0; kt < nkt; kt += 2) {
-      step(kt, I0{});
-      if (kt + 1 < nkt) step(kt + 1, I1{});
-    }
+  if (nkt > 0) {
+    load(0, I0{});
+    store(0, I0{});
+    __syncthreads();
+  }
+  for (int kt = 0; kt < nkt; ++kt) {
+    if (kt + 1 < nkt) load(kt + 1, I0{});
+    rescale(kt);
+    compute(kt & 1);
+    if (kt + 1 < nkt) store((kt + 1) & 1, I0{});
+    __syncthreads();
   }
 
   // ---- epilogue: lane holds C[m][n..n+3], m = row l%16 of block i, n = 4*(l/16) of block j ----

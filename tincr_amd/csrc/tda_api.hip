@@ -1733,7 +1733,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     }
     TDA_HIP(hipStreamSynchronize(c->stream));
     std::vector<double> h_evals, h_err;
-    int lp16_iters = 0, fp32_iters = 0, p32 = PREC_FP32;
+    int fp32_iters = 0, p32 = PREC_FP32;
     if (fp64_only) {
       res = davidson(c, n, n, max_iter, tol, PREC_FP64, false, 0, h_evals, h_err);
       res64 = res;
@@ -1763,7 +1763,6 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       // exports its lp_keep lowest Ritz vectors so that the FP64 phase does
       // not restart from a thin basis.
       int start64 = n;
-      lp16_iters = used;
       if (!done) {
         int kk32 = n;
         // the FP32-accuracy phase: FP16 hi / lo split products when the
