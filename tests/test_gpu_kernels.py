@@ -35,6 +35,7 @@ TOL = {"fp64": 1e-13, "fp32": 1e-5, "bf16": 2.5e-5, "fp16": 1e-5}
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16", "fp16"])
 @pytest.mark.parametrize("a_kc,b_kc", [(True, False), (True, True), (False, False), (False, True)])
 @pytest.mark.parametrize("shape", [(16, 16, 64), (37, 53, 70), (130, 129, 257), (300, 20, 4100), (7, 600, 96),
+                                   (300, 12, 4100), (13, 1000, 300), (5, 9, 9000),
                                    (600, 520, 700), (520, 600, 4200)])
 def test_gemm(prec, a_kc, b_kc, shape):
     nat = _nat()

@@ -14,11 +14,12 @@ namespace tda {
 
 GemmPlan gemm_plan(DType in, int64_t M, int64_t N, int64_t K, int batch, bool, bool, int ksplit_hint) {
   GemmPlan p;
-  if (N <= 32 && M > 32) p.cfg = CFG_SKINNY_N;
+  if (in == DT_F64 && N <= 16 && K >= 1024) p.cfg = CFG_SKINNY_N16;
+  else if (N <= 32 && M > 32) p.cfg = CFG_SKINNY_N;
   else if (M <= 32) p.cfg = CFG_SKINNY_M;
   else if (M * N <= (int64_t)256 * 256 * 4) p.cfg = CFG_SMALL;
   else p.cfg = CFG_BIG;
-  const int bk = bk_of(in);
+  const int bk = bk_of(in, p.cfg);
   const int64_t tiles = ceil_div(M, cfg_bm(p.cfg)) * ceil_div(N, cfg_bn(p.cfg)) * batch;
   int ks = 1;
   if (ksplit_hint > 0) {
@@ -149,6 +150,11 @@ void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g,
     pl.k_chunk = round_up(pl.k_chunk, 128);
     pl.ksplit = (int)ceil_div(g.K, pl.k_chunk);  // <= the planned split: the workspace suffices
   }
+  // 64-deep K slices need 64-aligned K-map boundaries; otherwise the 16-deep
+  // skinny tile (the planned k_chunk, a multiple of 64, stays valid)
+  if (pl.cfg == CFG_SKINNY_N16 &&
+      !((g.ak.div >= g.K || g.ak.div % CfgSkinnyN16::BK1 == 0) && (g.bk.div >= g.K || g.bk.div % CfgSkinnyN16::BK1 == 0)))
+    pl.cfg = CFG_SKINNY_N;
   g.ksplit = pl.ksplit;
   g.k_chunk = pl.k_chunk;
   const int bk = bk_of(in, pl.cfg);
