@@ -64,6 +64,17 @@ void launch_cg_dir(double* P, const double* R, const double* b, int n, int64_t L
 void launch_gs_block(const double* P, double* W, double* Q, int64_t L, int nc, double* scratch, int* count,
                      hipStream_t s);
 size_t gs_block_scratch_doubles(int nc);
+// The same block Gram-Schmidt in coefficient form, for nc <= 64.  With
+// u_j = sum_r T[j][r] w_r, the dots it needs are u_s . p_j = (T Gwp)[s][j]
+// and |u_j|^2 = T_j Gww T_j^T, so two Gram matrices replace the nc passes
+// over the vectors:  Gt [2 nc][nc] with Gt[l][r] = p_l . w_r (l < nc) and
+// Gt[nc + l][r] = w_l . w_r.  One wave runs the recurrence of
+// launch_gs_block on them and writes Tq [*count][nc], the rows T_j / |u_j|
+// of the accepted j in order (then Q = Tq W), wn[j] = |u_j| for every j,
+// count[0] = accepted, count[1] = 1 when some |u_j|^2 < 1e-4 |w_j|^2 (the
+// estimate has lost more than 4 of its 16 digits, the orthogonality of the
+// result would lose as many: the caller falls back to launch_gs_block).
+void launch_gs_gram(const double* Gt, int nc, double* Tq, double* wn, int* count, hipStream_t s);
 void launch_copy(const double* src, double* dst, int64_t n, hipStream_t s);
 void launch_fill(double* dst, double v, int64_t n, hipStream_t s);
 // zero columns [c0, c0 + cw) of `rows` rows of length ld (elements of `esz`

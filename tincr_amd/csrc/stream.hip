@@ -550,6 +550,72 @@ void launch_gs_block(const double* P, double* W, double* Q, int64_t L, int nc, d
 
 size_t gs_block_scratch_doubles(int nc) { return (size_t)nc * nc * NB_GS + (size_t)nc * nc + nc; }
 
+// One wave; lane t owns index t (nc <= 64).  LDS accesses between the steps
+// are ordered by the workgroup barrier (a single wave: no waiting on others).
+__global__ void __launch_bounds__(64) gs_gram_kernel(const double* __restrict__ Gt, int nc, double* __restrict__ Tq,
+                                                     double* __restrict__ wn, int* __restrict__ count) {
+  __shared__ double T[64][65];  // row j: coefficients of u_j over w_0..w_j
+  __shared__ double nrm2[64];
+  __shared__ double cs[64];
+  __shared__ int acc[64];
+  const int t = threadIdx.x;
+  const double* Gww = Gt + (int64_t)nc * nc;  // Gww[l * nc + r] = w_l . w_r
+  bool ill = false;
+  for (int j = 0; j < nc; ++j) {
+    // c_s = (u_s . p_j) / |u_s|^2 for the accepted s < j; u_s . p_j = sum_r T[s][r] (p_j . w_r)
+    if (t < j) {
+      double v = 0.0;
+      if (acc[t]) {
+        for (int r = 0; r <= t; ++r) v += T[t][r] * Gt[(int64_t)j * nc + r];
+        v /= nrm2[t];
+      }
+      cs[t] = v;
+    }
+    __syncthreads();
+    // u_j = w_j - sum_s c_s u_s  (T[s][t] = 0 for t > s)
+    if (t < nc) {
+      double v = (t == j) ? 1.0 : 0.0;
+      for (int s2 = t; s2 < j; ++s2) v -= cs[s2] * T[s2][t];
+      T[j][t] = v;
+    }
+    __syncthreads();
+    double part = 0.0;
+    if (t <= j) {
+      double g = 0.0;
+      for (int r = 0; r <= j; ++r) g += Gww[(int64_t)t * nc + r] * T[j][r];
+      part = T[j][t] * g;
+    }
+    const double nn = wave_sum(part);
+    if (t == 0) {
+      nrm2[j] = nn;
+      acc[j] = sqrt(fmax(nn, 0.0)) > 1.0e-8;  // davidson.rs:165
+    }
+    ill = ill || !(nn >= 1.0e-4 * Gww[(int64_t)j * nc + j]);
+    __syncthreads();
+  }
+  if (t < nc) {
+    wn[t] = sqrt(fmax(nrm2[t], 0.0));
+    if (acc[t]) {
+      int rank = 0;
+      for (int s2 = 0; s2 < t; ++s2) rank += acc[s2];
+      const double inv = 1.0 / sqrt(nrm2[t]);
+      for (int r = 0; r < nc; ++r) Tq[(int64_t)rank * nc + r] = r <= t ? T[t][r] * inv : 0.0;
+    }
+  }
+  if (t == 0) {
+    int tot = 0;
+    for (int s2 = 0; s2 < nc; ++s2) tot += acc[s2];
+    count[0] = tot;
+    count[1] = ill ? 1 : 0;
+  }
+}
+
+void launch_gs_gram(const double* Gt, int nc, double* Tq, double* wn, int* count, hipStream_t s) {
+  TDA_CHECK(nc >= 1 && nc <= 64, 2, "gs_gram: nc must be in 1..64");
+  gs_gram_kernel<<<1, 64, 0, s>>>(Gt, nc, Tq, wn, count);
+  TDA_LAUNCH_CHECK();
+}
+
 __global__ void copy_kernel(const double* __restrict__ src, double* __restrict__ dst, int64_t n) {
   for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) dst[t] = src[t];
 }

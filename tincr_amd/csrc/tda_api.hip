@@ -144,7 +144,15 @@ struct tda_ctx {
   // workspaces
   DevBuf ws_gemm, T, xlp, u, uu, vv32, ax_tmp, stage, stage2;
   // Davidson state
-  DevBuf V, AV, H, Y, evals, RZ, R, P, W, norms, npart, idx, count, gs_scratch, eigw;
+  DevBuf V, AV, H, Y, evals, RZ, R, PW, norms, npart, idx, count, gs_scratch, eigw;
+  // P (preconditioned vectors) and W (their projections; the kept Ritz
+  // vectors between phases) share one allocation, W = P + pw_rows L, so that
+  // one GEMM reads both as the rows of [P; W]
+  int64_t pw_rows = 0;
+  double* Pb() { return PW.as<double>(); }
+  double* Wb() { return PW.as<double>() + pw_rows * ovp; }
+  bool gs_gram = true;  // block Gram-Schmidt in coefficient form (launch_gs_gram)
+  int gs_fallbacks = 0;  // blocks that fell back to the vector-by-vector kernel
   // instrumentation
   bool timing = true;
   bool verbose = false;
@@ -168,7 +176,7 @@ struct tda_ctx {
       for (DevBuf* b : {&ls.qvv, &ls.G, &ls.Rt, &ls.Lm, &ls.Rpt, &ls.Lpt, &ls.scale}) b->release();
     for (DevBuf* b : {&C, &S, &SC, &orbe, &aooff, &gamma, &gamma_lr, &qov, &qoo, &qvv, &G, &de, &qov32,
                       &qvv_amax, &G_amax, &ws_gemm, &T, &xlp, &u, &uu, &vv32, &ax_tmp,
-                      &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &P, &W, &norms, &npart, &idx, &count,
+                      &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &PW, &norms, &npart, &idx, &count,
                       &gs_scratch, &eigw, &ao_atom, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
                       &Cl, &Sl, &SCl, &Shl, &aooff_l, &ctW, &ctZ, &Hd, &dxyz, &post,
                       &adjP, &adjQ, &adjX, &adjD, &one, &zvw})
@@ -843,6 +851,49 @@ int lp_stall_window(const tda_ctx* c) {
   return product_flops(c) >= 5e10 ? 10 : 4;
 }
 
+// In-block Gram-Schmidt of the nc new vectors (davidson.rs:162-168): W[j],
+// already projected against the basis, becomes u_j; the accepted q_j go to Q
+// in order.  Returns their number and, when wn is given, |u_j| per j.  P and
+// W are the context's rows (W = P + pw_rows L).  Coefficient form (one GEMM
+// for the Gram matrices of [P; W] against W, one writing Q) unless its norm
+// estimates have lost too many digits; then the vector-by-vector kernel.
+// Synchronizes the stream.
+int gs_block(tda_ctx* c, double* P, double* W, double* Q, int nc, double* wn) {
+  const int64_t L = c->ovp;
+  TDA_CHECK(P == c->Pb() && W == c->Wb(), 2, "gs_block: P / W must be the context's rows");
+  int* d_count = c->count.as<int>();
+  double* scr = c->gs_scratch.as<double>();
+  if (c->gs_gram && nc <= 64) {
+    double* Gt = scr;                         // [2 nc][nc]
+    double* Tq = scr + (size_t)2 * nc * nc;   // [nc][nc]
+    GemmArgs g = gargs(P, W, Gt, 2 * nc, nc, L, split(nc, c->pw_rows * L, L), plain(1), plain(1), plain(L),
+                       plain(nc), plain(1));
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
+    launch_gs_gram(Gt, nc, Tq, c->norms.as<double>(), d_count, c->stream);
+    int hc[2] = {0, 0};
+    TDA_HIP(hipMemcpyAsync(hc, d_count, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (wn) TDA_HIP(hipMemcpyAsync(wn, c->norms.p, (size_t)nc * 8, hipMemcpyDeviceToHost, c->stream));
+    TDA_HIP(hipStreamSynchronize(c->stream));
+    if (!hc[1]) {
+      if (hc[0] > 0) {
+        GemmArgs q = gargs(Tq, W, Q, hc[0], L, nc, plain(nc), plain(1), plain(L), plain(1), plain(L), plain(1));
+        run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, q);
+      }
+      return hc[0];
+    }
+    ++c->gs_fallbacks;
+  }
+  launch_gs_block(P, W, Q, L, nc, scr, d_count, c->stream);
+  if (wn) {
+    launch_norms(W, nc, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
+    TDA_HIP(hipMemcpyAsync(wn, c->norms.p, (size_t)nc * 8, hipMemcpyDeviceToHost, c->stream));
+  }
+  int h = 0;
+  TDA_HIP(hipMemcpyAsync(&h, d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  TDA_HIP(hipStreamSynchronize(c->stream));
+  return h;
+}
+
 DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec, bool first_can_converge,
                    int stop_stall, std::vector<double>& h_evals, std::vector<double>& h_err, int cap_in = 0,
                    int keep = 0, int* kk_out = nullptr, double stall_factor = 0.5,
@@ -861,9 +912,8 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
   double* Y = c->Y.as<double>();
   double* RZ = c->RZ.as<double>();
   double* R = c->R.as<double>();
-  double* P = c->P.as<double>();
-  double* W = c->W.as<double>();
-  int* d_count = c->count.as<int>();
+  double* P = c->Pb();
+  double* W = c->Wb();
   int m = m0, n_done = 0, dim_sub = m0;
   const int dim_sub_origin = m0;
   DavResult res;
@@ -940,8 +990,10 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       int cvd = 0;
       for (double e : h_err) { mx = std::max(mx, e); sum += e; cvd += (e < tol); }
       // columns as solvers/utils.rs:82-121 prints them, plus precision and basis size
-      fprintf(stderr, "tincr_tda davidson prec=%d iter=%4d conv=%3d left=%3d dim_sub=%4d m=%4d total=%.3e max=%.3e w0=%.10f\n",
-              prec, it + 1, cvd, n - cvd, dim_sub, m, sum, mx, h_evals[0]);
+      fprintf(stderr,
+              "tincr_tda davidson prec=%d iter=%4d conv=%3d left=%3d dim_sub=%4d m=%4d total=%.3e max=%.3e "
+              "w0=%.10f gs_fallbacks=%d\n",
+              prec, it + 1, cvd, n - cvd, dim_sub, m, sum, mx, h_evals[0], c->gs_fallbacks);
     }
     if (!finite)
       throw Error{TDA_ERR_HIP, "Davidson: non-finite residual norm or Ritz value (prec " + std::to_string(prec) +
@@ -1009,14 +1061,7 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
           launch_norms(P, nc, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
           TDA_HIP(hipMemcpyAsync(pn.data(), c->norms.p, nc * 8, hipMemcpyDeviceToHost, c->stream));
         }
-        launch_gs_block(P, W, V + (size_t)m * L, L, nc, c->gs_scratch.as<double>(), d_count, c->stream);
-        if (reorth) {
-          launch_norms(W, nc, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
-          TDA_HIP(hipMemcpyAsync(wn.data(), c->norms.p, nc * 8, hipMemcpyDeviceToHost, c->stream));
-        }
-        int h_count = 0;
-        TDA_HIP(hipMemcpyAsync(&h_count, d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        TDA_HIP(hipStreamSynchronize(c->stream));
+        int h_count = gs_block(c, P, W, V + (size_t)m * L, nc, reorth ? wn.data() : nullptr);
         // a vector that kept less than 1/sqrt(2) of its norm through the
         // projections (Daniel, Gragg, Kaufman and Stewart's criterion) has
         // lost orthogonality to rounding: then the block gets a second pass
@@ -1038,9 +1083,7 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
           run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, r1);
           launch_copy(Q, P, (int64_t)h_count * L, c->stream);
           launch_copy(Q, W, (int64_t)h_count * L, c->stream);
-          launch_gs_block(P, W, Q, L, h_count, c->gs_scratch.as<double>(), d_count, c->stream);
-          TDA_HIP(hipMemcpyAsync(&h_count, d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-          TDA_HIP(hipStreamSynchronize(c->stream));
+          h_count = gs_block(c, P, W, Q, h_count, nullptr);
         }
         m += h_count;
       }
@@ -1079,8 +1122,8 @@ void alloc_davidson(tda_ctx* c, int n, int keep) {
   c->evals.ensure((size_t)keep * 8);
   c->RZ.ensure((size_t)n * L * 8);
   c->R.ensure((size_t)n * L * 8);
-  c->P.ensure((size_t)keep * L * 8);
-  c->W.ensure((size_t)keep * L * 8);
+  c->PW.ensure((size_t)2 * keep * L * 8);
+  c->pw_rows = (int64_t)(c->PW.bytes / ((size_t)2 * L * 8));
   c->norms.ensure((size_t)n * 8);
   c->npart.ensure((size_t)n * 256 * 8);
   c->idx.ensure((size_t)n * sizeof(int));
@@ -1170,6 +1213,8 @@ int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx**
     if (ep) c->split32 = atoi(ep) != 0;
     const char* el = getenv("TINCR_TDA_LADDER_MIN_FLOPS");
     if (el && atof(el) >= 0) c->ladder_min_flops = atof(el);
+    const char* eg = getenv("TINCR_TDA_GS_GRAM");
+    if (eg) c->gs_gram = atoi(eg) != 0;
   }
   if (hbm_budget_bytes) c->budget = hbm_budget_bytes;
   int rc = guarded(c, [&] {
@@ -1755,7 +1800,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
                        c->lp_keep(n), &kk, 0.8, 1e300, true, c->floor_k, true);
         used += res.iters;
         done = res.converged;
-        if (kk > n) launch_copy(c->W.as<double>(), c->V.as<double>(), (int64_t)kk * L, c->stream);
+        if (kk > n) launch_copy(c->Wb(), c->V.as<double>(), (int64_t)kk * L, c->stream);
         else launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
         start = kk;
       }
@@ -1777,7 +1822,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
         res.iters = used;
         if (!r32.converged && kk32 > n) start64 = kk32;
       }
-      if (start64 > n) launch_copy(c->W.as<double>(), c->V.as<double>(), (int64_t)start64 * L, c->stream);
+      if (start64 > n) launch_copy(c->Wb(), c->V.as<double>(), (int64_t)start64 * L, c->stream);
       else launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
       res64 = davidson(c, n, start64, std::max(1, max_iter - used), tol, PREC_FP64, true, 0, h_evals, h_err, 0, 0,
                        nullptr, 0.5, 1e300, false, 0.0, true);
