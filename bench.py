@@ -329,8 +329,8 @@ def main():
     args = ap.parse_args()
     args.prec_given = any(a.startswith("--prec") for a in sys.argv[1:])
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is a design (DESIGN.md), not a measured path", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); "
+              "sharding the solve across GPUs is not implemented", file=sys.stderr)
         raise SystemExit(2)
     if args.calibrate_copy:
         from tincr_amd import _native as nat
