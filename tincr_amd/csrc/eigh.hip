@@ -23,6 +23,14 @@ constexpr int EIGH_THREADS = 1024;
 constexpr int EIGH_NW = EIGH_THREADS / 64;
 constexpr int EIGH_MAXM = 1024;
 constexpr int EIGH_POOL = 12288;  // doubles of LDS pool (96 KiB)
+// Subspaces up to 256 (15 n for n <= 17 roots): the per-index LDS arrays
+// shrink to 256 entries and the pool grows to 144 KiB (trailing block in LDS
+// from 128 rows instead of 101, more inverse-iteration slots, the
+// back-transformation in LDS for more (m, n)).  Measured at m = 240, 16
+// roots: 2.82 ms instead of 2.85; a tridiagonalisation step costs about the
+// same from LDS as from L2 (both are one CU's bandwidth plus 3-4 barriers).
+constexpr int EIGH_MAXM_SMALL = 256;
+constexpr int EIGH_POOL_SMALL = 18432;
 
 size_t eigh_work_doubles(int m) {
   // A copy (m*m) + tau (m) + spill area for vectors (64*m) + slots (6*m*64)
@@ -133,20 +141,21 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
+template <int MAXM, int POOL>
 __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __restrict__ H, int ldh, int m, int n,
                                                             double* __restrict__ work, double* __restrict__ evals,
                                                             double* __restrict__ Y, int ldy,
                                                             unsigned long long* __restrict__ clk) {
 #define EIGH_STAMP(i) \
   if (clk && threadIdx.x == 0) clk[i] = wall_clock64();
-  __shared__ double pool[EIGH_POOL];
-  __shared__ double sv[EIGH_MAXM];   // pending Householder vector v_{k-1}
-  __shared__ double sw[EIGH_MAXM];   // pending w_{k-1}
-  __shared__ double snv[EIGH_MAXM];  // current Householder vector
-  __shared__ double snp[EIGH_MAXM];  // current p
-  __shared__ double sd[EIGH_MAXM];   // diagonal of T
-  __shared__ double se[EIGH_MAXM];   // off-diagonal of T
-  __shared__ double se2[EIGH_MAXM];  // squares of off-diagonal
+  __shared__ double pool[POOL];
+  __shared__ double sv[MAXM];   // pending Householder vector v_{k-1}
+  __shared__ double sw[MAXM];   // pending w_{k-1}
+  __shared__ double snv[MAXM];  // current Householder vector
+  __shared__ double snp[MAXM];  // current p
+  __shared__ double sd[MAXM];   // diagonal of T
+  __shared__ double se[MAXM];   // off-diagonal of T
+  __shared__ double se2[MAXM];  // squares of off-diagonal
   __shared__ double red[EIGH_NW];
   __shared__ double sval[4];
   __shared__ int cl_start[65];
@@ -190,7 +199,7 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
   double* Am = A;
   int k0 = 0, lda = m;
   bool in_lds = false;
-  double* part = pool + EIGH_POOL - 2048;  // segment partial sums (global phase only)
+  double* part = pool + POOL - 2048;  // segment partial sums (global phase only)
   __shared__ double red2[2][EIGH_NW];
   int rb = 0;
   // one-barrier block sum (alternating buffers)
@@ -216,7 +225,7 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
   }
   for (int k = 0; k < m - 2; ++k) {
     const int nt = m - k - 1;  // trailing size (rows/cols k+1..m-1)
-    if (!in_lds && (m - k) * (m - k) <= EIGH_POOL - 2048) {
+    if (!in_lds && (m - k) * (m - k) <= POOL - 2048) {
       // move rows/cols k..m-1 into LDS (the pending update travels along)
       const int nb = m - k;
       for (int t = tid; t < nb * nb; t += EIGH_THREADS) {
@@ -386,7 +395,7 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
   // one wave per cluster with an LDS slot of 4*m doubles (ri, u1, u2, z):
   // lane 0 runs the LU recurrences, the wave does the vector operations
   const int slot_sz = 4 * m;
-  const int nslots = max(1, min(EIGH_NW, EIGH_POOL / slot_sz));
+  const int nslots = max(1, min(EIGH_NW, POOL / slot_sz));
   if (wave < nslots) {
     double* sl = pool + (size_t)wave * slot_sz;
     double *ri = sl, *u1 = sl + m, *u2 = sl + 2 * m, *z = sl + 3 * m;
@@ -449,7 +458,7 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
   //    Z -= V (T (V^T Z)).  Z and the V panel live in LDS when they fit.
   constexpr int BW = 16;
   const int nref = max(0, m - 2);  // reflectors 0..m-3
-  const bool fits = (size_t)n * m + (size_t)BW * m + 2 * BW * BW + (size_t)BW * n <= (size_t)EIGH_POOL;
+  const bool fits = (size_t)n * m + (size_t)BW * m + 2 * BW * BW + (size_t)BW * n <= (size_t)POOL;
   if (fits && nref > 0) {
     double* Zs = pool;                       // [n][m]
     double* Vp = pool + (size_t)n * m;       // [BW][m]  (row q = reflector s+q)
@@ -543,7 +552,11 @@ void launch_eigh(const double* H, int ldh, int m, int n, double* work, double* e
   TDA_CHECK(m >= 1 && m <= EIGH_MAXM, 2, "eigh: subspace dimension out of range (1..1024)");
   TDA_CHECK(n >= 1 && n <= m && n <= 64, 2, "eigh: number of roots out of range");
   TDA_CHECK(ldy >= m, 2, "eigh: ldy < m");
-  eigh_kernel<<<1, EIGH_THREADS, 0, s>>>(H, ldh, m, n, work, evals, Y, ldy, phase_clk);
+  if (m <= EIGH_MAXM_SMALL)
+    eigh_kernel<EIGH_MAXM_SMALL, EIGH_POOL_SMALL><<<1, EIGH_THREADS, 0, s>>>(H, ldh, m, n, work, evals, Y, ldy,
+                                                                         phase_clk);
+  else
+    eigh_kernel<EIGH_MAXM, EIGH_POOL><<<1, EIGH_THREADS, 0, s>>>(H, ldh, m, n, work, evals, Y, ldy, phase_clk);
   TDA_LAUNCH_CHECK();
 }
 
