@@ -225,6 +225,23 @@ def test_small_system_mixed_context_runs_fp64(monkeypatch):
     slp.close()
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_second_gram_schmidt_pass(monkeypatch, prec):
+    """TINCR_TDA_REORTH=1: the mixed phases give every block of new vectors
+    the second Gram-Schmidt pass; the solve still lands within 1 meV of FP64."""
+    print(f"second gram-schmidt pass {prec}", flush=True)
+    m = molecule("benzene")
+    s64 = _system(m)
+    w64, _ = s64.run_tda(4, 200, 1e-5)
+    s64.close()
+    monkeypatch.setenv("TINCR_TDA_REORTH", "1")
+    slp = _system(m, precision=prec)
+    w, _ = slp.run_tda(4, 200, 1e-5)
+    assert slp.last_stats["converged"] == 1
+    assert np.abs(w - w64).max() * HARTREE_TO_EV < 1e-3
+    slp.close()
+
+
 @pytest.mark.parametrize("prec,n", [("fp64", 10), ("fp16", 10), ("fp64", 40)])
 def test_block_gram_schmidt_forms_agree(monkeypatch, prec, n):
     """The in-block Gram-Schmidt of the new Davidson vectors in coefficient
