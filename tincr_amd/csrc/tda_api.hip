@@ -154,7 +154,8 @@ struct tda_ctx {
   bool gs_gram = true;  // block Gram-Schmidt in coefficient form (launch_gs_gram)
   int gs_fallbacks = 0;  // blocks that fell back to the vector-by-vector kernel
   int reorth_passes = 0;  // second passes taken (log only)
-  double reorth_ratio = 0.01;  // second Gram-Schmidt pass below this |u_j| / |p_j| (ladder phases)
+  double reorth_ratio = 0.01;
+  bool lp_phase = true;  // the FP16 / BF16 phase of the ladder (off: start at the FP32-accuracy phase)  // second Gram-Schmidt pass below this |u_j| / |p_j| (ladder phases)
   // instrumentation
   bool timing = true;
   bool verbose = false;
@@ -1222,6 +1223,8 @@ int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx**
     if (el && atof(el) >= 0) c->ladder_min_flops = atof(el);
     const char* eg = getenv("TINCR_TDA_GS_GRAM");
     if (eg) c->gs_gram = atoi(eg) != 0;
+    const char* elp = getenv("TINCR_TDA_LP_PHASE");
+    if (elp) c->lp_phase = atoi(elp) != 0;
     const char* er = getenv("TINCR_TDA_REORTH");
     if (er && atof(er) >= 0 && atof(er) <= 1) c->reorth_ratio = atof(er);
   }
@@ -1804,7 +1807,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       int used = 0;
       bool done = false;
       int start = n, kk = n;
-      if (c->prec == PREC_FP16 || c->prec == PREC_BF16) {
+      if ((c->prec == PREC_FP16 || c->prec == PREC_BF16) && c->lp_phase) {
         res = davidson(c, n, n, max_iter, tol, c->prec, false, lp_stall_window(c), h_evals, h_err, c->lp_cap(n),
                        c->lp_keep(n), &kk, 0.8, 1e300, true, c->floor_k, true);
         used += res.iters;
