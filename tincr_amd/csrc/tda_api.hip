@@ -154,6 +154,15 @@ struct tda_ctx {
   bool gs_gram = true;  // block Gram-Schmidt in coefficient form (launch_gs_gram)
   int gs_fallbacks = 0;  // blocks that fell back to the vector-by-vector kernel
   int reorth_passes = 0;  // second passes taken (log only)
+  // Pinned host staging of the Davidson's per-iteration scalars (a copy to
+  // pageable memory would wait for the stream): [0, 64) eigenvalues,
+  // [64, 128) residual norms, [128, 192) |p_j|, [192, 256) |u_j|,
+  // [256, 320) as int: the expansion indices, [320, 322) as int: counts.
+  double* hpin = nullptr;
+  double* pinned() {
+    if (!hpin) TDA_HIP(hipHostMalloc((void**)&hpin, 512 * sizeof(double), hipHostMallocDefault));
+    return hpin;
+  }
   double reorth_ratio = 0.01;
   bool lp_phase = true;  // the FP16 / BF16 phase of the ladder (off: start at the FP32-accuracy phase)  // second Gram-Schmidt pass below this |u_j| / |p_j| (ladder phases)
   // instrumentation
@@ -174,6 +183,8 @@ struct tda_ctx {
   std::vector<hipEvent_t> ev_pool;
 
   ~tda_ctx() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (hpin) (void)hipHostFree(hpin);
     for (auto e : ev_pool) (void)hipEventDestroy(e);
     for (auto& ls : lps)
       for (DevBuf* b : {&ls.qvv, &ls.G, &ls.Rt, &ls.Lm, &ls.Rpt, &ls.Lpt, &ls.scale}) b->release();
@@ -873,10 +884,12 @@ int gs_block(tda_ctx* c, double* P, double* W, double* Q, int nc, double* wn) {
                        plain(nc), plain(1));
     run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
     launch_gs_gram(Gt, nc, Tq, c->norms.as<double>(), d_count, c->stream);
-    int hc[2] = {0, 0};
+    int* hc = (int*)(c->pinned() + 320);
+    double* hwn = c->pinned() + 192;
     TDA_HIP(hipMemcpyAsync(hc, d_count, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    if (wn) TDA_HIP(hipMemcpyAsync(wn, c->norms.p, (size_t)nc * 8, hipMemcpyDeviceToHost, c->stream));
+    if (wn) TDA_HIP(hipMemcpyAsync(hwn, c->norms.p, (size_t)nc * 8, hipMemcpyDeviceToHost, c->stream));
     TDA_HIP(hipStreamSynchronize(c->stream));
+    if (wn) std::memcpy(wn, hwn, (size_t)nc * 8);
     if (!hc[1]) {
       if (hc[0] > 0) {
         GemmArgs q = gargs(Tq, W, Q, hc[0], L, nc, plain(nc), plain(1), plain(L), plain(1), plain(L), plain(1));
@@ -887,14 +900,16 @@ int gs_block(tda_ctx* c, double* P, double* W, double* Q, int nc, double* wn) {
     ++c->gs_fallbacks;
   }
   launch_gs_block(P, W, Q, L, nc, scr, d_count, c->stream);
+  double* hwn = c->pinned() + 192;
+  int* hc = (int*)(c->pinned() + 320);
   if (wn) {
     launch_norms(W, nc, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
-    TDA_HIP(hipMemcpyAsync(wn, c->norms.p, (size_t)nc * 8, hipMemcpyDeviceToHost, c->stream));
+    TDA_HIP(hipMemcpyAsync(hwn, c->norms.p, (size_t)nc * 8, hipMemcpyDeviceToHost, c->stream));
   }
-  int h = 0;
-  TDA_HIP(hipMemcpyAsync(&h, d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  TDA_HIP(hipMemcpyAsync(hc, d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   TDA_HIP(hipStreamSynchronize(c->stream));
-  return h;
+  if (wn) std::memcpy(wn, hwn, (size_t)nc * 8);
+  return hc[0];
 }
 
 DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec, bool first_can_converge,
@@ -978,9 +993,12 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       launch_norms(R, n, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
       h_evals.resize(n);
       h_err.resize(n);
-      TDA_HIP(hipMemcpyAsync(h_evals.data(), c->evals.p, n * 8, hipMemcpyDeviceToHost, c->stream));
-      TDA_HIP(hipMemcpyAsync(h_err.data(), c->norms.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+      double* hp = c->pinned();
+      TDA_HIP(hipMemcpyAsync(hp, c->evals.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipMemcpyAsync(hp + 64, c->norms.p, n * 8, hipMemcpyDeviceToHost, c->stream));
       TDA_HIP(hipStreamSynchronize(c->stream));
+      std::memcpy(h_evals.data(), hp, n * 8);
+      std::memcpy(h_err.data(), hp + 64, n * 8);
       float ms = 0;
       TDA_HIP(hipEventElapsedTime(&ms, t.a, t.b));
       c->t_products_ms += ms;
@@ -1046,7 +1064,9 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       const int nc = (int)idx.size();
       dim_sub += roots_lft;  // davidson.rs:159 (counts dropped vectors too)
       if (nc > 0) {
-        TDA_HIP(hipMemcpyAsync(c->idx.p, idx.data(), nc * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        int* hidx = (int*)(c->pinned() + 256);  // not in flight: the last copy from it finished at a sync
+        std::memcpy(hidx, idx.data(), nc * sizeof(int));
+        TDA_HIP(hipMemcpyAsync(c->idx.p, hidx, nc * sizeof(int), hipMemcpyHostToDevice, c->stream));
         launch_precondition_many(R, c->evals.as<double>(), c->idx.as<int>(), nc, c->de.as<double>(), L,
                                  c->pre_thr, P, c->stream);
         // classical Gram-Schmidt (davidson.rs:162-168): W = P - V (V^T P) ...
@@ -1062,9 +1082,10 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
         std::vector<double> pn(nc), wn(nc);
         if (reorth) {
           launch_norms(P, nc, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
-          TDA_HIP(hipMemcpyAsync(pn.data(), c->norms.p, nc * 8, hipMemcpyDeviceToHost, c->stream));
+          TDA_HIP(hipMemcpyAsync(c->pinned() + 128, c->norms.p, nc * 8, hipMemcpyDeviceToHost, c->stream));
         }
         int h_count = gs_block(c, P, W, V + (size_t)m * L, nc, reorth ? wn.data() : nullptr);
+        if (reorth) std::memcpy(pn.data(), c->pinned() + 128, nc * 8);  // gs_block synchronized the stream
         // one classical pass leaves a kept vector q_j = u_j / |u_j| with
         // components of about eps |p_j| / |u_j| along the basis: when some
         // vector kept less than reorth_ratio of its norm through the
