@@ -151,10 +151,22 @@ def test_gemm16_store(prec, shape):
     assert np.all(np.abs(out - ref) <= ulp * (np.abs(ref) + ACC16 * scale) + ACC16 * scale + 1e-30)
 
 
+@pytest.fixture(params=["rule", "large"])
+def tile16(request, monkeypatch):
+    """The tile rule of small problems, or the 256x256 tile forced (FP64
+    outputs stay on 128x128)."""
+    if request.param == "large":
+        monkeypatch.setenv("TINCR_TDA_TILE16", "1")
+    else:
+        monkeypatch.delenv("TINCR_TDA_TILE16", raising=False)
+    return request.param
+
+
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
 @pytest.mark.parametrize("shape", [(37, 53, 70, 2), (200, 333, 501, 2), (520, 520, 72, 60)])
-def test_gemm16_diag(prec, shape):
+def test_gemm16_diag(prec, shape, tile16):
     nat = _nat()
+    print(f"gemm16 diag {prec} {shape} tile={tile16}", flush=True)
     m, n, k, bt = shape
     rng = np.random.default_rng(k)
     a, b = _inputs16(rng, prec, (m, k), (bt, n, k))
@@ -201,13 +213,13 @@ def test_gemm16_hadamard(prec, case, shape):
 
 @pytest.mark.parametrize("epi", ["store", "diag", "hadamard"])
 @pytest.mark.parametrize("shape", [(37, 53, 70, 2), (200, 333, 501, 2), (520, 520, 72, 60)])
-def test_gemm16_split(epi, shape):
+def test_gemm16_split(epi, shape, tile16):
     """FP16 hi / lo split operands and output: about FP32 accuracy.  Plain
     FP16 inputs would be off by ~2^-11 per term (~2e-5 of sum |a b| here);
     the split must stay two orders below that."""
     nat = _nat()
     m, n, k, bt = shape
-    print(f"gemm16 split {epi} {shape}", flush=True)
+    print(f"gemm16 split {epi} {shape} tile={tile16}", flush=True)
     rng = np.random.default_rng(3 * m + n + k)
     a = rng.uniform(-1, 1, (m, k))
     b = rng.uniform(-1, 1, (bt, n, k))

@@ -94,6 +94,11 @@ void dispatch16(DType in, Epi epi, GemmArgs& g, hipStream_t s, int tile) {
 }
 
 int tune16(DType in, Epi epi, GemmArgs& g, hipStream_t s) {
+  // TINCR_TDA_TILE16=0 / 1 forces the 128x128 / 256x256 tile (tests of the
+  // tiles on small problems; the FP64-output epilogue stays 128x128); read
+  // per call so that a process can switch
+  const char* ft = getenv("TINCR_TDA_TILE16");
+  if (ft && (ft[0] == '0' || ft[0] == '1') && ft[1] == 0) return ft[0] - '0';
   if (epi == EPI_DIAG || !autotune16_on()) return -1;
   const double flops = 2.0 * (double)g.M * (double)g.N * (double)g.K * g.batch * (g.a_lo ? 3.0 : 1.0);
   if (flops < 1e10) return -1;
