@@ -37,6 +37,16 @@ size_t eigh_work_doubles(int m) {
   return (size_t)m * m + (size_t)m + (size_t)64 * m + (size_t)6 * m * 64 + 64;
 }
 
+// 1 / q from the hardware reciprocal refined by two Newton steps
+// (r <- r + r (1 - q r)): a relative error of a few ulp, against the ~10
+// dependent instructions of the correctly rounded division.  A Sturm count
+// only needs each q_i to be that of slightly perturbed data.
+__device__ inline double sturm_recip(double q) {
+  double r = __builtin_amdgcn_rcp(q);
+  r = fma(fma(-q, r, 1.0), r, r);
+  return fma(fma(-q, r, 1.0), r, r);
+}
+
 __device__ inline int sturm_count(const double* d, const double* e2, int m, double x, double pivmin) {
   int cnt = 0;
   double q = d[0] - x;
@@ -47,21 +57,21 @@ __device__ inline int sturm_count(const double* d, const double* e2, int m, doub
   for (; i + 4 <= m; i += 4) {
     const double d0 = d[i] - x, d1 = d[i + 1] - x, d2 = d[i + 2] - x, d3 = d[i + 3] - x;
     const double f0 = e2[i - 1], f1 = e2[i], f2 = e2[i + 1], f3 = e2[i + 2];
-    q = d0 - f0 / q;
+    q = d0 - f0 * sturm_recip(q);
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += (q < 0.0);
-    q = d1 - f1 / q;
+    q = d1 - f1 * sturm_recip(q);
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += (q < 0.0);
-    q = d2 - f2 / q;
+    q = d2 - f2 * sturm_recip(q);
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += (q < 0.0);
-    q = d3 - f3 / q;
+    q = d3 - f3 * sturm_recip(q);
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += (q < 0.0);
   }
   for (; i < m; ++i) {
-    q = d[i] - x - e2[i - 1] / q;
+    q = d[i] - x - e2[i - 1] * sturm_recip(q);
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += (q < 0.0);
   }
