@@ -306,9 +306,11 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
     __syncthreads();
     staged = R * Cn <= CAP;
     if (staged) {
-      for (int e = tid; e < R * Cn; e += NT) {
-        const int r = (uint32_t)e / (uint32_t)Cn, c = e - r * Cn;
-        ht[e] = (float)g.hsrc[g.hm(rkey[r]) + g.hn(ckey[c])];
+      // one wave per table row (its H row offset computed once), lanes along
+      // the columns: no index division, coalesced gathers along an H row
+      for (int r = wave; r < R; r += NW) {
+        const int64_t hr = g.hm(rkey[r]);
+        for (int c = lane; c < Cn; c += 64) ht[r * Cn + c] = (float)g.hsrc[hr + g.hn(ckey[c])];
       }
     }
 #pragma unroll
