@@ -225,6 +225,24 @@ def test_small_system_mixed_context_runs_fp64(monkeypatch):
     slp.close()
 
 
+def test_run_tda_without_host_x():
+    """copy_x=False (X = NULL through the C ABI): the CI vectors stay on the
+    device; omega, f and the adjoint of the device vectors equal those of a
+    run that copies X out."""
+    m = molecule("uracil")
+    s = _system(m, precision="fp16")
+    w1, x1 = s.run_tda(4, 200, 1e-6)
+    f1 = s.properties["oscillator_strengths"].copy()
+    a1 = s.adjoint(4, x1)
+    w2, x2 = s.run_tda(4, 200, 1e-6, copy_x=False)
+    assert x2 is None
+    assert np.array_equal(w1, w2)
+    assert np.array_equal(f1, s.properties["oscillator_strengths"])
+    a2 = s.adjoint(4)
+    assert np.abs(a1["d_gamma"] - a2["d_gamma"]).max() < 1e-12
+    s.close()
+
+
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
 def test_second_gram_schmidt_pass(monkeypatch, prec):
     """TINCR_TDA_REORTH=1: the mixed phases give every block of new vectors

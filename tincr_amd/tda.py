@@ -148,10 +148,12 @@ class _TdaEngine:
         return self.n_occ * self.n_virt
 
     # -- run_tda (tda/mod.rs:74-131) ----------------------------------------
-    def run_tda(self, n_roots, max_iter, tolerance):
+    def run_tda(self, n_roots, max_iter, tolerance, copy_x=True):
+        """Solve; returns (omega, X).  copy_x=False leaves the CI vectors on
+        the device (X is None; tda_adjoint / adjoint() still use them)."""
         N, ov = self.n_atoms, self.n_occ * self.n_virt
         omega = np.empty(n_roots)
-        X = np.empty((ov, n_roots))
+        X = np.empty((ov, n_roots)) if copy_x else None
         q_trans = np.empty((N, n_roots))
         tr_dip = np.empty((3, n_roots))
         f = np.empty(n_roots)
