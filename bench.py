@@ -7,7 +7,9 @@ all-trans polyene C200H202 (402 atoms, 1002 AO, 501 x 501, ov = 251,001).
 One step = one full solve from device-resident orbitals: transition-charge
 build, operator set-up, Davidson (FP16 products, then FP16 hi/lo split
 products of FP32 accuracy, then the FP64 verification) and the outputs
-(omega, X, q_trans, dipoles, f) back on the host.
+(omega, X, q_trans, dipoles, f) back on the host (X, ov x n FP64, into
+page-locked memory).  Both the mixed path and the plain FP64 path get one
+warm-up solve and then --steps timed solves; states/s = n / median.
 
     python bench.py [--steps K] [--warmup W] [--prec fp16|fp32|bf16|fp64]
                     [--system water512|polyene200|...] [--states n]
@@ -80,7 +82,7 @@ def kernel_table(tot_ms, tot_fl, tot_n, operator, prec, split=False):
     return rows
 
 
-def build_inputs(system, seed):
+def build_inputs(system, seed, kind="synthetic"):
     from tincr_amd import systems as sy
     if system.startswith("polyene"):
         mol = sy.polyene(int(system[len("polyene"):]), seed=seed)
@@ -88,7 +90,10 @@ def build_inputs(system, seed):
         mol = sy.water_cluster(int(system[len("water"):]), seed=seed)
     else:
         raise SystemExit(f"unknown system {system}")
-    return mol, sy.tda_inputs(mol, seed=seed)
+    inp = sy.tda_inputs(mol, seed=seed)
+    inp["data"] = ("synthetic orbitals (seeded Haar-random C, S = I, gapped orbital energies) with the real "
+                   "gamma / gamma_lr of the seeded geometry")
+    return mol, inp
 
 
 def make_system(inp, precision, operator):
@@ -107,12 +112,25 @@ def sync():
         pass
 
 
-def solve(sysg, n_states, max_iter, tol, copy_x=False):
-    """One solve.  The CI vectors stay in HBM unless copy_x (the host copy of
-    X, ov x n FP64, is a PCIe transfer, reported on its own)."""
+def pinned_x(ov, n):
+    """Page-locked [ov, n] float64 host array (the device copies X into it
+    directly); a plain array when torch cannot pin."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return torch.empty((ov, n), dtype=torch.float64, pin_memory=True).numpy()
+    except Exception:
+        pass
+    return np.empty((ov, n))
+
+
+def solve(sysg, n_states, max_iter, tol, copy_x=True, x_out=None):
+    """One solve from device-resident orbitals to host-resident omega, X
+    (into x_out when given), q_trans, dipoles and f.  copy_x=False leaves X
+    in HBM (reported on the side, never as the value)."""
     sysg.clear_charges()
     t0 = time.perf_counter()
-    w, x = sysg.run_tda(n_states, max_iter, tol, copy_x=copy_x)
+    w, x = sysg.run_tda(n_states, max_iter, tol, copy_x=copy_x, x_out=x_out)
     t = time.perf_counter() - t0
     return t, w, x, dict(sysg.last_stats), sysg.properties["oscillator_strengths"].copy()
 
@@ -325,6 +343,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="single", choices=["single", "fmo"])
+    ap.add_argument("--inputs", default="synthetic", choices=["synthetic"])
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 256 MiB calibration copy first (reference for counter runs)")
@@ -340,37 +359,46 @@ def main():
     if args.workload == "fmo":
         return main_fmo(args)
 
-    mol, inp = build_inputs(args.system, args.seed)
+    mol, inp = build_inputs(args.system, args.seed, args.inputs)
     n = args.states
+    ov = len(inp["occ_indices"]) * len(inp["virt_indices"])
     s64 = make_system(inp, "fp64", args.operator)
     slp = make_system(inp, args.prec, args.operator) if args.prec != "fp64" else s64
+    xbuf = pinned_x(ov, n)
 
-    # warm-up (kernel loading, allocations) -- untimed
+    # plain FP64 path on the same GPU: one warm-up, then the median of
+    # args.steps solves, each ending with omega, X, f in host memory
+    solve(s64, n, args.max_iter, args.tol, x_out=xbuf)
+    t64s = []
+    for _ in range(args.steps):
+        t, w64, x64, st64, f64 = solve(s64, n, args.max_iter, args.tol, x_out=xbuf)
+        t64s.append(t)
+    x64 = x64.copy()
+    t64 = float(np.median(t64s))
+
+    # the measured path: warm-up (kernel loading, allocations) untimed, then
+    # exactly args.steps timed solves; value = states / median solve time
     for _ in range(max(args.warmup, 0)):
-        solve(slp, n, args.max_iter, args.tol)
-    # plain FP64 reference solve on the same GPU
-    t64, w64, x64, st64, f64 = solve(s64, n, args.max_iter, args.tol)
-    if slp is not s64:
-        t64b, w64, x64, st64, f64 = solve(s64, n, args.max_iter, args.tol)
-        t64 = min(t64, t64b)
-
-    # timed steps
+        solve(slp, n, args.max_iter, args.tol, x_out=xbuf)
     sync()
     times, stats = [], []
     t_begin = time.perf_counter()
     for _ in range(args.steps):
-        t, w, x, st, f = solve(slp, n, args.max_iter, args.tol)
+        t, w, x, st, f = solve(slp, n, args.max_iter, args.tol, x_out=xbuf)
         times.append(t)
         stats.append(st)
     sync()
     elapsed = time.perf_counter() - t_begin
     ms_per_step = 1e3 * elapsed / args.steps
-    # one more solve, untimed for the value, with X copied to the host
-    t_x, _, _, _, _ = solve(slp, n, args.max_iter, args.tol, copy_x=True)
-    value = n / (elapsed / args.steps)
+    t_med = float(np.median(times))
+    # beside the value: the same solve leaving X in HBM
+    t_hbm, _, _, _, _ = solve(slp, n, args.max_iter, args.tol, copy_x=False)
+    value = n / t_med
     fp64_value = n / t64
-    dw_mev = float(np.abs(w - w64).max() * HARTREE_TO_EV * 1e3)
-    df = float(np.abs(f - f64).max())
+    dw = np.abs(w - w64) * HARTREE_TO_EV * 1e3
+    dfs = np.abs(f - f64)
+    # the CI vectors agree up to sign
+    ov_x = np.abs(np.einsum("ij,ij->j", x, x64))
 
     # roofline of the dominant kernel (largest total time in the timed steps)
     tot_ms = np.zeros((2, 3))
@@ -395,33 +423,44 @@ def main():
         "flops_per_launch": tot_fl[si, pi] / max(tot_n[si, pi], 1),
     }
     last = stats[-1]
+
+    def products_per_s(st):
+        return round(st["n_products"] / (st["t_products_ms"] * 1e-3), 2) if st["t_products_ms"] > 0 else None
+
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "states/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": f"{args.prec}+fp64" if args.prec != "fp64" else "fp64",
-        "data": ("synthetic orbitals (seeded Haar-random C, S = I, gapped orbital energies) with the real "
-                 "gamma / gamma_lr of the seeded geometry"),
+        "data": inp.get("data", "synthetic"),
         "config": {
             "workload": f"LC-TDA {mol.name}: {mol.n_atoms} atoms, {mol.n_ao} AO, {mol.n_occ} occ x "
                         f"{mol.n_virt} virt (ov={mol.n_occ * mol.n_virt}), lowest {n} states, tol {args.tol}",
             "precision": args.prec, "operator": args.operator, "n_states": n, "tol": args.tol,
+            "inputs": args.inputs,
         },
-        "fp64": {"states_per_s": round(fp64_value, 4), "ms_per_solve": round(1e3 * t64, 3),
+        "timing": ("states / median of the timed solves; a solve runs from device-resident orbitals to omega, "
+                   "X (ov x n FP64, into page-locked host memory), q_trans, dipoles and f on the host"),
+        "ms_per_solve_median": round(1e3 * t_med, 3), "ms_per_solve": [round(1e3 * t, 3) for t in times],
+        "fp64": {"states_per_s": round(fp64_value, 4), "ms_per_solve_median": round(1e3 * t64, 3),
+                 "ms_per_solve": [round(1e3 * t, 3) for t in t64s],
                  "n_iter": st64["n_iter"], "n_products": st64["n_products"],
-                 "t_products_ms": round(st64["t_products_ms"], 2)},
+                 "t_products_ms": round(st64["t_products_ms"], 2), "products_per_s": products_per_s(st64)},
         "speedup_vs_fp64": round(value / fp64_value, 3),
-        "max_abs_domega_meV": dw_mev, "max_abs_df": df,
+        "max_abs_domega_meV": float(dw.max()), "max_abs_df": float(dfs.max()),
+        "per_state": {"omega_fp64_eV": [round(float(e) * HARTREE_TO_EV, 6) for e in w64],
+                      "abs_domega_meV": [float(e) for e in dw], "f_fp64": [float(e) for e in f64],
+                      "abs_df": [float(e) for e in dfs], "abs_overlap_x": [round(float(e), 12) for e in ov_x]},
         "mixed": {"n_iter": last["n_iter"], "n_iter_fp32": last["n_iter_fp32"], "n_iter_fp64": last["n_iter_fp64"],
                   "n_products": last["n_products"], "n_products_fp32": last["n_products_fp32"],
                   "n_products_fp64": last["n_products_fp64"],
                   "max_residual": last["max_residual"], "t_qbuild_ms": last["t_qbuild_ms"],
-                  "t_products_ms": last["t_products_ms"]},
+                  "t_products_ms": last["t_products_ms"], "products_per_s": products_per_s(last)},
         "roofline": roofline,
         "kernels": kernel_table(tot_ms, tot_fl, tot_n, args.operator, args.prec, split),
-        # the same solve handing X (ov x n FP64) back in host memory over PCIe
-        "with_host_x": {"states_per_s": round(n / t_x, 4), "ms_per_solve": round(1e3 * t_x, 3),
-                        "x_bytes": len(inp["occ_indices"]) * len(inp["virt_indices"]) * n * 8},
+        # the same solve with the CI vectors left in HBM (not the value)
+        "x_in_hbm": {"states_per_s": round(n / t_hbm, 4), "ms_per_solve": round(1e3 * t_hbm, 3),
+                     "x_bytes": ov * n * 8},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(inp, n, st64["n_iter"], st64["n_products"])

@@ -260,6 +260,29 @@ def test_second_gram_schmidt_pass(monkeypatch, prec):
     slp.close()
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("op", ["ao", "charges"])
+def test_fp64_handover_without_second_pass(monkeypatch, prec, op):
+    """TINCR_TDA_REORTH=0 (no second Gram-Schmidt pass in the 16-bit phases,
+    the setting under which a BF16 phase on benzene once built a nearly
+    dependent basis): the FP64 phase orthonormalises its start block and
+    accepts only unit Ritz vectors, so the solve still lands within 1 meV of
+    FP64 with orthonormal CI vectors."""
+    print(f"fp64 hand-over without reorth {prec} {op}", flush=True)
+    m = molecule("benzene")
+    s64 = _system(m)
+    w64, _ = s64.run_tda(4, 200, 1e-5)
+    s64.close()
+    monkeypatch.setenv("TINCR_TDA_REORTH", "0")
+    slp = _system(m, precision=prec, operator=op)
+    w, x = slp.run_tda(4, 200, 1e-5)
+    assert slp.last_stats["converged"] == 1
+    assert np.abs(w - w64).max() * HARTREE_TO_EV < 1e-3
+    assert w.min() > 0.1
+    assert np.abs(x.T @ x - np.eye(4)).max() < 1e-10
+    slp.close()
+
+
 @pytest.mark.parametrize("prec,n", [("fp64", 10), ("fp16", 10), ("fp64", 40)])
 def test_block_gram_schmidt_forms_agree(monkeypatch, prec, n):
     """The in-block Gram-Schmidt of the new Davidson vectors in coefficient

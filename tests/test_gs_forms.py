@@ -103,3 +103,44 @@ def test_vectors_in_the_basis_span_are_dropped_without_fallback():
     assert len(q_ref) == len(q) == 4
     assert np.abs(q - q_ref).max() < 1e-12
     assert n_ref[2] < 1e-8 and n[2] < 1e-8
+
+
+def _rayleigh_ritz(a, basis, n):
+    """One Davidson step on a basis assumed orthonormal (davidson.rs:89-107):
+    eigh of V^T A V, Ritz vectors V y, residual norms |A z - theta z|."""
+    h = basis.T @ a @ basis
+    theta, y = np.linalg.eigh(h)
+    z = basis @ y[:, :n]
+    r = a @ z - z * theta[:n]
+    return theta[:n], z, np.linalg.norm(r, axis=0)
+
+
+def test_fp64_handover_model_rejects_nearly_dependent_basis():
+    """Model of the FP64 hand-over of the precision ladder.  A basis that a
+    low-precision phase left nearly dependent (two columns equal to 1e-9)
+    gives a spurious Ritz pair: theta ~ 0 below every eigenvalue of A, a Ritz
+    vector of norm << 1 and a residual far below any tolerance -- it would
+    pass the convergence test.  The two guards of the FP64 phase: the start
+    block is orthonormalised first (the block Gram-Schmidt modelled above,
+    1e-8 drop), and converged roots must have |z| = 1 to 1e-10."""
+    rng = np.random.default_rng(11)
+    dim, m, n = 400, 12, 3
+    q, _ = np.linalg.qr(rng.standard_normal((dim, dim)))
+    lam = np.sort(rng.uniform(0.2, 1.5, dim))
+    a = (q * lam) @ q.T
+    basis = _basis(rng, dim, m)
+    basis[:, 5] = basis[:, 2] + 1e-9 * basis[:, 7]       # nearly dependent pair
+    basis[:, 5] /= np.linalg.norm(basis[:, 5])
+    theta, z, res = _rayleigh_ritz(a, basis, n)
+    spurious = theta < lam[0] - 1e-3
+    assert spurious.any()                                  # theta ~ 0: below the spectrum
+    norms = np.linalg.norm(z, axis=0)
+    assert (res[spurious] < 1e-5).all()                    # "converged" by the residual test alone
+    assert (np.abs(norms[spurious] - 1) > 1e-10).all()     # the norm guard catches it
+    # orthonormalised start block: columns one at a time against the kept ones
+    kept, _ = reference_loop(np.zeros((dim, 0)), basis.T)
+    ortho = kept.T
+    assert ortho.shape[1] == m - 1 or np.abs(ortho.T @ ortho - np.eye(ortho.shape[1])).max() < 1e-12
+    theta2, z2, res2 = _rayleigh_ritz(a, ortho, n)
+    assert (theta2 >= lam[0] - 1e-12).all()                # Cauchy interlacing: no spurious root
+    assert np.abs(np.linalg.norm(z2, axis=0) - 1).max() < 1e-10

@@ -77,6 +77,8 @@ size_t gs_block_scratch_doubles(int nc);
 void launch_gs_gram(const double* Gt, int nc, double* Tq, double* wn, int* count, hipStream_t s);
 void launch_copy(const double* src, double* dst, int64_t n, hipStream_t s);
 void launch_fill(double* dst, double v, int64_t n, hipStream_t s);
+// X[r][pos[r]] = 1 for r < n (rows of length L; pos on the device)
+void launch_set_ones(double* X, int64_t L, const int64_t* pos, int n, hipStream_t s);
 // zero columns [c0, c0 + cw) of `rows` rows of length ld (elements of `esz`
 // bytes, esz even): the never-written pad rows of the exchange intermediate
 void launch_zero_runs(void* base, int64_t rows, int64_t ld, int64_t c0, int64_t cw, int esz, hipStream_t s);

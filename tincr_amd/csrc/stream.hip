@@ -633,6 +633,17 @@ void launch_fill(double* dst, double v, int64_t n, hipStream_t s) {
   TDA_LAUNCH_CHECK();
 }
 
+// X[r][pos[r]] = 1 for r < n (unit guess vectors; rows of length L)
+__global__ void set_ones_kernel(double* __restrict__ X, int64_t L, const int64_t* __restrict__ pos, int n) {
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  if (r < n) X[(int64_t)r * L + pos[r]] = 1.0;
+}
+void launch_set_ones(double* X, int64_t L, const int64_t* pos, int n, hipStream_t s) {
+  if (n == 0) return;
+  set_ones_kernel<<<(unsigned)ceil_div(n, 64), 64, 0, s>>>(X, L, pos, n);
+  TDA_LAUNCH_CHECK();
+}
+
 __global__ void zero_words_kernel(uint32_t* __restrict__ dst, int64_t nw) {
   for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nw; t += (int64_t)gridDim.x * 256) dst[t] = 0u;
 }

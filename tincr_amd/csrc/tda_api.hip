@@ -24,14 +24,20 @@ using namespace tda;
 
 namespace {
 
+// Stream of the context whose C-ABI call runs on this thread (set by
+// guarded()): the only stream that can still read a buffer being regrown.
+thread_local hipStream_t tl_stream = nullptr;
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
   // Grows the buffer; returns true when (re)allocated.  Work already queued on
-  // `stream` may still read the old allocation, so the stream is drained
-  // before it is freed.
+  // the owning context's stream may still read the old allocation, so that
+  // stream (not the whole device: other contexts' workers keep running) is
+  // drained before it is freed.
   bool ensure(size_t n, hipStream_t stream = nullptr) {
     if (n <= bytes) return false;
+    if (!stream) stream = tl_stream;
     if (p) TDA_HIP(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize());
     release();
     if (n == 0) return false;
@@ -145,6 +151,7 @@ struct tda_ctx {
   DevBuf ws_gemm, T, xlp, u, uu, vv32, ax_tmp, stage, stage2;
   // Davidson state
   DevBuf V, AV, H, Y, evals, RZ, R, PW, norms, npart, idx, count, gs_scratch, eigw;
+  DevBuf guess;  // int64 positions of the n guess unit vectors in the padded layout
   // P (preconditioned vectors) and W (their projections; the kept Ritz
   // vectors between phases) share one allocation, W = P + pw_rows L, so that
   // one GEMM reads both as the rows of [P; W]
@@ -154,6 +161,7 @@ struct tda_ctx {
   bool gs_gram = true;  // block Gram-Schmidt in coefficient form (launch_gs_gram)
   int gs_fallbacks = 0;  // blocks that fell back to the vector-by-vector kernel
   int reorth_passes = 0;  // second passes taken (log only)
+  int norm_restarts = 0;  // FP64 verifications that found non-unit Ritz vectors
   // Pinned host staging of the Davidson's per-iteration scalars (a copy to
   // pageable memory would wait for the stream): [0, 64) eigenvalues,
   // [64, 128) residual norms, [128, 192) |p_j|, [192, 256) |u_j|,
@@ -191,7 +199,7 @@ struct tda_ctx {
     for (DevBuf* b : {&C, &S, &SC, &orbe, &aooff, &gamma, &gamma_lr, &qov, &qoo, &qvv, &G, &de, &qov32,
                       &qvv_amax, &G_amax, &ws_gemm, &T, &xlp, &u, &uu, &vv32, &ax_tmp,
                       &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &PW, &norms, &npart, &idx, &count,
-                      &gs_scratch, &eigw, &ao_atom, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
+                      &gs_scratch, &eigw, &guess, &ao_atom, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
                       &Cl, &Sl, &SCl, &Shl, &aooff_l, &ctW, &ctZ, &Hd, &dxyz, &post,
                       &adjP, &adjQ, &adjX, &adjD, &one, &zvw})
       b->release();
@@ -820,6 +828,9 @@ struct DavResult {
   int iters = 0;
   double max_err = 0;
   double delta = 0;  // measured product error of the phase's precision (0: not measured)
+  // a low-precision phase met a non-finite product (16-bit overflow): it
+  // stopped, and V[0, n) -- its last finite start vectors -- hold the restart
+  bool nonfinite = false;
 };
 
 // Runs the block Davidson with products in `prec` starting from the m0
@@ -912,11 +923,58 @@ int gs_block(tda_ctx* c, double* P, double* W, double* Q, int nc, double* wn) {
   return hc[0];
 }
 
+// Fills rows [0, n) of P with the guess unit vectors (their positions in the
+// padded layout are in c->guess, set by tda_run).
+void fill_guess(tda_ctx* c, double* P, int n) {
+  const int64_t L = c->ovp;
+  launch_fill(P, 0.0, (int64_t)n * L, c->stream);
+  launch_set_ones(P, L, c->guess.as<int64_t>(), n, c->stream);
+}
+
+// The start block of the FP64 phase, orthonormalised in FP64: the nc rows at
+// src (the previous phase's Ritz vectors) go through the block Gram-Schmidt
+// (1e-8 drop) into V[0, m).  Ritz vectors of a nearly dependent
+// low-precision basis can have norms far from 1 (and Ritz values near 0 with
+// tiny residuals); orthonormalised, every FP64 residual of the next
+// Rayleigh-Ritz step is that of a unit vector.  When fewer than n vectors
+// survive, the guess unit vectors are projected against them and appended.
+// Returns m >= n.
+int orthonormal_start(tda_ctx* c, const double* src, int nc, int n) {
+  const int64_t L = c->ovp;
+  double* V = c->V.as<double>();
+  double* P = c->Pb();
+  double* W = c->Wb();
+  TDA_CHECK(nc <= c->pw_rows && n <= c->pw_rows, TDA_ERR_BAD_ARG, "orthonormal_start: block too large");
+  if (src == W) {
+    launch_copy(W, P, (int64_t)nc * L, c->stream);
+  } else {
+    if (src != P) launch_copy(src, P, (int64_t)nc * L, c->stream);
+    launch_copy(P, W, (int64_t)nc * L, c->stream);
+  }
+  int m = gs_block(c, P, W, V, nc, nullptr);
+  for (int pass = 0; pass < 2 && m < n; ++pass) {
+    fill_guess(c, P, n);
+    double* C0 = c->stage2.as<double>();
+    launch_copy(P, W, (int64_t)n * L, c->stream);
+    if (m > 0) {
+      GemmArgs g0 = gargs(V, P, C0, m, n, L, plain(L), plain(1), plain(1), plain(L), plain(n), plain(1));
+      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g0);
+      GemmArgs g1 = gargs(C0, V, W, n, L, m, plain(1), plain(n), plain(L), plain(1), plain(L), plain(1));
+      g1.alpha = -1.0;
+      g1.beta = 1.0;
+      run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, g1);
+    }
+    m += gs_block(c, P, W, V + (size_t)m * L, n, nullptr);
+  }
+  TDA_CHECK(m >= n, TDA_ERR_NOT_CONVERGED, "Davidson: no orthonormal start block of n vectors");
+  return std::min(m, c->space(n));
+}
+
 DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec, bool first_can_converge,
                    int stop_stall, std::vector<double>& h_evals, std::vector<double>& h_err, int cap_in = 0,
                    int keep = 0, int* kk_out = nullptr, double stall_factor = 0.5,
                    double stall_floor = 1e300, bool thick_restart = true, double floor_k = 0.0,
-                   bool reorth = false) {
+                   bool reorth = false, bool verify = false) {
   const int64_t L = c->ovp;
   const int cap = cap_in > 0 ? cap_in : 15 * n;  // max_space (davidson.rs:74)
   const int ldh = c->space(n);                    // leading dimension of H and Y
@@ -1016,6 +1074,15 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
               "w0=%.10f gs_fallbacks=%d reorth_passes=%d\n",
               prec, it + 1, cvd, n - cvd, dim_sub, m, sum, mx, h_evals[0], c->gs_fallbacks, c->reorth_passes);
     }
+    if (!finite && prec != PREC_FP64 && !c->dense) {
+      // a low-precision product overflowed: the phase ends here and the next
+      // (wider) precision restarts from the basis' first n vectors, which
+      // are finite (the phase's start vectors or its last restart's Ritz
+      // vectors: only products and what is derived from them went bad)
+      res.nonfinite = true;
+      if (kk_out) *kk_out = n;
+      return res;
+    }
     if (!finite)
       throw Error{TDA_ERR_HIP, "Davidson: non-finite residual norm or Ritz value (prec " + std::to_string(prec) +
                                    ", iteration " + std::to_string(it + 1) + ", basis " + std::to_string(m) + ")"};
@@ -1028,6 +1095,23 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
     res.max_err = max_err;
     const int roots_lft = n - roots_cvd;
     if (roots_lft == 0 && (it > 0 || first_can_converge)) {  // davidson.rs:121
+      if (verify) {
+        // the returned roots must be unit vectors: a residual below tol says
+        // nothing about a Ritz vector of norm << 1 (nearly dependent basis)
+        launch_norms(RZ, n, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
+        double* hn = c->pinned() + 128;
+        TDA_HIP(hipMemcpyAsync(hn, c->norms.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+        TDA_HIP(hipStreamSynchronize(c->stream));
+        bool unit = true;
+        for (int r = 0; r < n; ++r) unit = unit && std::fabs(hn[r] - 1.0) <= 1e-10;
+        if (!unit) {
+          ++c->norm_restarts;
+          if (c->verbose) fprintf(stderr, "tincr_tda davidson: Ritz vector norms off 1, restarting orthonormalised\n");
+          m = dim_sub = orthonormal_start(c, RZ, n, n);
+          n_done = 0;
+          continue;
+        }
+      }
       res.converged = true;
       finish(true);
       return res;
@@ -1184,6 +1268,11 @@ std::vector<int> ao_offsets(int n_atoms, const int32_t* opa) {
 
 template <typename F>
 int guarded(tda_ctx* c, F&& f) {
+  struct StreamScope {  // DevBuf regrowth drains this context's stream
+    hipStream_t keep;
+    explicit StreamScope(hipStream_t s) : keep(tl_stream) { tl_stream = s; }
+    ~StreamScope() { tl_stream = keep; }
+  } scope(c ? c->stream : nullptr);
   try {
     if (c) set_device(c);
     f();
@@ -1785,6 +1874,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     TDA_CHECK(!c->dense || (!q_trans && !tr_dipoles && !f), TDA_ERR_BAD_ARG,
               "a dense engine has no q_trans / tr_dipoles / f");
     c->reset_counters();
+    c->last_roots = 0;  // RZ is about to be overwritten: no stale CI vectors for the adjoint
     const auto tq0 = std::chrono::steady_clock::now();
     prepare(c);
     const double tq = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tq0).count();
@@ -1801,15 +1891,14 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     std::partial_sort(order.begin(), order.begin() + n, order.end(), [&](int64_t a, int64_t b) {
       return c->h_de[a] < c->h_de[b] || (c->h_de[a] == c->h_de[b] && a < b);
     });
-    launch_fill(c->V.as<double>(), 0.0, (int64_t)n * L, c->stream);
-    std::vector<double> one(1, 1.0);
-    for (int r = 0; r < n; ++r) {
-      const int64_t ia = order[r];
-      const int64_t pos = (ia / c->v) * c->vp + (ia % c->v);
-      TDA_HIP(hipMemcpyAsync(c->V.as<double>() + (size_t)r * L + pos, one.data(), 8, hipMemcpyHostToDevice,
-                             c->stream));
+    {
+      std::vector<int64_t> pos(n);
+      for (int r = 0; r < n; ++r) pos[r] = (order[r] / c->v) * c->vp + (order[r] % c->v);
+      c->guess.ensure((size_t)n * sizeof(int64_t));
+      TDA_HIP(hipMemcpyAsync(c->guess.p, pos.data(), (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+      fill_guess(c, c->V.as<double>(), n);
+      TDA_HIP(hipStreamSynchronize(c->stream));  // pos goes out of scope
     }
-    TDA_HIP(hipStreamSynchronize(c->stream));
     std::vector<double> h_evals, h_err;
     int fp32_iters = 0, p32 = PREC_FP32;
     if (fp64_only) {
@@ -1825,15 +1914,21 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       // The FP16 / BF16 phase uses a thick restart (basis cap lp_cap, keep
       // lp_keep Ritz vectors) and hands its lp_keep lowest Ritz vectors to the
       // FP32 phase, which follows the reference rule.
+      // A phase that met a non-finite product (16-bit overflow) hands V[0, n)
+      // -- finite, orthonormal -- to the next one, and the 16-bit types are
+      // not used again in this solve (the FP16 hi / lo split has the range of
+      // FP16): FP16 / BF16 -> FP32 MFMA products -> FP64.
       int used = 0;
-      bool done = false;
+      bool done = false, lp_overflow = false;
       int start = n, kk = n;
       if ((c->prec == PREC_FP16 || c->prec == PREC_BF16) && c->lp_phase) {
         res = davidson(c, n, n, max_iter, tol, c->prec, false, lp_stall_window(c), h_evals, h_err, c->lp_cap(n),
                        c->lp_keep(n), &kk, 0.8, 1e300, true, c->floor_k, true);
         used += res.iters;
         done = res.converged;
-        if (kk > n) launch_copy(c->Wb(), c->V.as<double>(), (int64_t)kk * L, c->stream);
+        lp_overflow = res.nonfinite;
+        if (res.nonfinite) kk = n;  // V[0, n) already holds the restart
+        else if (kk > n) launch_copy(c->Wb(), c->V.as<double>(), (int64_t)kk * L, c->stream);
         else launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
         start = kk;
       }
@@ -1841,24 +1936,32 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       // exports its lp_keep lowest Ritz vectors so that the FP64 phase does
       // not restart from a thin basis.
       int start64 = n;
+      const double* src64 = c->RZ.as<double>();
       if (!done) {
         int kk32 = n;
         // the FP32-accuracy phase: FP16 hi / lo split products when the
         // context runs FP16 / BF16 on the AO operator, else FP32
         const bool ao_op = c->has_lc && c->op_kind == 0 && !c->ct;
-        p32 = (c->split32 && ao_op && (c->prec == PREC_FP16 || c->prec == PREC_BF16)) ? PREC_F16X : PREC_FP32;
+        p32 = (c->split32 && ao_op && !lp_overflow && (c->prec == PREC_FP16 || c->prec == PREC_BF16)) ? PREC_F16X
+                                                                                                        : PREC_FP32;
         DavResult r32 = davidson(c, n, start, std::max(1, max_iter - used), tol, p32, used > 0,
                                  3 * lp_stall_window(c), h_evals, h_err, c->fp32_cap(n), c->lp_keep(n), &kk32, 0.8, 1e300,
                                  c->fp32_thick, c->floor_k, true);
         used += r32.iters;
         fp32_iters = r32.iters;
         res.iters = used;
-        if (!r32.converged && kk32 > n) start64 = kk32;
+        if (r32.nonfinite) src64 = c->V.as<double>();  // its start vectors, V[0, n)
+        else if (!r32.converged && kk32 > n) start64 = kk32, src64 = c->Wb();
       }
-      if (start64 > n) launch_copy(c->Wb(), c->V.as<double>(), (int64_t)start64 * L, c->stream);
-      else launch_copy(c->RZ.as<double>(), c->V.as<double>(), (int64_t)n * L, c->stream);
+      // FP64 verification / continuation from the low-precision Ritz vectors,
+      // orthonormalised in FP64 first; converged roots must be unit vectors
+      if (src64 == c->V.as<double>()) {
+        launch_copy(src64, c->Pb(), (int64_t)n * L, c->stream);
+        src64 = c->Pb();
+      }
+      start64 = orthonormal_start(c, src64, start64, n);
       res64 = davidson(c, n, start64, std::max(1, max_iter - used), tol, PREC_FP64, true, 0, h_evals, h_err, 0, 0,
-                       nullptr, 0.5, 1e300, false, 0.0, true);
+                       nullptr, 0.5, 1e300, false, 0.0, true, true);
       res.iters = used;
     }
     // outputs (tda/mod.rs:88-126)

@@ -148,12 +148,21 @@ class _TdaEngine:
         return self.n_occ * self.n_virt
 
     # -- run_tda (tda/mod.rs:74-131) ----------------------------------------
-    def run_tda(self, n_roots, max_iter, tolerance, copy_x=True):
+    def run_tda(self, n_roots, max_iter, tolerance, copy_x=True, x_out=None):
         """Solve; returns (omega, X).  copy_x=False leaves the CI vectors on
-        the device (X is None; tda_adjoint / adjoint() still use them)."""
+        the device (X is None; tda_adjoint / adjoint() still use them).
+        x_out: a C-contiguous float64 [ov, n_roots] array to receive X (e.g. a
+        view of page-locked memory, which the device copies into directly)."""
         N, ov = self.n_atoms, self.n_occ * self.n_virt
         omega = np.empty(n_roots)
-        X = np.empty((ov, n_roots)) if copy_x else None
+        if not copy_x:
+            X = None
+        elif x_out is not None:
+            X = x_out
+            if X.shape != (ov, n_roots) or X.dtype != np.float64 or not X.flags.c_contiguous:
+                raise ValueError(f"x_out must be a C-contiguous float64 array of shape {(ov, n_roots)}")
+        else:
+            X = np.empty((ov, n_roots))
         q_trans = np.empty((N, n_roots))
         tr_dip = np.empty((3, n_roots))
         f = np.empty(n_roots)
