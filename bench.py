@@ -104,24 +104,15 @@ def make_system(inp, precision, operator):
 
 
 def sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    from tincr_amd import _native as nat
+    nat.device_synchronize()
 
 
 def pinned_x(ov, n):
-    """Page-locked [ov, n] float64 host array (the device copies X into it
-    directly); a plain array when torch cannot pin."""
-    try:
-        import torch
-        if torch.cuda.is_available():
-            return torch.empty((ov, n), dtype=torch.float64, pin_memory=True).numpy()
-    except Exception:
-        pass
-    return np.empty((ov, n))
+    """Page-locked [ov, n] float64 host buffer (hipHostMalloc through the
+    library's HIP runtime; the device copies X into it directly)."""
+    from tincr_amd import _native as nat
+    return nat.PinnedArray((ov, n))
 
 
 def solve(sysg, n_states, max_iter, tol, copy_x=True, x_out=None):
@@ -364,7 +355,8 @@ def main():
     ov = len(inp["occ_indices"]) * len(inp["virt_indices"])
     s64 = make_system(inp, "fp64", args.operator)
     slp = make_system(inp, args.prec, args.operator) if args.prec != "fp64" else s64
-    xbuf = pinned_x(ov, n)
+    xpin = pinned_x(ov, n)
+    xbuf = xpin.array
 
     # plain FP64 path on the same GPU: one warm-up, then the median of
     # args.steps solves, each ending with omega, X, f in host memory

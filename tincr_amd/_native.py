@@ -124,6 +124,60 @@ def load():
     return lib
 
 
+_hip = None
+
+
+def hip_runtime():
+    """The HIP runtime libtincr_tda.so is linked against (same loaded
+    instance: libamdhip64.so.7 is resolved by soname after the library), for
+    page-locked host buffers and device-wide synchronisation."""
+    global _hip
+    if _hip is None:
+        load()
+        h = ctypes.CDLL("libamdhip64.so.7")
+        h.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+        h.hipHostMalloc.restype = ctypes.c_int
+        h.hipHostFree.argtypes = [ctypes.c_void_p]
+        h.hipHostFree.restype = ctypes.c_int
+        h.hipDeviceSynchronize.restype = ctypes.c_int
+        _hip = h
+    return _hip
+
+
+class PinnedArray:
+    """A float64 numpy array in page-locked host memory (hipHostMalloc): the
+    device copies into it directly.  `.array` is valid while this object
+    lives."""
+
+    def __init__(self, shape):
+        h = hip_runtime()
+        n = int(np.prod(shape))
+        self.ptr = ctypes.c_void_p()
+        rc = h.hipHostMalloc(ctypes.byref(self.ptr), max(n, 1) * 8, 0)
+        if rc != 0:
+            raise TdaError(TDA_ERR_OOM, f"hipHostMalloc of {n * 8} bytes failed ({rc})")
+        buf = (ctypes.c_double * max(n, 1)).from_address(self.ptr.value)
+        self.array = np.frombuffer(buf, dtype=np.float64, count=n).reshape(shape)
+
+    def close(self):
+        if self.ptr:
+            self.array = None
+            hip_runtime().hipHostFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_synchronize():
+    rc = hip_runtime().hipDeviceSynchronize()
+    if rc != 0:
+        raise TdaError(TDA_ERR_HIP, f"hipDeviceSynchronize failed ({rc})")
+
+
 def dptr(a):
     if a is None:
         return None
