@@ -88,6 +88,26 @@ int tda_trans_charges(tda_ctx* ctx, int n_atoms, int n_ao, const int32_t* orbs_p
                       const double* s, int occ_begin, int n_occ, int virt_begin, int n_virt, double* q_ov,
                       double* q_oo, double* q_vv);
 
+/* No counterpart in the reference: tda_trans_charges for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and outputs, but every
+ * array is a DEVICE pointer owned by the caller (orbs_per_atom int32
+ * [n_atoms], orbs and s [n_ao, n_ao], outputs in the reference layouts; NULL
+ * skips an output), and the call allocates nothing: S C and the AO offsets
+ * live in a fixed per-GPU scratch array of TDA_NOALLOC_SCRATCH_BYTES (held by
+ * each GPU the library runs on until the process ends), which limits
+ * n_ao^2 * 8 + 4 (n_atoms + 2) to that size (n_ao <= 3547): larger systems
+ * return TDA_ERR_BAD_ARG before any work.  Runs on the context's stream and
+ * returns when the outputs are written.  All calls on one GPU share its
+ * scratch array: the library serialises its own calls within a process;
+ * calls from different processes on one GPU must not overlap.  Counts that
+ * are negative or do not sum to n_ao give TDA_ERR_BAD_ARG (outputs then
+ * undefined; reads stay inside the n_ao rows). */
+#define TDA_NOALLOC_SCRATCH_BYTES ((size_t)96 << 20)
+int tda_trans_charges_noalloc(tda_ctx* ctx, int n_atoms, int n_ao, const int32_t* orbs_per_atom, const double* orbs,
+                              const double* s, int occ_begin, int n_occ, int virt_begin, int n_virt, double* q_ov,
+                              double* q_oo, double* q_vv);
+
 /* Replaces orbe_differences (src/excited_states/utils.rs:8-24):
  * omega[i*n_virt+a] = virt[a] - occ[i]. */
 int tda_orbe_differences(const double* occ, int n_occ, const double* virt, int n_virt, double* omega);

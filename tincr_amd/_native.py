@@ -24,7 +24,7 @@ OPERATOR = {"ao": 0, "charges": 1}
 # every symbol include/tincr_tda.h declares
 EXPORTS = [
     "tda_ctx_create", "tda_ctx_destroy", "tda_last_error", "tda_set_precision", "tda_set_operator",
-    "tda_trans_charges", "tda_orbe_differences", "tda_set_system", "tda_set_ct_pair", "tda_set_dense", "tda_prepare",
+    "tda_trans_charges", "tda_trans_charges_noalloc", "tda_orbe_differences", "tda_set_system", "tda_set_ct_pair", "tda_set_dense", "tda_prepare",
     "tda_clear_charges", "tda_get_charges", "tda_compute_products", "tda_precondition", "tda_run",
     "tda_adjoint", "tda_adjoint_nuclear", "tda_zvector",
     "tda_batch_create", "tda_batch_destroy", "tda_batch_last_error", "tda_batch_set_operator",
@@ -92,6 +92,7 @@ def load():
     lib.tda_set_precision.argtypes = [P, i]
     lib.tda_set_operator.argtypes = [P, i]
     lib.tda_trans_charges.argtypes = [P, i, i, I32, D, D, i, i, i, i, D, D, D]
+    lib.tda_trans_charges_noalloc.argtypes = [P, i, i, P, P, P, i, i, i, i, P, P, P]
     lib.tda_orbe_differences.argtypes = [D, i, D, i, D]
     lib.tda_set_system.argtypes = [P, i, i, I32, D, D, D, i, i, i, i, D, D, D]
     lib.tda_set_ct_pair.argtypes = [P, i, i, I32, D, D, D, i, i, i, i, I32, D, D, D, i, i, D, D, D, D, D, D]

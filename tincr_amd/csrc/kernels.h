@@ -77,6 +77,9 @@ size_t gs_block_scratch_doubles(int nc);
 void launch_gs_gram(const double* Gt, int nc, double* Tq, double* wn, int* count, hipStream_t s);
 void launch_copy(const double* src, double* dst, int64_t n, hipStream_t s);
 void launch_fill(double* dst, double v, int64_t n, hipStream_t s);
+// AO offsets [n_atoms + 1] from device per-atom AO counts, clamped to n_ao;
+// *bad = 1 when the counts are negative or do not sum to n_ao
+void launch_ao_offsets(const int32_t* opa, int n_atoms, int n_ao, int* off, int* bad, hipStream_t s);
 // X[r][pos[r]] = 1 for r < n (rows of length L; pos on the device)
 void launch_set_ones(double* X, int64_t L, const int64_t* pos, int n, hipStream_t s);
 // zero columns [c0, c0 + cw) of `rows` rows of length ld (elements of `esz`

@@ -633,6 +633,31 @@ void launch_fill(double* dst, double v, int64_t n, hipStream_t s) {
   TDA_LAUNCH_CHECK();
 }
 
+// AO offsets from per-atom AO counts on the device (one thread; n_atoms is
+// small): off[A + 1] = off[A] + opa[A], each clamped to [off[A], n_ao] so that
+// no later kernel can index past the n_ao rows; *bad = 1 when some count was
+// negative or the counts do not sum to n_ao.
+__global__ void ao_offsets_kernel(const int32_t* __restrict__ opa, int n_atoms, int n_ao, int* __restrict__ off,
+                                  int* __restrict__ bad) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int64_t sum = 0;
+  int b = 0, o = 0;
+  off[0] = 0;
+  for (int A = 0; A < n_atoms; ++A) {
+    const int c = opa[A];
+    if (c < 0) b = 1;
+    sum += c;
+    const int64_t next = (int64_t)o + (c > 0 ? c : 0);
+    o = (int)(next < n_ao ? next : n_ao);
+    off[A + 1] = o;
+  }
+  *bad = (b || sum != n_ao) ? 1 : 0;
+}
+void launch_ao_offsets(const int32_t* opa, int n_atoms, int n_ao, int* off, int* bad, hipStream_t s) {
+  ao_offsets_kernel<<<1, 64, 0, s>>>(opa, n_atoms, n_ao, off, bad);
+  TDA_LAUNCH_CHECK();
+}
+
 // X[r][pos[r]] = 1 for r < n (unit guess vectors; rows of length L)
 __global__ void set_ones_kernel(double* __restrict__ X, int64_t L, const int64_t* __restrict__ pos, int n) {
   const int r = blockIdx.x * 64 + threadIdx.x;

@@ -8,7 +8,9 @@
 //   exchange intermediate T  [k][v_p][NB*o_p]  per atom block of NB atoms
 // Pads are zero, so every dot product over ovp equals the one over o*v.
 #include <algorithm>
+#include <array>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1427,6 +1429,81 @@ int tda_trans_charges(tda_ctx* c, int n_atoms, int n_ao, const int32_t* opa, con
     }
     TDA_HIP(hipStreamSynchronize(c->stream));
     for (DevBuf* b : {&dC, &dS, &dSC, &dao, &dq}) b->release();
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free transition charges.  The only device memory beyond the
+// caller's inputs and outputs is `scratch` below: S C [n_ao][n_ao] FP64, the
+// AO offsets [n_atoms + 1] and a flag.  Each GPU the library runs on holds
+// its own copy of the array until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress returns for the current
+// device (the array's name is a host-side stand-in, not a device address).
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kScratchBytes = TDA_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;  // guards scratch_ptr and serialises the calls per process
+std::array<void*, kMaxDevices> scratch_ptr{};
+
+// Device address of `scratch` on the current device (set up on the first
+// call on that device, with it current).
+void* scratch_address(int device) {
+  TDA_CHECK(device >= 0 && device < kMaxDevices, TDA_ERR_BAD_ARG, "device index out of range");
+  if (!scratch_ptr[device]) {
+    void* ptr = nullptr;
+    TDA_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    TDA_CHECK(ptr != nullptr, TDA_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    scratch_ptr[device] = ptr;
+  }
+  return scratch_ptr[device];
+}
+}  // namespace
+
+extern "C" {
+
+int tda_trans_charges_noalloc(tda_ctx* c, int n_atoms, int n_ao, const int32_t* d_opa, const double* d_orbs,
+                              const double* d_s, int occ_begin, int n_occ, int virt_begin, int n_virt,
+                              double* d_q_ov, double* d_q_oo, double* d_q_vv) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] {
+    TDA_CHECK(n_atoms > 0 && n_ao > 0, TDA_ERR_BAD_ARG, "n_atoms and n_ao must be positive");
+    TDA_CHECK(n_occ > 0 && n_virt > 0, TDA_ERR_BAD_ARG, "n_occ and n_virt must be positive");
+    TDA_CHECK(occ_begin >= 0 && occ_begin + n_occ <= n_ao, TDA_ERR_BAD_ARG, "occupied range outside the MOs");
+    TDA_CHECK(virt_begin >= 0 && virt_begin + n_virt <= n_ao, TDA_ERR_BAD_ARG, "virtual range outside the MOs");
+    TDA_CHECK(d_opa && d_orbs && d_s, TDA_ERR_BAD_ARG, "NULL input array");
+    const size_t sc_bytes = (size_t)round_up((int64_t)n_ao * n_ao * 8, 256);
+    const size_t need = sc_bytes + ((size_t)n_atoms + 2) * sizeof(int);
+    TDA_CHECK(need <= kScratchBytes, TDA_ERR_BAD_ARG,
+              "tda_trans_charges_noalloc: n_ao = " + std::to_string(n_ao) + ", n_atoms = " +
+                  std::to_string(n_atoms) + " need " + std::to_string(need) + " scratch bytes, more than the " +
+                  std::to_string(kScratchBytes) + " available");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    unsigned char* base = (unsigned char*)scratch_address(c->device);
+    double* dSC = (double*)base;
+    int* off = (int*)(base + sc_bytes);
+    int* bad = off + n_atoms + 1;
+    launch_ao_offsets(d_opa, n_atoms, n_ao, off, bad, c->stream);
+    // SC = S C without split-K (a split would need a workspace)
+    GemmArgs g = gargs(d_s, d_orbs, dSC, n_ao, n_ao, n_ao, plain(n_ao), plain(1), plain(n_ao), plain(1), plain(n_ao),
+                       plain(1));
+    GemmPlan plan = gemm_plan(DT_F64, n_ao, n_ao, n_ao, 1, true, false, 1);
+    TDA_CHECK(gemm_workspace_bytes(DT_F64, plan, n_ao, n_ao, 1) == 0, TDA_ERR_HIP, "S C plan needs a workspace");
+    gemm_launch(DT_F64, DT_F64, EPI_STORE, true, false, g, plan, c->stream);
+    struct Blk { double* out; int pb, P, qb, Q; };
+    const Blk blks[3] = {{d_q_ov, occ_begin, n_occ, virt_begin, n_virt},
+                         {d_q_oo, occ_begin, n_occ, occ_begin, n_occ},
+                         {d_q_vv, virt_begin, n_virt, virt_begin, n_virt}};
+    for (const Blk& b : blks)
+      if (b.out) launch_qblock(d_orbs, dSC, n_ao, off, n_atoms, b.pb, b.P, b.P, b.qb, b.Q, b.Q, b.out, c->stream);
+    int h_bad = 0;
+    TDA_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    TDA_HIP(hipStreamSynchronize(c->stream));
+    TDA_CHECK(!h_bad, TDA_ERR_BAD_ARG, "orbs_per_atom: negative counts or sum != n_ao (outputs undefined)");
   });
 }
 
