@@ -28,6 +28,10 @@ DAT_FILES = [
     "S", "orbs_after_scc", "orbe_after_scc", "occupation", "orbs_per_atom",
     "gamma_atomwise", "gamma_atomwise_lc", "gamma_atomwise_grad",
     "gamma_atomwise_grad_lc", "distance_matrix",
+    # SCC pieces (src/scc/helpers.rs:302-518 tests): H0, reference and
+    # converged density matrices, H1, long-range exchange, charges, energies
+    "H0", "P0", "P_after_scc", "H1_after_scc", "g0_lr_ao", "lc_exact_exchange", "lc_exchange_energy",
+    "q_after_scc", "dq_after_scc", "E_band_structure", "E_coulomb_after_scc",
 ]
 
 

@@ -1,0 +1,175 @@
+"""Full-size parity of the benchmark configurations on the GPU.
+
+* C200H202 (402 atoms, 1002 AO, 501 x 501), 20 states, FP64, tol 1e-5: the
+  oracle's solve of the same seeded system is committed as a fixture
+  (tests/golden/make_oracle_fixture.py): omega within 1e-6 eV, the same
+  state order and Davidson iteration count, f within 1e-8, and the CI
+  vectors' projections on seeded vectors equal up to the sign per state.
+* (H2O)512 (1536 atoms, 3072 AO, 2048 x 1024, ov = 2,097,152), 16 states:
+  the FP64 and the FP16-ladder solves agree per state to 1 meV (f reported
+  and bounded); the returned X are orthonormal to 1e-10 and an independent
+  FP64 product of them (tda_compute_products) gives |A x_k - w_k x_k| below
+  the tolerance; the AO-factored FP64 operator equals the per-atom-charges
+  operator (q_oo 51.5 GB, q_vv 12.9 GB, G = gamma_lr q_oo 51.5 GB on the
+  device) on two vectors to 1e-12 relative -- two contractions of different
+  structure, each pinned to the oracle on the fixture molecules.
+* The batched FMO workload at full size: 1024 fragments + 2048 CT pairs,
+  8 states each: every item converges in FP64 and in the FP32 context, the
+  two agree per item to 1 meV, and 16 seeded items equal the oracle's solves.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import tda_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+HARTREE_TO_EV = orc.HARTREE_TO_EV
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _system(inp, precision="fp64", operator="ao"):
+    from tincr_amd.tda import TdaSystem
+    return TdaSystem(inp["n_atoms"], inp["orbs_per_atom"], inp["xyz"], inp["occ_indices"], inp["virt_indices"],
+                     inp["orbs"], inp["s"], inp["orbe"], inp["gamma"], inp["gamma_lr"], precision=precision,
+                     operator=operator)
+
+
+def _synthetic(system):
+    from tincr_amd import systems as sy
+    if system.startswith("polyene"):
+        mol = sy.polyene(int(system[len("polyene"):]), seed=0)
+    else:
+        mol = sy.water_cluster(int(system[len("water"):]), seed=0)
+    return sy.tda_inputs(mol, seed=0)
+
+
+def test_c1_polyene200_fp64_vs_oracle_fixture():
+    with open(os.path.join(GOLDEN, "oracle_polyene200_n20.json")) as fh:
+        ref = json.load(fh)
+    print("C1 polyene200 fp64 20 states", flush=True)
+    inp = _synthetic("polyene200")
+    sysg = _system(inp)
+    n = ref["n_roots"]
+    w, x = sysg.run_tda(n, ref["max_iter"], ref["tol"])
+    st = sysg.last_stats
+    f = sysg.properties["oscillator_strengths"]
+    sysg.close()
+    w_ref = np.array(ref["omega"])
+    assert np.abs(w - w_ref).max() * HARTREE_TO_EV < 1e-6
+    assert np.array_equal(np.argsort(w, kind="stable"), np.argsort(w_ref, kind="stable"))
+    assert st["n_iter"] == ref["n_iter"]
+    assert np.abs(f - np.array(ref["f"])).max() < 1e-8
+    rng = np.random.default_rng(ref["proj_seed"])
+    g = rng.standard_normal((len(ref["proj"][0]), x.shape[0]))
+    proj = (g @ x).T
+    pref = np.array(ref["proj"])
+    sign = np.sign(proj[:, 0] * pref[:, 0])
+    assert np.abs(proj * sign[:, None] - pref).max() < 1e-6 * np.abs(pref).max()
+
+
+def test_c2_water512_ladder_vs_fp64_and_residuals():
+    print("C2 water512 fp64 + fp16 ladder", flush=True)
+    inp = _synthetic("water512")
+    n, tol = 16, 1e-5
+    s64 = _system(inp, "fp64")
+    w64, x64 = s64.run_tda(n, 450, tol)
+    f64 = s64.properties["oscillator_strengths"].copy()
+    slp = _system(inp, "fp16")
+    w, x = slp.run_tda(n, 450, tol)
+    f = slp.properties["oscillator_strengths"].copy()
+    assert slp.last_stats["converged"] == 1 and slp.last_stats["split_fp32"] == 1
+    dw = np.abs(w - w64) * HARTREE_TO_EV * 1e3
+    df = np.abs(f - f64)
+    print(f"  max |dw| {dw.max():.3e} meV, max |df| {df.max():.3e}", flush=True)
+    assert dw.max() <= 1.0
+    assert df.max() < 1e-4
+    for xx in (x, x64):
+        assert np.abs(xx.T @ xx - np.eye(n)).max() < 1e-10
+    # independent FP64 products of the returned vectors
+    print("  C2 FP64 residuals", flush=True)
+    ax = s64.compute_products(x)
+    res = np.linalg.norm(ax - x * w[None, :], axis=0)
+    assert res.max() < tol
+    assert np.allclose(np.diag(x.T @ ax), w, rtol=0, atol=1e-9)
+    s64.close()
+    slp.close()
+
+
+def test_c2_water512_ao_operator_equals_charges_operator():
+    print("C2 water512 AO vs charges operator", flush=True)
+    inp = _synthetic("water512")
+    ov = len(inp["occ_indices"]) * len(inp["virt_indices"])
+    x = np.random.default_rng(5).standard_normal((ov, 2))
+    x /= np.linalg.norm(x, axis=0)
+    sa = _system(inp, "fp64", "ao")
+    ax_ao = sa.compute_products(x)
+    sa.close()
+    sc = _system(inp, "fp64", "charges")
+    ax_ch = sc.compute_products(x)
+    sc.close()
+    rel = np.linalg.norm(ax_ao - ax_ch, axis=0) / np.linalg.norm(ax_ch, axis=0)
+    print(f"  relative difference {rel.max():.3e}", flush=True)
+    assert rel.max() < 1e-12
+
+
+def _fmo():
+    from tincr_amd import systems as sy
+    frags = sy.fmo_fragments(1024, 0)
+    pairs = sy.fmo_pairs(frags, 2048)
+    return sy.fmo_inputs(frags, pairs, 0)
+
+
+def _batch(prec, mons, cts, n, tol):
+    from tincr_amd.tda import Fragment, TdaBatch
+    b = TdaBatch(prec, n_workers=8)
+    keys = ("n_atoms", "orbs_per_atom", "xyz", "occ_indices", "virt_indices", "orbs", "s", "orbe", "gamma")
+    for m in mons:
+        o, v = len(m["occ_indices"]), len(m["virt_indices"])
+        b.add_system(m["n_atoms"], m["orbs_per_atom"], m["xyz"], m["occ_indices"], m["virt_indices"], m["orbs"],
+                     m["s"], m["orbe"], m["gamma"], m["gamma_lr"], min(n, o * v), 150, tol)
+    for c in cts:
+        o, v = len(c["h"]["occ_indices"]), len(c["l"]["virt_indices"])
+        b.add_ct_pair(Fragment(**{k: c["h"][k] for k in keys}), Fragment(**{k: c["l"][k] for k in keys}),
+                      c["s_hl"], c["gamma_hl"], c["gamma_lr_hl"], min(n, o * v), 150, tol)
+    res = b.run()
+    b.close()
+    return res
+
+
+def test_c3_full_fmo_batch():
+    print("C3 full FMO batch: inputs", flush=True)
+    mons, cts = _fmo()
+    n, tol = 8, 1e-5
+    print("C3 fp64 batch", flush=True)
+    r64 = _batch("fp64", mons, cts, n, tol)
+    print("C3 fp32 batch", flush=True)
+    r32 = _batch("fp32", mons, cts, n, tol)
+    assert len(r64) == len(r32) == 3072
+    for a, b in zip(r64, r32):
+        assert a["status"] == 0 and b["status"] == 0
+        assert a["stats"]["converged"] == 1 and b["stats"]["converged"] == 1
+        assert np.abs(a["ci_eigenvalues"] - b["ci_eigenvalues"]).max() * HARTREE_TO_EV * 1e3 <= 1.0
+    # seeded sample against the oracle (reference-faithful engines)
+    print("C3 oracle sample", flush=True)
+    rng = np.random.default_rng(7)
+    for i in rng.choice(len(r64), 16, replace=False):
+        if i < len(mons):
+            d = mons[i]
+            o, v = len(d["occ_indices"]), len(d["virt_indices"])
+            q = orc.trans_charges(d["n_atoms"], d["orbs_per_atom"], d["orbs"], d["s"], d["occ_indices"],
+                                  d["virt_indices"])
+            eng = orc.SystemEngine(*q, d["gamma"], d["gamma_lr"], orc.orbe_differences(d["orbe"][:o], d["orbe"][o:]),
+                                   o, v, faithful=False)
+            xyz = d["xyz"]
+        else:
+            c = cts[i - len(mons)]
+            eng = orc.ct_pair_engine(c["h"], c["l"], c["s_hl"], c["gamma_hl"], c["gamma_lr_hl"])
+            o, v = eng.n_occ, eng.n_virt
+            xyz = np.concatenate([c["h"]["xyz"], c["l"]["xyz"]])
+        ref = orc.run_tda(eng, xyz, min(n, o * v), 150, tol)
+        assert np.abs(r64[i]["ci_eigenvalues"] - ref["omega"]).max() * HARTREE_TO_EV < 1e-6
+        assert r64[i]["stats"]["n_iter"] == ref["n_iter"]

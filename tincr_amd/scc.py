@@ -1,0 +1,455 @@
+"""LC-DFTB ground state: the SCC input generator of the excited-state path.
+
+Produces what System::prepare_tda reads -- MO coefficients, orbital
+energies, the AO overlap, gamma and gamma_lr -- for a molecule given by
+atomic numbers and coordinates (bohr), so that the TDA solves run on
+converged SCC orbitals.  Host code (numpy / scipy), outside the timed solve.
+
+Follows the reference:
+
+* parameters: confined / free pseudo atoms and Slater-Koster tables
+  (src/initialization/atom.rs:62-94, src/initialization/parameters.rs:186-250;
+  data written by params/extract_params.py), tables interpolated by
+  interpolating cubic splines (FITPACK splrep with s = 0, as rusty_fitpack::
+  splrep with default arguments; scipy's splrep is the same FITPACK routine);
+* H0 and S (src/scc/h0_and_s.rs:92-181) with the Slater-Koster rules for s
+  and p shells (src/param/slako_transformations.rs:35-57 and the l <= 1 cases
+  of slako_transformation);
+* gamma / gamma_lr (src/scc/gamma_approximation.rs:175-193, 313-336);
+* the restricted SCC loop with long-range exchange, Broyden mixing of the AO
+  Mulliken charges and level shifting (src/scc/scc_routine.rs:73-290,
+  src/scc/helpers.rs:63-256, src/scc/mixer/broyden.rs, src/scc/
+  level_shifting.rs, src/scc/fermi_occupation.rs:66-86, src/scc/
+  mulliken.rs:9-24).
+
+Deviation: beyond the last tabulated distance (10 bohr for every table
+here) the Slater-Koster integrals are zero.  The reference evaluates its
+spline there (up to PROXIMITY_CUTOFF = 30 bohr, src/defaults.rs:40); a cubic
+continued 20 bohr past its data grows to O(0.1-1) Ha, no fixture covers such
+distances (the fixture molecules are all under 10 bohr across), so that
+behaviour is neither pinned nor reproduced.
+"""
+import json
+import math
+import os
+from functools import lru_cache
+
+import numpy as np
+from scipy.interpolate import splev, splrep
+
+from . import systems as sy
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PARAMS = os.path.join(HERE, "params")
+SYMBOL = {1: "h", 6: "c", 7: "n", 8: "o"}
+PROXIMITY_CUTOFF = 30.0          # src/defaults.rs:40 (bohr)
+MAX_ITER = 250                   # src/defaults.rs:62
+SCF_CHARGE_CONV = 1.0e-5         # src/defaults.rs:64
+SCF_ENERGY_CONV = 1.0e-5         # src/defaults.rs:65
+HOMO_LUMO_TOL = 0.05             # src/defaults.rs:76
+HOMO_LUMO_SHIFT = 0.1            # src/defaults.rs:77
+BROYDEN_OMEGA0 = 1.0e-2          # src/defaults.rs:86-90
+BROYDEN_MIN_WEIGHT = 1.0
+BROYDEN_MAX_WEIGHT = 1.0e5
+BROYDEN_WEIGHT_FACTOR = 1.0e-2
+BROYDEN_MIXING_PARAMETER = 0.25
+
+
+# ---------------------------------------------------------------------------
+# parameters
+# ---------------------------------------------------------------------------
+@lru_cache(maxsize=None)
+def _atoms_json():
+    with open(os.path.join(PARAMS, "atoms.json")) as fh:
+        return json.load(fh)["atoms"]
+
+
+@lru_cache(maxsize=None)
+def atom_params(z):
+    """Valence shells of element z (atom.rs:62-94): list of (l, energy,
+    occupation per orbital), the Hubbard U and the valence electrons."""
+    a = _atoms_json()[SYMBOL[z]]
+    conf, free = a["confined"], a["free"]
+    shells, n_elec = [], 0
+    for i, j in zip(conf["valence_orbitals"], free["valence_orbitals"]):
+        i, j = int(i), int(j)
+        l_ = int(conf["angular_momenta"][i])
+        occ = conf["orbital_occupation"][i]
+        shells.append((l_, free["energies"][j], occ / (2 * l_ + 1)))
+        n_elec += int(occ)
+    return {"shells": shells, "hubbard": conf["hubbard_u"], "n_elec": n_elec}
+
+
+def valence_orbitals(z):
+    """(l, m, energy, occupation) per AO of element z, m = -l..l (atom.rs:75-79)."""
+    out = []
+    for l_, e, f in atom_params(z)["shells"]:
+        out += [(l_, m, e, f) for m in range(-l_, l_ + 1)]
+    return out
+
+
+class SlaterKosterTable:
+    """Splined S / H0 tables of an element pair (z1 <= z2), parameters.rs:191-246."""
+
+    def __init__(self, z1, z2):
+        name = f"{SYMBOL[z1]}_{SYMBOL[z2]}"
+        data = np.load(os.path.join(PARAMS, f"slako_{name}.npz"))
+        self.d = data["d"]
+        self.dmax = float(self.d[-1])
+        self.splines = {}
+        for key in data.files:
+            if key[0] in "sh" and key[1] == "_":
+                kind, _, _, idx = key.split("_")
+                self.splines[(kind, int(idx))] = splrep(self.d, data[key])
+
+    def eval(self, kind, idx, r):
+        """Integral `idx` of S (kind 's') or H0 ('h') at distances r; zero
+        beyond the table (see the module docstring)."""
+        r = np.asarray(r, dtype=np.float64)
+        v = splev(r, self.splines[(kind, idx)])
+        return np.where(r <= self.dmax, v, 0.0)
+
+
+@lru_cache(maxsize=None)
+def slako_table(z1, z2):
+    return SlaterKosterTable(min(z1, z2), max(z1, z2))
+
+
+def _sk_rules(tab, kind, r, x, y, z, l1, m1, l2, m2):
+    """slako_transformation (slako_transformations.rs:57-...) for l1, l2 <= 1:
+    direction cosines x, y, z from atom 1 to atom 2, integral indices
+    0 ss-sigma, 2 sp-sigma, 4 ps-sigma, 5 pp-pi, 6 pp-sigma."""
+    cos = {-1: y, 0: z, 1: x}
+    if l1 == 0 and l2 == 0:
+        return tab.eval(kind, 0, r)
+    if l1 == 0 and l2 == 1:
+        return cos[m2] * tab.eval(kind, 2, r)
+    if l1 == 1 and l2 == 0:
+        return cos[m1] * tab.eval(kind, 4, r)
+    t5, t6 = tab.eval(kind, 5, r), tab.eval(kind, 6, r)
+    if m1 == m2:
+        others = {-1: x ** 2 + z ** 2, 0: x ** 2 + y ** 2, 1: y ** 2 + z ** 2}[m1]
+        return others * t5 + cos[m1] ** 2 * t6
+    return cos[m1] * cos[m2] * (-t5 + t6)
+
+
+# ---------------------------------------------------------------------------
+# H0, S (h0_and_s.rs:92-181), gamma (gamma_approximation.rs)
+# ---------------------------------------------------------------------------
+def ao_layout(z):
+    """Per-AO atom index, (l, m), on-site energy and reference occupation."""
+    atom, lm, energy, occ = [], [], [], []
+    for a, za in enumerate(z):
+        for l_, m, e, f in valence_orbitals(int(za)):
+            atom.append(a)
+            lm.append((l_, m))
+            energy.append(e)
+            occ.append(f)
+    return np.array(atom), lm, np.array(energy), np.array(occ)
+
+
+def h0_and_s(z, xyz):
+    """H0 and S [n_ao, n_ao].  Pair elements are evaluated with the lighter
+    atom first (the table's order) and, for equal elements, the atom of lower
+    index first; on-site: H0 = orbital energy, S = 1, off-diagonal zero."""
+    z = np.asarray(z, dtype=int)
+    xyz = np.asarray(xyz, dtype=np.float64)
+    atom, lm, energy, _ = ao_layout(z)
+    n_ao = len(atom)
+    first = np.searchsorted(atom, np.arange(len(z)))
+    s = np.eye(n_ao)
+    h0 = np.diag(energy)
+    elems = sorted(set(int(v) for v in z))
+    for ia, za in enumerate(elems):
+        for zb in elems[ia:]:
+            idx_a = np.nonzero(z == za)[0]
+            idx_b = np.nonzero(z == zb)[0]
+            A, B = np.meshgrid(idx_a, idx_b, indexing="ij")
+            A, B = A.ravel(), B.ravel()
+            if za == zb:
+                keep = A < B
+                A, B = A[keep], B[keep]
+            diff = xyz[B] - xyz[A]                     # from the lighter (or lower-index) atom
+            r = np.sqrt(diff[:, 0] ** 2 + diff[:, 1] ** 2 + diff[:, 2] ** 2)
+            keep = r < PROXIMITY_CUTOFF
+            A, B, diff, r = A[keep], B[keep], diff[keep], r[keep]
+            if len(r) == 0:
+                continue
+            with np.errstate(invalid="ignore", divide="ignore"):
+                x, y, zz = (np.where(r > 0, diff[:, k] / r, 0.0 if k < 2 else 1.0) for k in range(3))
+            tab = slako_table(za, zb)
+            orbs_a = valence_orbitals(za)
+            orbs_b = valence_orbitals(zb)
+            for oa, (l1, m1, _, _) in enumerate(orbs_a):
+                for ob, (l2, m2, _, _) in enumerate(orbs_b):
+                    mu = first[A] + oa
+                    nu = first[B] + ob
+                    for kind, mat in (("s", s), ("h", h0)):
+                        val = _sk_rules(tab, kind, r, x, y, zz, l1, m1, l2, m2)
+                        mat[mu, nu] = val
+                        mat[nu, mu] = val
+    return s, h0
+
+
+def gamma_ao_wise(gamma_atomwise, atom):
+    """g0_ao[mu, nu] = gamma[A(mu), B(nu)] (gamma_approximation.rs:313-336)."""
+    return gamma_atomwise[np.ix_(atom, atom)]
+
+
+def hubbard_of(z):
+    return {int(v): atom_params(int(v))["hubbard"] for v in set(int(w) for w in z)}
+
+
+# ---------------------------------------------------------------------------
+# SCC pieces (scc/helpers.rs)
+# ---------------------------------------------------------------------------
+def density_matrix_ref(occ_ao):
+    return np.diag(occ_ao)                                     # helpers.rs:219-233
+
+
+def density_matrix(orbs, f):
+    """P = sum_a f_a C_a C_a^T over occupied a (helpers.rs:198-215)."""
+    f = np.asarray(f)
+    occ = np.nonzero(f > 0.0)[0]
+    c = orbs[:, occ]
+    return (c * f[occ][None, :]) @ c.T
+
+
+def construct_h1(atom, gamma, dq):
+    """h1[mu, nu] = (V_A(mu) + V_B(nu)) / 2, V = gamma dq (helpers.rs:235-258)."""
+    v = gamma @ dq
+    va = v[atom]
+    return 0.5 * (va[:, None] + va[None, :])
+
+
+def lc_exact_exchange(s, g0_lr_ao, dp):
+    """Long-range exchange Hamiltonian (helpers.rs:109-155):
+    H^x = -1/8 [ (g o (S dP)) S + its transpose + g o (S dP S) + S (dP o g) S ],
+    symmetrised."""
+    s_dp = s @ dp
+    tmp = (g0_lr_ao * s_dp) @ s
+    hx = tmp + tmp.T
+    hx = hx + g0_lr_ao * (s_dp @ s)
+    hx = hx + (s @ (dp * g0_lr_ao)) @ s
+    hx *= -0.125
+    return 0.5 * (hx + hx.T)
+
+
+def lc_exchange_energy(s, g0_lr_ao, p0, p):
+    """helpers.rs:157-170."""
+    dp = p - p0
+    e = np.sum((s @ (dp @ s)) * dp * g0_lr_ao)
+    e += np.sum((s @ dp) * (dp @ s) * g0_lr_ao)
+    return -0.125 * e
+
+
+def electronic_energy(p, p0, s, h0, dq, gamma, g0_lr_ao):
+    """helpers.rs:63-90: band structure + monopole Coulomb (+ LC exchange)."""
+    e = np.sum(p * h0) + 0.5 * dq @ (gamma @ dq)
+    if g0_lr_ao is not None:
+        e += lc_exchange_energy(s, g0_lr_ao, p0, p)
+    return e
+
+
+def mulliken(p, s, atom, n_atoms):
+    """q_A = sum_{mu in A} (S P)_{mu mu} (mulliken.rs:9-24)."""
+    q_ao = np.einsum("ij,ji->i", s, p)
+    return np.bincount(atom, weights=q_ao, minlength=n_atoms)
+
+
+def fermi_occupation_t0(orbe, n_elec):
+    """T = 0 closed shell: 2 electrons per orbital upwards (fermi_occupation.rs:66-86)."""
+    f = np.zeros(len(orbe))
+    left = float(n_elec)
+    for a in np.argsort(orbe, kind="stable"):
+        if left <= 0.0:
+            break
+        f[a] = min(2.0, left)
+        left -= 2.0
+    return f
+
+
+class BroydenMixer:
+    """Modified Broyden mixing of the AO charges (mixer/broyden.rs; J. Chem.
+    Phys. 152, 124101 (2020)), with the reference's restart when the
+    residual grows and its linear first step."""
+
+    def __init__(self, dim, maxiter=MAX_ITER):
+        self.maxiter = maxiter
+        self.dim = dim
+        self.it = -1
+        self._reset()
+
+    def _reset(self):
+        m = self.maxiter - 1
+        self.weights = np.zeros(m)
+        self.a_mat = np.zeros((m, m))
+        self.delta_q_old = np.zeros(self.dim)
+        self.q_old = np.zeros(self.dim)
+        self.df = np.zeros((self.dim, m))
+        self.uu = np.zeros((self.dim, m))
+
+    def next(self, q, delta_q):
+        q = np.array(q, dtype=np.float64)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            rel_change = np.linalg.norm(delta_q) / np.linalg.norm(self.delta_q_old)
+        if self.it > 0 and rel_change > 1.0:
+            self.it = 0
+            self._reset()
+        if self.it == -1:
+            self.q_old = delta_q.copy()
+            self.delta_q_old = delta_q.copy()
+            out = delta_q.copy()
+        elif self.it == 0:
+            self.q_old = q.copy()
+            self.delta_q_old = delta_q.copy()
+            out = q + delta_q * BROYDEN_MIXING_PARAMETER
+        elif self.it < self.maxiter - 1:
+            it = self.it
+            idx = it - 1
+            weight = math.sqrt(delta_q @ delta_q)
+            weight = BROYDEN_WEIGHT_FACTOR / weight if weight > BROYDEN_WEIGHT_FACTOR / BROYDEN_MAX_WEIGHT \
+                else BROYDEN_MAX_WEIGHT
+            weight = max(weight, BROYDEN_MIN_WEIGHT)
+            self.weights[idx] = weight
+            df_idx = delta_q - self.delta_q_old
+            inv_norm = 1.0 / math.sqrt(df_idx @ df_idx)
+            df_idx = df_idx * inv_norm
+            c = np.zeros(it)
+            for i in range(idx):
+                self.a_mat[i, idx] = self.df[:, i] @ df_idx
+                self.a_mat[idx, i] = self.a_mat[i, idx]
+                c[i] = self.weights[i] * (self.df[:, i] @ delta_q)
+            self.a_mat[idx, idx] = 1.0
+            c[idx] = self.weights[idx] * (df_idx @ delta_q)
+            w = self.weights[:it]
+            beta = w[:, None] * (w[None, :] * self.a_mat[:it, :it].T)
+            beta[np.diag_indices(it)] += BROYDEN_OMEGA0 ** 2
+            gamma = c @ np.linalg.inv(beta)
+            self.df[:, idx] = df_idx
+            self.uu[:, idx] = df_idx * BROYDEN_MIXING_PARAMETER + (q - self.q_old) * inv_norm
+            self.q_old = q.copy()
+            self.delta_q_old = delta_q.copy()
+            out = q + BROYDEN_MIXING_PARAMETER * delta_q
+            out = out - self.uu[:, :it] @ (w * gamma)
+        else:
+            raise RuntimeError("SCC did not converge")
+        self.it += 1
+        return out
+
+
+class SCCError(RuntimeError):
+    """scc_routine.rs:21-50."""
+
+
+def _eigh(h, n_occ_plus=None):
+    """Eigenpairs of a symmetric matrix (upper triangle, as eigh(UPLO::Upper))."""
+    from scipy.linalg import eigh
+    return eigh(h, lower=False, driver="evd", check_finite=False)
+
+
+def prepare_scc(z, xyz, lc=True, r_lr=sy.LONG_RANGE_RADIUS):
+    """RestrictedSCC::prepare_scc (scc_routine.rs:73-124): H0, S, gamma,
+    gamma_lr (None without LC), the reference density and the AO layout."""
+    z = np.asarray(z, dtype=int)
+    xyz = np.asarray(xyz, dtype=np.float64)
+    atom, _, _, occ_ao = ao_layout(z)
+    s, h0 = h0_and_s(z, xyz)
+    hub = hubbard_of(z)
+    return dict(z=z, atom=atom, occ_ao=occ_ao, n_elec=sum(atom_params(int(v))["n_elec"] for v in z), s=s, h0=h0,
+                gamma=sy.gamma_atomwise(z, xyz, 0.0, hubbard=hub),
+                gamma_lr=sy.gamma_atomwise(z, xyz, r_lr, hubbard=hub) if lc else None)
+
+
+def scc_loop(s, h0, gamma, gamma_lr, atom, occ_ao, n_elec, max_iter=MAX_ITER, charge_conv=SCF_CHARGE_CONV,
+             energy_conv=SCF_ENERGY_CONV, verbose=False):
+    """RestrictedSCC::run_scc (scc_routine.rs:127-290) for a closed shell at
+    T = 0 from given H0, S, gamma, gamma_lr (None: no long-range
+    correction).  Returns orbs, orbe, p, dq, occupation, energy (electronic
+    SCC energy; the repulsive energy is not included), n_iter, converged."""
+    atom = np.asarray(atom)
+    n_atoms = int(atom.max()) + 1
+    n_orbs = len(atom)
+    lc = gamma_lr is not None
+    g0_lr_ao = gamma_ao_wise(gamma_lr, atom) if lc else None
+    p0 = density_matrix_ref(occ_ao)
+    p = p0.copy()
+    dq = np.zeros(n_atoms)
+    dp = p - p0
+    q_ao = np.zeros(n_orbs)
+    orbs = np.zeros((n_orbs, n_orbs))
+    orbe = np.zeros(n_orbs)
+    f = np.zeros(n_orbs)
+    last_energy = 0.0
+    scf_energy = 0.0
+    diff_dq_max = 0.0
+    mixer = BroydenMixer(n_orbs)
+    # level shifter state (level_shifting.rs)
+    ls_on, ls_weight, ls_empty = False, 0.0, True
+    homo = n_elec // 2 - 1
+    lumo = homo + 1
+    # Loewdin: X = S^-1/2 (scc_routine.rs:165-167)
+    ev, u = np.linalg.eigh(s)
+    x = (u / np.sqrt(ev)[None, :]) @ u.T
+    converged = False
+    it = 0
+    for it in range(max_iter):
+        h = construct_h1(atom, gamma, dq) * s + h0
+        if lc:
+            h = h + lc_exact_exchange(s, g0_lr_ao, dp)
+        if ls_on:
+            if ls_empty:
+                ls_empty, ls_weight = False, 1.0
+            else:
+                if diff_dq_max < 1.0e5 * charge_conv:
+                    ls_weight *= 0.5
+                if diff_dq_max < 1.0e3 * charge_conv:
+                    ls_on = False
+            cv = orbs[:, lumo:]
+            h = h + (ls_weight * HOMO_LUMO_SHIFT) * (cv @ cv.T)
+        hp = x.T @ h @ x
+        orbe, cp = _eigh(hp)
+        orbs = x @ cp
+        f = fermi_occupation_t0(orbe, n_elec)
+        if not ls_on:
+            ls_on = (orbe[lumo] - orbe[homo]) < HOMO_LUMO_TOL
+        p = density_matrix(orbs, f)
+        q_ao_n = np.einsum("ij,ji->i", s, p)
+        delta_dq = q_ao_n - q_ao
+        diff_dq_max = math.sqrt(np.mean((q_ao - q_ao_n) ** 2))
+        if diff_dq_max < charge_conv and abs(last_energy - scf_energy) < energy_conv:
+            converged = True
+        q_ao = mixer.next(q_ao, delta_dq)
+        p = p * (q_ao / q_ao_n)[None, :]
+        dp = p - p0
+        dq = mulliken(dp, s, atom, n_atoms)
+        scf_energy = electronic_energy(p, p0, s, h0, dq, gamma, g0_lr_ao)
+        if verbose:
+            print(f"scc iter {it:3d} E = {scf_energy:.10f} dq_rms = {diff_dq_max:.3e}", flush=True)
+        if converged:
+            break
+        last_energy = scf_energy
+    return dict(orbs=orbs, orbe=orbe, p=p, dq=dq, occupation=f, energy=scf_energy, n_iter=it + 1,
+                converged=converged)
+
+
+def run_scc(z, xyz, lc=True, r_lr=sy.LONG_RANGE_RADIUS, verbose=False, **kw):
+    """prepare_scc + scc_loop; the result also carries s, h0, gamma, gamma_lr."""
+    prep = prepare_scc(z, xyz, lc, r_lr)
+    out = scc_loop(prep["s"], prep["h0"], prep["gamma"], prep["gamma_lr"], prep["atom"], prep["occ_ao"],
+                   prep["n_elec"], verbose=verbose, **kw)
+    out.update({k: prep[k] for k in ("s", "h0", "gamma", "gamma_lr", "atom", "n_elec")})
+    return out
+
+
+def tda_inputs_scc(mol, lc=True, verbose=False):
+    """TDA inputs (the dict of systems.tda_inputs) from converged LC-SCC
+    orbitals of a Molecule: occupied = the doubly occupied MOs."""
+    r = run_scc(mol.z, mol.xyz, lc=lc, verbose=verbose)
+    if not r["converged"]:
+        raise SCCError(f"SCC did not converge in {r['n_iter']} iterations")
+    occ = [i for i, v in enumerate(r["occupation"]) if v > 1.0]
+    virt = [i for i, v in enumerate(r["occupation"]) if v <= 1.0]
+    return dict(n_atoms=mol.n_atoms, orbs_per_atom=mol.orbs_per_atom, xyz=mol.xyz, occ_indices=occ,
+                virt_indices=virt, orbs=r["orbs"], s=r["s"], orbe=r["orbe"], gamma=r["gamma"],
+                gamma_lr=r["gamma_lr"], scc_iter=r["n_iter"], scc_energy=r["energy"])
