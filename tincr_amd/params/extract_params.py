@@ -9,10 +9,13 @@ Reads, as text, the reference's parameter files (RON):
 * ``src/param/slaterkoster/slako_tables/<a>_<b>.ron`` -- the distance grid
   ``d`` and the tabulated S and H0 Slater-Koster integrals ``s`` / ``h``,
   keyed by (l1, l2, i) where i is the integral index the transformation
-  rules use (src/param/slako_transformations.rs:57).
+  rules use (src/param/slako_transformations.rs:57);
+* ``src/param/basis_sets/sto-3g.json`` -- the STO-3G valence shell (the
+  last electron shell) of each element, which the Molden export writes as
+  the [GTO] section (src/io/basis_set.rs:38-67, :87-97).
 
 Nothing is executed; the numbers are parsed from the text.  Output:
-``atoms.json`` and ``slako_<a>_<b>.npz`` beside this script.
+``atoms.json``, ``slako_<a>_<b>.npz`` and ``sto3g.json`` beside this script.
 
     python tincr_amd/params/extract_params.py [REF_ROOT]
 """
@@ -101,6 +104,18 @@ def main():
                   indent=1)
     for pair in PAIRS:
         np.savez_compressed(os.path.join(HERE, f"slako_{pair}.npz"), **slako(pair))
+    with open(os.path.join(REF, "src/param/basis_sets/sto-3g.json")) as fh:
+        bse = json.load(fh)
+    z_of = {"h": "1", "c": "6", "n": "7", "o": "8"}
+    sto = {}
+    for el in ELEMENTS:
+        shell = bse["elements"][z_of[el]]["electron_shells"][-1]
+        sto[el] = {"angular_momentum": shell["angular_momentum"],
+                   "exponents": [float(x) for x in shell["exponents"]],
+                   "coefficients": [[float(x) for x in c] for c in shell["coefficients"]]}
+    with open(os.path.join(HERE, "sto3g.json"), "w") as fh:
+        json.dump({"source": "src/param/basis_sets/sto-3g.json (last shell per element)", "elements": sto}, fh,
+                  indent=1)
     print("wrote", HERE)
 
 

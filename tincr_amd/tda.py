@@ -178,6 +178,26 @@ class _TdaEngine:
                                oscillator_strengths=f)
         return omega, X
 
+    def tda_states(self, total_energy):
+        """The TdaStates summary System::run_tda prints (tda/new_mod.rs:52-142)
+        for the last run_tda with copy_x=True; total_energy: the ground-state
+        total energy (Ha)."""
+        from . import output
+        p = self.properties
+        x = p["ci_coefficients"]
+        return output.tda_states_text(total_energy, p["ci_eigenvalues"],
+                                      x.reshape(self.n_occ, self.n_virt, x.shape[1]),
+                                      p["oscillator_strengths"], p["tr_dipoles"])
+
+    def write_outputs(self, directory, total_energy=None):
+        """The files System::run_tda writes (full_energies.npy,
+        oscillator_strengths.npy; tda/mod.rs:118-119); returns the TdaStates
+        text when total_energy is given."""
+        from . import output
+        p = self.properties
+        output.write_tda_npy(directory, p["ci_eigenvalues"], p["oscillator_strengths"])
+        return None if total_energy is None else self.tda_states(total_energy)
+
     def tdm(self, idx):
         """Transition density of state idx as [n_occ, n_virt] (properties/getter.rs:130-148)."""
         return self.properties["ci_coefficients"][:, idx].reshape(self.n_occ, self.n_virt)
