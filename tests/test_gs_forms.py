@@ -144,3 +144,40 @@ def test_fp64_handover_model_rejects_nearly_dependent_basis():
     theta2, z2, res2 = _rayleigh_ritz(a, ortho, n)
     assert (theta2 >= lam[0] - 1e-12).all()                # Cauchy interlacing: no spurious root
     assert np.abs(np.linalg.norm(z2, axis=0) - 1).max() < 1e-10
+
+
+def _cgs_block(basis, p):
+    """One classical pass of a block against the basis, then in-block, 1e-8
+    drop (the reference loop); returns accepted vectors (columns), |p|, |u|."""
+    q, n_u = reference_loop(basis, p.T)
+    return q.T, np.linalg.norm(p, axis=0), n_u
+
+
+def test_orthogonality_loss_grows_by_p_over_u_and_the_tracked_second_pass_stops_it():
+    """Model of the Davidson expansion on a spectrum where every new
+    preconditioned residual lies largely in the basis (|u| ~ |p| / 30, as on
+    the SCC orbitals of (H2O)512): with one classical pass the basis loses
+    orthogonality by about that factor per block until it is dependent;
+    tracking eta <- (eta + eps) max |p|/|u| and taking a second pass once eta
+    would exceed 1e-10 (tda_api.hip, davidson) keeps it orthonormal."""
+    rng = np.random.default_rng(2)
+    dim, nb, blocks = 4000, 4, 12
+
+    def run(tracked):
+        v, _ = np.linalg.qr(rng.standard_normal((dim, nb)))
+        eta, worst = 1e-15, 0.0
+        for _ in range(blocks):
+            p = v @ rng.standard_normal((v.shape[1], nb)) * 30.0 + rng.standard_normal((dim, nb)) / np.sqrt(dim)
+            q, n_p, n_u = _cgs_block(v, p)
+            grow = max(a / b for a, b in zip(n_p, n_u) if b > 1e-8)
+            eta_new = (eta + 2.3e-16) * grow
+            if tracked and eta_new > 1e-10:
+                q, _, _ = _cgs_block(v, q)          # second pass
+            else:
+                eta = max(eta, eta_new)
+            v = np.concatenate([v, q], axis=1)
+            worst = np.abs(v.T @ v - np.eye(v.shape[1])).max()
+        return worst
+
+    assert run(tracked=False) > 1e-6          # one pass per block: orthogonality lost
+    assert run(tracked=True) < 1e-8          # bounded near eta_tol (the estimate is an order of magnitude)

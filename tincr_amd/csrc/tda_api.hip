@@ -167,14 +167,18 @@ struct tda_ctx {
   // Pinned host staging of the Davidson's per-iteration scalars (a copy to
   // pageable memory would wait for the stream): [0, 64) eigenvalues,
   // [64, 128) residual norms, [128, 192) |p_j|, [192, 256) |u_j|,
-  // [256, 320) as int: the expansion indices, [320, 322) as int: counts.
+  // [256, 320) as int: the expansion indices, [320, 322) as int: counts,
+  // [384, 448) norms of the vectors a Gram-Schmidt block accepted.
   double* hpin = nullptr;
   double* pinned() {
     if (!hpin) TDA_HIP(hipHostMalloc((void**)&hpin, 512 * sizeof(double), hipHostMallocDefault));
     return hpin;
   }
-  double reorth_ratio = 0.01;
-  bool lp_phase = true;  // the FP16 / BF16 phase of the ladder (off: start at the FP32-accuracy phase)  // second Gram-Schmidt pass below this |u_j| / |p_j| (ladder phases)
+  // second Gram-Schmidt pass: 0 never, 1 every block, 2 when the estimated
+  // loss of orthogonality of the basis would exceed eta_tol
+  int reorth_mode = 2;
+  double eta_tol = 1e-10;
+  bool lp_phase = true;  // the FP16 / BF16 phase of the ladder (off: start at the FP32-accuracy phase)
   // instrumentation
   bool timing = true;
   bool verbose = false;
@@ -856,8 +860,8 @@ struct DavResult {
 // norm), the size of the product error every residual of the phase carries,
 // and leaves once max residual < floor_k * delta: below that the computed
 // residuals no longer say how far the roots are from convergence.
-// reorth: a second Gram-Schmidt pass for the new basis vectors (the phases
-// of the precision ladder; the FP64 path keeps the reference's single pass).
+// reorth: a second Gram-Schmidt pass for a block of new basis vectors when
+// the estimated loss of orthogonality of the basis would exceed eta_tol.
 // Algorithmic flops of one single-vector product (AO form: the four GEMMs;
 // charges form: the three-stage exchange).
 double product_flops(const tda_ctx* c) {
@@ -904,11 +908,21 @@ int gs_block(tda_ctx* c, double* P, double* W, double* Q, int nc, double* wn) {
     TDA_HIP(hipStreamSynchronize(c->stream));
     if (wn) std::memcpy(wn, hwn, (size_t)nc * 8);
     if (!hc[1]) {
-      if (hc[0] > 0) {
-        GemmArgs q = gargs(Tq, W, Q, hc[0], L, nc, plain(nc), plain(1), plain(L), plain(1), plain(L), plain(1));
-        run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, q);
-      }
-      return hc[0];
+      const int nq = hc[0];
+      if (nq == 0) return 0;
+      GemmArgs q = gargs(Tq, W, Q, nq, L, nc, plain(nc), plain(1), plain(L), plain(1), plain(L), plain(1));
+      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, q);
+      // The Gram estimate of |u_j| is a difference of large terms when the
+      // block's vectors are nearly dependent among themselves (large
+      // coefficients T): the accepted q_j must come out unit vectors, else
+      // the block is redone vector by vector from the intact P and W.
+      launch_norms(Q, nq, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
+      double* hq = c->pinned() + 384;
+      TDA_HIP(hipMemcpyAsync(hq, c->norms.p, (size_t)nq * 8, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipStreamSynchronize(c->stream));
+      bool unit = true;
+      for (int j = 0; j < nq; ++j) unit = unit && std::fabs(hq[j] - 1.0) <= 1e-10;
+      if (unit) return nq;
     }
     ++c->gs_fallbacks;
   }
@@ -994,9 +1008,10 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
   double* W = c->Wb();
   int m = m0, n_done = 0, dim_sub = m0;
   const int dim_sub_origin = m0;
+  double eta = 1e-15;  // estimated loss of orthogonality of the basis
   DavResult res;
   std::vector<int> idx;
-  double best = 1e300;
+  double best = 1e300, prev_ritz_sum = 1e300;
   int stall = 0;
   bool rotated = false;
   int n_eig_last = n, m_last = m;
@@ -1134,9 +1149,18 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       finish(false);
       return res;
     }
+    // Progress of the Ritz values: while the sum of the n lowest still falls
+    // by more than n times the phase's product error per iteration, the
+    // subspace is still finding the low states (a dense spectrum, states
+    // below every orbital-energy difference) and a residual that does not
+    // shrink is no sign of the precision floor.
+    double ritz_sum = 0.0;
+    for (double e : h_evals) ritz_sum += e;
+    const bool ritz_moving = prev_ritz_sum - ritz_sum > n * std::max(res.delta, tol);
+    prev_ritz_sum = ritz_sum;
     if (stop_stall > 0 && max_err < stall_floor) {
-      if (max_err < stall_factor * best) {
-        best = max_err;
+      if (max_err < stall_factor * best || ritz_moving) {
+        best = std::min(best, max_err);
         stall = 0;
       } else if (++stall >= stop_stall) {
         finish(false);
@@ -1172,15 +1196,26 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
         }
         int h_count = gs_block(c, P, W, V + (size_t)m * L, nc, reorth ? wn.data() : nullptr);
         if (reorth) std::memcpy(pn.data(), c->pinned() + 128, nc * 8);  // gs_block synchronized the stream
-        // one classical pass leaves a kept vector q_j = u_j / |u_j| with
-        // components of about eps |p_j| / |u_j| along the basis: when some
-        // vector kept less than reorth_ratio of its norm through the
-        // projections the block gets a second pass (the criterion of Daniel,
-        // Gragg, Kaufman and Stewart, with their 1/sqrt(2) relaxed: at the
-        // default 0.01 the orthogonality after one pass is still ~1e-14)
+        // Orthogonality bookkeeping.  One classical pass against a basis
+        // whose columns are orthogonal to eta leaves a kept vector
+        // q_j = u_j / |u_j| with components of about (eta + eps) |p_j| / |u_j|
+        // along the basis, so the loss of orthogonality grows by the factor
+        // |p_j| / |u_j| with every block whose preconditioned residuals lie
+        // largely in the basis; a nearly dependent basis then returns
+        // spurious Ritz pairs near 0 with residuals that pass the test.  The
+        // block gets a second pass ("twice is enough": Daniel, Gragg,
+        // Kaufman and Stewart) once the estimate would exceed eta_tol; after
+        // it the new vectors are orthogonal to the basis to ~eps and the
+        // estimate stays where it was.
         bool need = false;
-        for (int j = 0; j < nc && reorth; ++j)
-          need = need || (wn[j] > 1e-8 && wn[j] < c->reorth_ratio * pn[j]);
+        if (reorth && c->reorth_mode != 0) {
+          double grow = 0.0;
+          for (int j = 0; j < nc; ++j)
+            if (wn[j] > 1e-8) grow = std::max(grow, pn[j] / wn[j]);
+          const double eta_new = (eta + 2.3e-16) * grow;
+          need = c->reorth_mode == 1 || eta_new > c->eta_tol;
+          if (!need) eta = std::max(eta, eta_new);
+        }
         if (need && h_count > 0) {
           ++c->reorth_passes;
           // second Gram-Schmidt pass for the accepted vectors Q ("twice is
@@ -1239,11 +1274,14 @@ void alloc_davidson(tda_ctx* c, int n, int keep) {
   c->R.ensure((size_t)n * L * 8);
   c->PW.ensure((size_t)2 * keep * L * 8);
   c->pw_rows = (int64_t)(c->PW.bytes / ((size_t)2 * L * 8));
-  c->norms.ensure((size_t)n * 8);
-  c->npart.ensure((size_t)n * 256 * 8);
+  // Gram-Schmidt blocks hold up to n new vectors, or the keep Ritz vectors
+  // that orthonormal_start passes through gs_block
+  const int nb = std::max(n, keep);
+  c->norms.ensure((size_t)nb * 8);
+  c->npart.ensure((size_t)nb * 256 * 8);
   c->idx.ensure((size_t)n * sizeof(int));
   c->count.ensure(sizeof(int) * 4);
-  c->gs_scratch.ensure(gs_block_scratch_doubles(n) * 8);
+  c->gs_scratch.ensure(gs_block_scratch_doubles(nb) * 8);
   c->eigw.ensure(eigh_work_doubles(cap) * 8);
   c->stage2.ensure((size_t)cap * n * 8);
 }
@@ -1338,7 +1376,9 @@ int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx**
     const char* elp = getenv("TINCR_TDA_LP_PHASE");
     if (elp) c->lp_phase = atoi(elp) != 0;
     const char* er = getenv("TINCR_TDA_REORTH");
-    if (er && atof(er) >= 0 && atof(er) <= 1) c->reorth_ratio = atof(er);
+    if (er && (atoi(er) == 0 || atoi(er) == 1)) c->reorth_mode = atoi(er);
+    const char* ee = getenv("TINCR_TDA_ETA_TOL");
+    if (ee && atof(ee) > 0) c->eta_tol = atof(ee);
   }
   if (hbm_budget_bytes) c->budget = hbm_budget_bytes;
   int rc = guarded(c, [&] {
@@ -1979,7 +2019,16 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     std::vector<double> h_evals, h_err;
     int fp32_iters = 0, p32 = PREC_FP32;
     if (fp64_only) {
-      res = davidson(c, n, n, max_iter, tol, PREC_FP64, false, 0, h_evals, h_err);
+      // the reference rules, plus two guards that change nothing where one
+      // classical Gram-Schmidt pass keeps the basis orthonormal: a second
+      // pass for a block in which some vector kept less than 1 % of its norm
+      // (a preconditioned residual lying almost in the basis: one pass then
+      // leaves it with components of ~eps |p| / |u| along the basis, and the
+      // Rayleigh-Ritz step of a nearly dependent basis returns spurious Ritz
+      // pairs near 0 whose residuals pass the test), and converged roots
+      // must be unit vectors
+      res = davidson(c, n, n, max_iter, tol, PREC_FP64, false, 0, h_evals, h_err, 0, 0, nullptr, 0.5, 1e300, false,
+                     0.0, true, true);
       res64 = res;
     } else {
       // Precision ladder.  Each low-precision phase measures the error of
