@@ -113,7 +113,11 @@ struct tda_ctx {
   // thick restart of the FP16 / BF16 Davidson phase, in multiples of n_roots
   // (environment TINCR_TDA_LP_CAP / TINCR_TDA_LP_KEEP); the FP32 and FP64
   // phases keep the reference rule (cap 15 n, collapse to n)
-  int lp_cap_mult = 15, lp_keep_mult = 2;
+  // FP16 / BF16 phase: basis cap 8 n (measured on SCC (H2O)512, 16 states:
+  // caps 6 / 7 / 8 / 10 / 12 / 15 / 20 n gave 720 / 652 / 608 / 663 / 811 /
+  // 842 / 873 ms per solve: the subspace work of a large basis costs more
+  // than the extra restarts)
+  int lp_cap_mult = 8, lp_keep_mult = 2;
   int fp32_cap_mult = 15;  // basis cap of the FP32-accuracy phase, multiples of n
   int lp_keep(int n) const { return std::min(64, std::max(n, lp_keep_mult * n)); }
   // rows of the basis / subspace buffers: the reference's 15 n (davidson.rs:74)
@@ -177,7 +181,7 @@ struct tda_ctx {
   // second Gram-Schmidt pass: 0 never, 1 every block, 2 when the estimated
   // loss of orthogonality of the basis would exceed eta_tol
   int reorth_mode = 2;
-  double eta_tol = 1e-10;
+  double eta_tol = 1e-8;
   bool lp_phase = true;  // the FP16 / BF16 phase of the ladder (off: start at the FP32-accuracy phase)
   // instrumentation
   bool timing = true;
