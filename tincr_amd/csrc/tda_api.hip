@@ -18,6 +18,9 @@
 #include <numeric>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "../../include/tincr_tda.h"
 #include "gemm.h"
 #include "kernels.h"
@@ -119,10 +122,10 @@ struct tda_ctx {
   // than the extra restarts)
   int lp_cap_mult = 8, lp_keep_mult = 2;
   int fp32_cap_mult = 15;  // basis cap of the FP32-accuracy phase, multiples of n
-  int lp_keep(int n) const { return std::min(64, std::max(n, lp_keep_mult * n)); }
+  int lp_keep(int n) const { return std::max(n, std::min(64, lp_keep_mult * n)); }
   // rows of the basis / subspace buffers: the reference's 15 n (davidson.rs:74)
-  // or a larger cap of the low-precision phases, at most the eigensolver's 1024
-  int space(int n) const { return std::min(1024, std::max(15, std::max(lp_cap_mult, fp32_cap_mult)) * n); }
+  // or a larger cap of the low-precision phases
+  int space(int n) const { return std::max(15, std::max(lp_cap_mult, fp32_cap_mult)) * n; }
   int lp_cap(int n) const { return std::min(space(n), std::max(lp_cap_mult * n, lp_keep(n) + n)); }
   int fp32_cap(int n) const { return std::min(space(n), std::max(fp32_cap_mult * n, lp_keep(n) + n)); }
   double pre_thr = 1e-4;
@@ -169,15 +172,29 @@ struct tda_ctx {
   int reorth_passes = 0;  // second passes taken (log only)
   int norm_restarts = 0;  // FP64 verifications that found non-unit Ritz vectors
   // Pinned host staging of the Davidson's per-iteration scalars (a copy to
-  // pageable memory would wait for the stream): [0, 64) eigenvalues,
-  // [64, 128) residual norms, [128, 192) |p_j|, [192, 256) |u_j|,
-  // [256, 320) as int: the expansion indices, [320, 322) as int: counts,
-  // [384, 448) norms of the vectors a Gram-Schmidt block accepted.
+  // pageable memory would wait for the stream).  Gram-Schmidt blocks (at most
+  // 64 vectors): [128, 192) |p_j|, [192, 256) |u_j|, [320, 322) as int:
+  // counts, [384, 448) norms of the accepted vectors; then per root (pin_n
+  // slots each): eigenvalues, residual norms, expansion indices (as int).
   double* hpin = nullptr;
+  int pin_n = 0;
+  void ensure_pinned(int n) {
+    if (hpin && n <= pin_n) return;
+    if (hpin) {
+      TDA_HIP(hipStreamSynchronize(stream));
+      (void)hipHostFree(hpin);
+      hpin = nullptr;
+    }
+    pin_n = std::max(64, n);
+    TDA_HIP(hipHostMalloc((void**)&hpin, (512 + 3 * (size_t)pin_n) * sizeof(double), hipHostMallocDefault));
+  }
   double* pinned() {
-    if (!hpin) TDA_HIP(hipHostMalloc((void**)&hpin, 512 * sizeof(double), hipHostMallocDefault));
+    if (!hpin) ensure_pinned(64);
     return hpin;
   }
+  double* pin_evals() { return pinned() + 512; }
+  double* pin_res() { return pinned() + 512 + pin_n; }
+  int* pin_idx() { return (int*)(pinned() + 512 + 2 * (size_t)pin_n); }
   // second Gram-Schmidt pass: 0 never, 1 every block, 2 when the estimated
   // loss of orthogonality of the basis would exceed eta_tol
   int reorth_mode = 2;
@@ -200,8 +217,13 @@ struct tda_ctx {
   std::vector<Rec> recs;
   std::vector<hipEvent_t> ev_pool;
 
+  // rocBLAS handle of the large-subspace eigensolver (created on first use)
+  rocblas_handle blas = nullptr;
+  DevBuf ev_e, ev_info;  // off-diagonal / info of rocsolver_dsyevd
+
   ~tda_ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (blas) (void)rocblas_destroy_handle(blas);
     if (hpin) (void)hipHostFree(hpin);
     for (auto e : ev_pool) (void)hipEventDestroy(e);
     for (auto& ls : lps)
@@ -209,7 +231,7 @@ struct tda_ctx {
     for (DevBuf* b : {&C, &S, &SC, &orbe, &aooff, &gamma, &gamma_lr, &qov, &qoo, &qvv, &G, &de, &qov32,
                       &qvv_amax, &G_amax, &ws_gemm, &T, &xlp, &u, &uu, &vv32, &ax_tmp,
                       &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &PW, &norms, &npart, &idx, &count,
-                      &gs_scratch, &eigw, &guess, &ao_atom, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
+                      &gs_scratch, &eigw, &guess, &ev_e, &ev_info, &ao_atom, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
                       &Cl, &Sl, &SCl, &Shl, &aooff_l, &ctW, &ctZ, &Hd, &dxyz, &post,
                       &adjP, &adjQ, &adjX, &adjD, &one, &zvw})
       b->release();
@@ -943,12 +965,104 @@ int gs_block(tda_ctx* c, double* P, double* W, double* Q, int nc, double* wn) {
   return hc[0];
 }
 
+// Lowest n eigenpairs of the m x m projection H (upper triangle of the
+// row-major [m][ldh] array; davidson.rs:93 eigh(UPLO::Upper)): evals [n]
+// ascending, Y [n][ldh] (row r = eigenvector r).  Subspaces of up to 1024
+// vectors and 64 roots run in the one-workgroup kernel of eigh.hip; larger
+// ones (many roots: the basis holds 15 per root) in rocsolver_dsyevd on a
+// copy (H's upper triangle is the column-major lower one; the column-major
+// eigenvectors are the rows of Y).
+void subspace_eigh(tda_ctx* c, const double* H, int ldh, int m, int n, double* evals, double* Y) {
+  if (m <= 1024 && n <= 64) {
+    launch_eigh(H, ldh, m, n, c->eigw.as<double>(), evals, Y, ldh, c->stream);
+    return;
+  }
+  if (!c->blas) {
+    if (rocblas_create_handle(&c->blas) != rocblas_status_success)
+      throw Error{TDA_ERR_HIP, "rocblas_create_handle failed"};
+  }
+  if (rocblas_set_stream(c->blas, c->stream) != rocblas_status_success)
+    throw Error{TDA_ERR_HIP, "rocblas_set_stream failed"};
+  const size_t mm = (size_t)m * ldh;
+  c->eigw.ensure(std::max(c->eigw.bytes, (mm + (size_t)ldh) * 8));
+  c->ev_e.ensure((size_t)ldh * 8);
+  c->ev_info.ensure(sizeof(rocblas_int));
+  double* A = c->eigw.as<double>();
+  double* D = A + mm;
+  launch_copy(H, A, (int64_t)mm, c->stream);
+  if (rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_lower, m, A, ldh, D, c->ev_e.as<double>(),
+                       c->ev_info.as<rocblas_int>()) != rocblas_status_success)
+    throw Error{TDA_ERR_HIP, "rocsolver_dsyevd failed"};
+  launch_copy(A, Y, (int64_t)n * ldh, c->stream);
+  launch_copy(D, evals, n, c->stream);
+}
+
+// Appends the nc <= 64 vectors in the rows [0, nc) of P to the basis
+// V[0, m): classical Gram-Schmidt against the basis (W = P - V (V^T P)), then
+// in-block against the accepted vectors in order with the 1e-8 drop
+// (davidson.rs:162-168).  Returns the number appended at V[m].  With
+// `reorth`, the loss of orthogonality eta of the basis is tracked: one
+// classical pass against a basis orthogonal to eta leaves a kept vector
+// q_j = u_j / |u_j| with components of about (eta + eps) |p_j| / |u_j| along
+// it, so the loss grows by |p_j| / |u_j| with every block whose preconditioned
+// residuals lie largely in the basis, and a nearly dependent basis returns
+// spurious Ritz pairs near 0 whose residuals pass the test.  The block gets a
+// second pass ("twice is enough": Daniel, Gragg, Kaufman and Stewart) once
+// the estimate would exceed eta_tol; after it the new vectors are orthogonal
+// to the basis to ~eps and eta stays where it was.
+int append_block(tda_ctx* c, int m, int nc, bool reorth, double& eta) {
+  const int64_t L = c->ovp;
+  double* V = c->V.as<double>();
+  double* P = c->Pb();
+  double* W = c->Wb();
+  double* C0 = c->stage2.as<double>();
+  launch_copy(P, W, (int64_t)nc * L, c->stream);
+  if (m > 0) {
+    GemmArgs g0 = gargs(V, P, C0, m, nc, L, plain(L), plain(1), plain(1), plain(L), plain(nc), plain(1));
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g0);
+    GemmArgs g1 = gargs(C0, V, W, nc, L, m, plain(1), plain(nc), plain(L), plain(1), plain(L), plain(1));
+    g1.alpha = -1.0;
+    g1.beta = 1.0;
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, g1);
+  }
+  std::vector<double> pn(nc), wn(nc);
+  if (reorth) {
+    launch_norms(P, nc, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
+    TDA_HIP(hipMemcpyAsync(c->pinned() + 128, c->norms.p, nc * 8, hipMemcpyDeviceToHost, c->stream));
+  }
+  double* Q = V + (size_t)m * L;
+  int h_count = gs_block(c, P, W, Q, nc, reorth ? wn.data() : nullptr);
+  if (reorth) std::memcpy(pn.data(), c->pinned() + 128, nc * 8);  // gs_block synchronized the stream
+  bool need = false;
+  if (reorth && c->reorth_mode != 0 && m > 0) {
+    double grow = 0.0;
+    for (int j = 0; j < nc; ++j)
+      if (wn[j] > 1e-8) grow = std::max(grow, pn[j] / wn[j]);
+    const double eta_new = (eta + 2.3e-16) * grow;
+    need = c->reorth_mode == 1 || eta_new > c->eta_tol;
+    if (!need) eta = std::max(eta, eta_new);
+  }
+  if (need && h_count > 0) {
+    ++c->reorth_passes;
+    GemmArgs r0 = gargs(V, Q, C0, m, h_count, L, plain(L), plain(1), plain(1), plain(L), plain(h_count), plain(1));
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, r0);
+    GemmArgs r1 = gargs(C0, V, Q, h_count, L, m, plain(1), plain(h_count), plain(L), plain(1), plain(L), plain(1));
+    r1.alpha = -1.0;
+    r1.beta = 1.0;
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, r1);
+    launch_copy(Q, P, (int64_t)h_count * L, c->stream);
+    launch_copy(Q, W, (int64_t)h_count * L, c->stream);
+    h_count = gs_block(c, P, W, Q, h_count, nullptr);
+  }
+  return h_count;
+}
+
 // Fills rows [0, n) of P with the guess unit vectors (their positions in the
 // padded layout are in c->guess, set by tda_run).
-void fill_guess(tda_ctx* c, double* P, int n) {
+void fill_guess(tda_ctx* c, double* P, int n, int first = 0) {
   const int64_t L = c->ovp;
   launch_fill(P, 0.0, (int64_t)n * L, c->stream);
-  launch_set_ones(P, L, c->guess.as<int64_t>(), n, c->stream);
+  launch_set_ones(P, L, c->guess.as<int64_t>() + first, n, c->stream);
 }
 
 // The start block of the FP64 phase, orthonormalised in FP64: the nc rows at
@@ -961,30 +1075,20 @@ void fill_guess(tda_ctx* c, double* P, int n) {
 // Returns m >= n.
 int orthonormal_start(tda_ctx* c, const double* src, int nc, int n) {
   const int64_t L = c->ovp;
-  double* V = c->V.as<double>();
   double* P = c->Pb();
-  double* W = c->Wb();
-  TDA_CHECK(nc <= c->pw_rows && n <= c->pw_rows, TDA_ERR_BAD_ARG, "orthonormal_start: block too large");
-  if (src == W) {
-    launch_copy(W, P, (int64_t)nc * L, c->stream);
-  } else {
-    if (src != P) launch_copy(src, P, (int64_t)nc * L, c->stream);
-    launch_copy(P, W, (int64_t)nc * L, c->stream);
+  int m = 0;
+  double eta = 1e-15;
+  for (int b0 = 0; b0 < nc; b0 += 64) {
+    const int nb = std::min(64, nc - b0);
+    if (src + (size_t)b0 * L != P) launch_copy(src + (size_t)b0 * L, P, (int64_t)nb * L, c->stream);
+    m += append_block(c, m, nb, true, eta);
   }
-  int m = gs_block(c, P, W, V, nc, nullptr);
   for (int pass = 0; pass < 2 && m < n; ++pass) {
-    fill_guess(c, P, n);
-    double* C0 = c->stage2.as<double>();
-    launch_copy(P, W, (int64_t)n * L, c->stream);
-    if (m > 0) {
-      GemmArgs g0 = gargs(V, P, C0, m, n, L, plain(L), plain(1), plain(1), plain(L), plain(n), plain(1));
-      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g0);
-      GemmArgs g1 = gargs(C0, V, W, n, L, m, plain(1), plain(n), plain(L), plain(1), plain(L), plain(1));
-      g1.alpha = -1.0;
-      g1.beta = 1.0;
-      run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, g1);
+    for (int b0 = 0; b0 < n && m < n; b0 += 64) {
+      const int nb = std::min(64, n - b0);
+      fill_guess(c, P, nb, b0);
+      m += append_block(c, m, nb, true, eta);
     }
-    m += gs_block(c, P, W, V + (size_t)m * L, n, nullptr);
   }
   TDA_CHECK(m >= n, TDA_ERR_NOT_CONVERGED, "Davidson: no orthonormal start block of n vectors");
   return std::min(m, c->space(n));
@@ -999,7 +1103,7 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
   const int cap = cap_in > 0 ? cap_in : 15 * n;  // max_space (davidson.rs:74)
   const int ldh = c->space(n);                    // leading dimension of H and Y
   TDA_CHECK(cap <= ldh, TDA_ERR_BAD_ARG,
-            "Davidson: a basis of " + std::to_string(cap) + " vectors exceeds the subspace eigensolver's 1024");
+            "Davidson: a basis of " + std::to_string(cap) + " vectors exceeds the subspace buffers");
   if (keep < n) keep = n;
   const bool thick = thick_restart && keep > n;
   double* V = c->V.as<double>();
@@ -1029,7 +1133,7 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       // the phase kept the reference rule (n eigenpairs per iteration): one
       // more eigensolve of the final projection for the export
       kk = std::min(keep, m_last);
-      launch_eigh(H, ldh, m_last, kk, c->eigw.as<double>(), c->evals.as<double>(), Y, ldh, c->stream);
+      subspace_eigh(c, H, ldh, m_last, kk, c->evals.as<double>(), Y);
     }
     if (kk <= n) return;
     GemmArgs gw = gargs(Y, V, W, kk, L, m_last, plain(ldh), plain(1), plain(L), plain(1), plain(L), plain(1));
@@ -1061,7 +1165,7 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
       n_done = m;
       // 2. subspace eigenproblem (davidson.rs:93-97)
-      launch_eigh(H, ldh, m, n_eig, c->eigw.as<double>(), c->evals.as<double>(), Y, ldh, c->stream);
+      subspace_eigh(c, H, ldh, m, n_eig, c->evals.as<double>(), Y);
       // 3.1 Ritz vectors RZ = Y V, 3.2 residuals R = Y AV - u RZ, 3.3 norms
       // (davidson.rs:101-107)
       GemmArgs gr = gargs(Y, V, RZ, n, L, m, plain(ldh), plain(1), plain(L), plain(1), plain(L), plain(1));
@@ -1072,12 +1176,11 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       launch_norms(R, n, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
       h_evals.resize(n);
       h_err.resize(n);
-      double* hp = c->pinned();
-      TDA_HIP(hipMemcpyAsync(hp, c->evals.p, n * 8, hipMemcpyDeviceToHost, c->stream));
-      TDA_HIP(hipMemcpyAsync(hp + 64, c->norms.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipMemcpyAsync(c->pin_evals(), c->evals.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipMemcpyAsync(c->pin_res(), c->norms.p, n * 8, hipMemcpyDeviceToHost, c->stream));
       TDA_HIP(hipStreamSynchronize(c->stream));
-      std::memcpy(h_evals.data(), hp, n * 8);
-      std::memcpy(h_err.data(), hp + 64, n * 8);
+      std::memcpy(h_evals.data(), c->pin_evals(), n * 8);
+      std::memcpy(h_err.data(), c->pin_res(), n * 8);
       float ms = 0;
       TDA_HIP(hipEventElapsedTime(&ms, t.a, t.b));
       c->t_products_ms += ms;
@@ -1120,7 +1223,7 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
         // the returned roots must be unit vectors: a residual below tol says
         // nothing about a Ritz vector of norm << 1 (nearly dependent basis)
         launch_norms(RZ, n, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
-        double* hn = c->pinned() + 128;
+        double* hn = c->pin_res();
         TDA_HIP(hipMemcpyAsync(hn, c->norms.p, n * 8, hipMemcpyDeviceToHost, c->stream));
         TDA_HIP(hipStreamSynchronize(c->stream));
         bool unit = true;
@@ -1178,68 +1281,19 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       const int nc = (int)idx.size();
       dim_sub += roots_lft;  // davidson.rs:159 (counts dropped vectors too)
       if (nc > 0) {
-        int* hidx = (int*)(c->pinned() + 256);  // not in flight: the last copy from it finished at a sync
+        int* hidx = c->pin_idx();  // not in flight: the last copy from it finished at a sync
         std::memcpy(hidx, idx.data(), nc * sizeof(int));
         TDA_HIP(hipMemcpyAsync(c->idx.p, hidx, nc * sizeof(int), hipMemcpyHostToDevice, c->stream));
-        launch_precondition_many(R, c->evals.as<double>(), c->idx.as<int>(), nc, c->de.as<double>(), L,
-                                 c->pre_thr, P, c->stream);
-        // classical Gram-Schmidt (davidson.rs:162-168): W = P - V (V^T P) ...
-        double* C0 = c->stage2.as<double>();
-        GemmArgs g0 = gargs(V, P, C0, m, nc, L, plain(L), plain(1), plain(1), plain(L), plain(nc), plain(1));
-        run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g0);
-        launch_copy(P, W, (int64_t)nc * L, c->stream);
-        GemmArgs g1 = gargs(C0, V, W, nc, L, m, plain(1), plain(nc), plain(L), plain(1), plain(L), plain(1));
-        g1.alpha = -1.0;
-        g1.beta = 1.0;
-        run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, g1);
-        // ... then against the accepted new vectors, in order, 1e-8 drop
-        std::vector<double> pn(nc), wn(nc);
-        if (reorth) {
-          launch_norms(P, nc, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
-          TDA_HIP(hipMemcpyAsync(c->pinned() + 128, c->norms.p, nc * 8, hipMemcpyDeviceToHost, c->stream));
+        // preconditioned residuals in blocks of at most 64, each projected
+        // against the basis as it stands (earlier blocks included): the
+        // reference's vector-by-vector classical Gram-Schmidt against the
+        // growing basis, in order (davidson.rs:162-168)
+        for (int b0 = 0; b0 < nc; b0 += 64) {
+          const int nb = std::min(64, nc - b0);
+          launch_precondition_many(R, c->evals.as<double>(), c->idx.as<int>() + b0, nb, c->de.as<double>(), L,
+                                   c->pre_thr, P, c->stream);
+          m += append_block(c, m, nb, reorth, eta);
         }
-        int h_count = gs_block(c, P, W, V + (size_t)m * L, nc, reorth ? wn.data() : nullptr);
-        if (reorth) std::memcpy(pn.data(), c->pinned() + 128, nc * 8);  // gs_block synchronized the stream
-        // Orthogonality bookkeeping.  One classical pass against a basis
-        // whose columns are orthogonal to eta leaves a kept vector
-        // q_j = u_j / |u_j| with components of about (eta + eps) |p_j| / |u_j|
-        // along the basis, so the loss of orthogonality grows by the factor
-        // |p_j| / |u_j| with every block whose preconditioned residuals lie
-        // largely in the basis; a nearly dependent basis then returns
-        // spurious Ritz pairs near 0 with residuals that pass the test.  The
-        // block gets a second pass ("twice is enough": Daniel, Gragg,
-        // Kaufman and Stewart) once the estimate would exceed eta_tol; after
-        // it the new vectors are orthogonal to the basis to ~eps and the
-        // estimate stays where it was.
-        bool need = false;
-        if (reorth && c->reorth_mode != 0) {
-          double grow = 0.0;
-          for (int j = 0; j < nc; ++j)
-            if (wn[j] > 1e-8) grow = std::max(grow, pn[j] / wn[j]);
-          const double eta_new = (eta + 2.3e-16) * grow;
-          need = c->reorth_mode == 1 || eta_new > c->eta_tol;
-          if (!need) eta = std::max(eta, eta_new);
-        }
-        if (need && h_count > 0) {
-          ++c->reorth_passes;
-          // second Gram-Schmidt pass for the accepted vectors Q ("twice is
-          // enough"): a single classical pass loses orthogonality when the
-          // preconditioned residuals lie nearly in the basis, and a nearly
-          // dependent basis shows up as spurious Ritz values near 0
-          double* Q = V + (size_t)m * L;
-          GemmArgs r0 = gargs(V, Q, C0, m, h_count, L, plain(L), plain(1), plain(1), plain(L), plain(h_count),
-                              plain(1));
-          run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, r0);
-          GemmArgs r1 = gargs(C0, V, Q, h_count, L, m, plain(1), plain(h_count), plain(L), plain(1), plain(L),
-                              plain(1));
-          r1.alpha = -1.0;
-          r1.beta = 1.0;
-          run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, r1);
-          launch_copy(Q, P, (int64_t)h_count * L, c->stream);
-          launch_copy(Q, W, (int64_t)h_count * L, c->stream);
-          h_count = gs_block(c, P, W, Q, h_count, nullptr);
-        }
-        m += h_count;
       }
     } else if (thick) {
       // thick restart: rotate basis and products onto the kk lowest Ritz vectors
@@ -1281,12 +1335,13 @@ void alloc_davidson(tda_ctx* c, int n, int keep) {
   // Gram-Schmidt blocks hold up to n new vectors, or the keep Ritz vectors
   // that orthonormal_start passes through gs_block
   const int nb = std::max(n, keep);
+  c->ensure_pinned(nb);
   c->norms.ensure((size_t)nb * 8);
   c->npart.ensure((size_t)nb * 256 * 8);
   c->idx.ensure((size_t)n * sizeof(int));
   c->count.ensure(sizeof(int) * 4);
   c->gs_scratch.ensure(gs_block_scratch_doubles(nb) * 8);
-  c->eigw.ensure(eigh_work_doubles(cap) * 8);
+  c->eigw.ensure(eigh_work_doubles(std::min(cap, 1024)) * 8);
   c->stage2.ensure((size_t)cap * n * 8);
 }
 
@@ -1989,8 +2044,7 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
   int rc = guarded(c, [&] {
     TDA_CHECK(c->has_system, TDA_ERR_STATE, "tda_run: no system loaded");
     const int64_t ov = (int64_t)c->o * c->v;
-    TDA_CHECK(n_roots >= 1 && n_roots <= 64 && n_roots <= ov, TDA_ERR_BAD_ARG,
-              "n_roots must be in 1..min(64, n_occ*n_virt)");
+    TDA_CHECK(n_roots >= 1 && n_roots <= ov, TDA_ERR_BAD_ARG, "n_roots must be in 1..n_occ*n_virt");
     TDA_CHECK(max_iter >= 1 && tol > 0.0, TDA_ERR_BAD_ARG, "bad max_iter / tolerance");
     TDA_CHECK(!c->dense || (!q_trans && !tr_dipoles && !f), TDA_ERR_BAD_ARG,
               "a dense engine has no q_trans / tr_dipoles / f");
