@@ -28,7 +28,11 @@ import os
 import sys
 import time
 
-import numpy as np
+# One OpenBLAS kernel set on every x86 host (set before numpy loads it): with
+# a fixed thread count the SCC input orbitals are then bit-identical across
+# machines, and so is the Davidson trajectory that depends on them.
+os.environ.setdefault("OPENBLAS_CORETYPE", "Haswell")
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)

@@ -199,6 +199,29 @@ def test_run_tda_many_roots(case, n):
     slp.close()
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_ao_pair_factor_table_matches_gather(monkeypatch, prec):
+    """The 16-bit density GEMM reads its gamma_lr factors from the AO-pair
+    table (FP32, built once per system) instead of gathering them per tile
+    through the AO -> atom map (TINCR_TDA_GFULL=0): the same FP32 factors, so
+    the same products and the same solve, bit for bit."""
+    from tincr_amd import systems as sy
+    m = _synthetic(sy.water_cluster, 16)
+    ov = len(m["occ_indices"]) * len(m["virt_indices"])
+    x = np.random.default_rng(9).standard_normal((ov, 5))
+    out = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("TINCR_TDA_GFULL", flag)
+        s = _system(m, precision=prec)
+        ax = s.compute_products(x)
+        w, _ = s.run_tda(6, 200, 1e-6)
+        out.append((ax, w, dict(s.last_stats)))
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2]["n_iter"] == out[1][2]["n_iter"]
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
 @pytest.mark.parametrize("op", ["ao", "charges"])
 def test_products_mixed_precision(prec, op):

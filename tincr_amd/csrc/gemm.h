@@ -49,6 +49,11 @@ struct GemmArgs {
   // (the lo x lo term, 2^-22 of the value, is dropped).  0 = not split.
   // c_lo != 0: the result is written split the same way, lo at offset c_lo.
   int64_t a_lo, b_lo, c_lo;
+  // EPI_HADAMARD in the 16-bit kernel: when set, the factor of element (m, n)
+  // is hfull[(m % hmod) * hld + (n % hmod)] (a precomputed FP32 table, e.g.
+  // gamma_lr expanded to AO pairs) instead of H gathered through hidx.
+  const float* hfull;
+  int64_t hld;
 };
 
 // Row / column index into H of the Hadamard epilogue.

@@ -263,7 +263,19 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
   int hrow[MI], hcol[NI][4];
   bool staged = false;
   float* ht = nullptr;
-  if constexpr (EPI == EPI_HADAMARD) {
+  if (EPI == EPI_HADAMARD && g.hfull) {
+    // precomputed factor table: the row / column offsets of each lane's
+    // elements; no staging, no barrier
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+      const int64_t m = min(m0 + wm0 + i * 16 + fr, g.M - 1);
+      hrow[i] = (int)(((uint32_t)m % (uint32_t)g.hmod) * (uint32_t)g.hld);
+    }
+#pragma unroll
+    for (int j = 0; j < NI; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) hcol[j][r] = (int)((uint32_t)min(n_first[j] + r, g.N - 1) % (uint32_t)g.hmod);
+  } else if constexpr (EPI == EPI_HADAMARD) {
     constexpr int NW = NT / 64;
     int* rslot = (int*)lds;   // [BM] slot of each tile row
     int* cslot = rslot + BM;  // [BN]
@@ -368,10 +380,25 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
       for (int j = 0; j < NI; ++j) {
         if (n_first[j] >= g.N) continue;
         float v[4];
+        if (g.hfull) {
+          const float* hf = g.hfull + hrow[i];
+          float f4[4];
+          if (hcol[j][0] % 4 == 0 && hcol[j][3] == hcol[j][0] + 3) {
+            const v4f t = *(const v4f*)(hf + hcol[j][0]);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int h = hrow[i] + hcol[j][r];
-          v[r] = alpha * acc[i][j][r] * (staged ? ht[h] : (float)g.hsrc[h]);
+            for (int r = 0; r < 4; ++r) f4[r] = t[r];
+          } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) f4[r] = hf[hcol[j][r]];
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) v[r] = alpha * acc[i][j][r] * f4[r];
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int h = hrow[i] + hcol[j][r];
+            v[r] = alpha * acc[i][j][r] * (staged ? ht[h] : (float)g.hsrc[h]);
+          }
         }
         put4(cmo, c_al, j, v);
       }
@@ -423,8 +450,13 @@ inline bool gemm16_ok(DType in, DType out, Epi epi, bool a_kc, bool b_kc, const 
   if (g.a_lo && (g.a_lo % 8 || g.b_lo % 8 || g.a_lo < g.K || g.b_lo < g.K)) return false;
   if (g.c_lo && epi == EPI_DIAG) return false;
   if (epi == EPI_DIAG) return out == DT_F64;
-  // EPI_HADAMARD: plain H maps with 32-bit offsets
-  if (epi == EPI_HADAMARD &&
+  // EPI_HADAMARD: a full factor table with 32-bit offsets, or plain H maps
+  // with 32-bit offsets
+  if (epi == EPI_HADAMARD && g.hfull &&
+      (g.hmod < 1 || g.hld < g.hmod || g.hld % 4 || (((uintptr_t)g.hfull) & 15) ||
+       (g.hmod - 1) * g.hld + g.hmod >= (INT64_C(1) << 31)))
+    return false;
+  if (epi == EPI_HADAMARD && !g.hfull &&
       (g.hm.div < PLAIN || g.hn.div < PLAIN || g.hn.s0 != 1 || g.hm.s0 < 0 || g.hm.s0 > 46340 ||
        (g.hidx && (g.hmod < 1 || g.hmod > INT32_MAX))))
     return false;

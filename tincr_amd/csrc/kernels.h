@@ -80,6 +80,8 @@ void launch_fill(double* dst, double v, int64_t n, hipStream_t s);
 // AO offsets [n_atoms + 1] from device per-atom AO counts, clamped to n_ao;
 // *bad = 1 when the counts are negative or do not sum to n_ao
 void launch_ao_offsets(const int32_t* opa, int n_atoms, int n_ao, int* off, int* bad, hipStream_t s);
+// out[mu][nu] = (float) g[atom[mu]][atom[nu]] (mu, nu < n_ao; g [N][N], out row stride ld)
+void launch_expand_pairs(const double* g, const int* atom, int n_ao, int N, int64_t ld, float* out, hipStream_t s);
 // X[r][pos[r]] = 1 for r < n (rows of length L; pos on the device)
 void launch_set_ones(double* X, int64_t L, const int64_t* pos, int n, hipStream_t s);
 // zero columns [c0, c0 + cw) of `rows` rows of length ld (elements of `esz`

@@ -658,6 +658,21 @@ void launch_ao_offsets(const int32_t* opa, int n_atoms, int n_ao, int* off, int*
   TDA_LAUNCH_CHECK();
 }
 
+// out[mu][nu] = (float) g[atom[mu]][atom[nu]] for mu, nu < n_ao (row stride
+// ld; g is [N][N]): an atom-pair table expanded to AO pairs
+__global__ void __launch_bounds__(256) expand_pairs_kernel(const double* __restrict__ g, const int* __restrict__ atom,
+                                                           int n_ao, int N, int64_t ld, float* __restrict__ out) {
+  const int mu = blockIdx.y;
+  const int64_t gr = (int64_t)atom[mu] * N;
+  for (int nu = blockIdx.x * 256 + threadIdx.x; nu < n_ao; nu += gridDim.x * 256)
+    out[(int64_t)mu * ld + nu] = (float)g[gr + atom[nu]];
+}
+void launch_expand_pairs(const double* g, const int* atom, int n_ao, int N, int64_t ld, float* out, hipStream_t s) {
+  if (n_ao == 0) return;
+  expand_pairs_kernel<<<dim3((unsigned)ceil_div(n_ao, 256), (unsigned)n_ao), 256, 0, s>>>(g, atom, n_ao, N, ld, out);
+  TDA_LAUNCH_CHECK();
+}
+
 // X[r][pos[r]] = 1 for r < n (unit guess vectors; rows of length L)
 __global__ void set_ones_kernel(double* __restrict__ X, int64_t L, const int64_t* __restrict__ pos, int n) {
   const int r = blockIdx.x * 64 + threadIdx.x;

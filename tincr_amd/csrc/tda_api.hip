@@ -149,6 +149,10 @@ struct tda_ctx {
   // AO-factored operands: Rt [2n_ao][v_p], Lm [2n_ao][o_p], Rpt [v_p][2n_ao],
   // Lpt [o_p][2n_ao]; the AO-expanded gamma_lr is gathered through ao_atom
   DevBuf ao_atom, aoRt, aoLm, aoRpt, aoLpt, aoXR, aoM, aoF;
+  // gamma_lr expanded to AO pairs, FP32 [n_ao][gfull_ld]: the Hadamard
+  // factors of the 16-bit density GEMM (coalesced loads, no gather)
+  DevBuf gfull;
+  int64_t gfull_ld = 0;
   // Low-precision operand copies, one set per precision (index PREC_*): the
   // precision ladder of tda_run uses the context's precision and FP32.
   struct LpSet {
@@ -168,6 +172,7 @@ struct tda_ctx {
   double* Pb() { return PW.as<double>(); }
   double* Wb() { return PW.as<double>() + pw_rows * ovp; }
   bool gs_gram = true;  // block Gram-Schmidt in coefficient form (launch_gs_gram)
+  bool no_gfull = false;  // TINCR_TDA_GFULL=0: 16-bit density GEMM gathers gamma_lr per tile instead
   int gs_fallbacks = 0;  // blocks that fell back to the vector-by-vector kernel
   int reorth_passes = 0;  // second passes taken (log only)
   int norm_restarts = 0;  // FP64 verifications that found non-unit Ritz vectors
@@ -231,7 +236,7 @@ struct tda_ctx {
     for (DevBuf* b : {&C, &S, &SC, &orbe, &aooff, &gamma, &gamma_lr, &qov, &qoo, &qvv, &G, &de, &qov32,
                       &qvv_amax, &G_amax, &ws_gemm, &T, &xlp, &u, &uu, &vv32, &ax_tmp,
                       &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &PW, &norms, &npart, &idx, &count,
-                      &gs_scratch, &eigw, &guess, &ev_e, &ev_info, &ao_atom, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
+                      &gs_scratch, &eigw, &guess, &ev_e, &ev_info, &gfull, &ao_atom, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
                       &Cl, &Sl, &SCl, &Shl, &aooff_l, &ctW, &ctZ, &Hd, &dxyz, &post,
                       &adjP, &adjQ, &adjX, &adjD, &one, &zvw})
       b->release();
@@ -473,6 +478,10 @@ void build_ao_operands(tda_ctx* c) {
   // the back-contraction pairs each D block with the other operator: (C; SC), stored transposed
   launch_ao_operand(C, SC, nao, c->vb, c->v, c->vp, 1, 1, ld, c->aoRpt.as<double>(), c->stream);
   launch_ao_operand(C, SC, nao, c->ob, c->o, c->op, 1, 1, ld, c->aoLpt.as<double>(), c->stream);
+  c->gfull_ld = round_up(nao, 4);
+  c->gfull.ensure((size_t)nao * c->gfull_ld * sizeof(float));
+  launch_expand_pairs(c->gamma_lr.as<double>(), c->ao_atom.as<int>(), nao, c->N, c->gfull_ld, c->gfull.as<float>(),
+                      c->stream);
 }
 
 void prepare(tda_ctx* c) {
@@ -683,6 +692,10 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
     g.hmod = nao;
     g.hm = plain(N);
     g.hn = plain(1);
+    if (lp && !c->no_gfull) {  // 16-bit kernel: the factors from the AO-pair table
+      g.hfull = c->gfull.as<float>();
+      g.hld = c->gfull_ld;
+    }
     run_gemm(c, dt, dt, EPI_HADAMARD, true, true, g, 0, e[1]);
   }
   // Coulomb: u = q_ov x from the D diagonals, w = gamma u, added to M
@@ -1432,6 +1445,8 @@ int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx**
     if (el && atof(el) >= 0) c->ladder_min_flops = atof(el);
     const char* eg = getenv("TINCR_TDA_GS_GRAM");
     if (eg) c->gs_gram = atoi(eg) != 0;
+    const char* egf = getenv("TINCR_TDA_GFULL");
+    if (egf) c->no_gfull = atoi(egf) == 0;
     const char* elp = getenv("TINCR_TDA_LP_PHASE");
     if (elp) c->lp_phase = atoi(elp) != 0;
     const char* er = getenv("TINCR_TDA_REORTH");

@@ -442,10 +442,21 @@ def run_scc(z, xyz, lc=True, r_lr=sy.LONG_RANGE_RADIUS, verbose=False, **kw):
     return out
 
 
-def tda_inputs_scc(mol, lc=True, verbose=False):
+def tda_inputs_scc(mol, lc=True, verbose=False, blas_threads=8):
     """TDA inputs (the dict of systems.tda_inputs) from converged LC-SCC
-    orbitals of a Molecule: occupied = the doubly occupied MOs."""
-    r = run_scc(mol.z, mol.xyz, lc=lc, verbose=verbose)
+    orbitals of a Molecule: occupied = the doubly occupied MOs.
+
+    The BLAS runs on `blas_threads` threads (None: the pool's default).
+    OpenBLAS results depend on its kernel set and thread count only, so with
+    both fixed (OPENBLAS_CORETYPE set before numpy is imported, as bench.py
+    does) the orbitals are the same to the last bit on every x86 host; the
+    iteration count of a mixed-precision Davidson on them then repeats too."""
+    if blas_threads:
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(limits=blas_threads, user_api="blas"):
+            r = run_scc(mol.z, mol.xyz, lc=lc, verbose=verbose)
+    else:
+        r = run_scc(mol.z, mol.xyz, lc=lc, verbose=verbose)
     if not r["converged"]:
         raise SCCError(f"SCC did not converge in {r['n_iter']} iterations")
     occ = [i for i, v in enumerate(r["occupation"]) if v > 1.0]
