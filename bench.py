@@ -184,15 +184,32 @@ def cpu_baseline(inp, n_states, n_iter, n_products, q_budget_bytes=2 << 30):
     t_call2 = time.perf_counter() - t0
     t_vec = max(t_call2 - t_g, 0.0) / 2.0
     t_solve = r * t_q + n_iter * r * r * t_g + n_products * r * t_vec
+    # tuned CPU arithmetic: the exchange of a call batched over its vectors
+    # (two GEMMs per call instead of two per vector) and gamma_lr.q_oo once
+    # per solve instead of once per call
+    eng_t = orc.SystemEngine(q_ov, q_oo, q_vv, gam, glr, omega, o, v, faithful=False)
+    xk = np.random.default_rng(1).standard_normal((o * v, 4))
+    t0 = time.perf_counter()
+    eng_t.apply(xk)
+    t_call4 = time.perf_counter() - t0
+    t_vec_t = max(t_call4 - t_g, 0.0) / 4.0
+    t_solve_t = r * t_q + r * r * t_g + n_products * r * t_vec_t
     what = "the whole system" if ns == n_at else f"the first {ns} of {n_at} atoms (overlap = identity on the " \
                                                   f"sample; scaled by N/N' = {r:.1f}, gamma_lr.q_oo by (N/N')^2)"
-    return {
+    faithful = {
         "value": n_states / t_solve, "unit": "states/s", "cores": int(cores), "kind": "port",
         "sample": (f"numpy FP64 oracle (reference-faithful: gamma_lr.q_oo per call, per-vector exchange) on "
                    f"{what}: trans_charges {t_q:.2f} s, gamma_lr.q_oo {t_g:.2f} s, {t_vec:.2f} s per A.x "
                    f"vector; scaled to the FP64 GPU solve's {n_iter} iterations / {n_products} products = "
                    f"{t_solve:.1f} s per solve"),
     }
+    tuned = {
+        "value": n_states / t_solve_t, "unit": "states/s", "cores": int(cores), "kind": "port",
+        "sample": (f"numpy FP64 oracle, tuned arithmetic (exchange batched over the 4 vectors of a call, "
+                   f"gamma_lr.q_oo once per solve) on the same sample: {t_vec_t:.2f} s per A.x vector; "
+                   f"scaled to the same {n_products} products = {t_solve_t:.1f} s per solve"),
+    }
+    return faithful, tuned
 
 
 def _frag(d):
@@ -470,7 +487,7 @@ def main():
                      "x_bytes": ov * n * 8},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(inp, n, st64["n_iter"], st64["n_products"])
+        out["cpu_baseline"], out["cpu_baseline_tuned"] = cpu_baseline(inp, n, st64["n_iter"], st64["n_products"])
     print(json.dumps(out), flush=True)
 
 

@@ -2094,12 +2094,11 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     if (fp64_only) {
       // the reference rules, plus two guards that change nothing where one
       // classical Gram-Schmidt pass keeps the basis orthonormal: a second
-      // pass for a block in which some vector kept less than 1 % of its norm
-      // (a preconditioned residual lying almost in the basis: one pass then
-      // leaves it with components of ~eps |p| / |u| along the basis, and the
-      // Rayleigh-Ritz step of a nearly dependent basis returns spurious Ritz
-      // pairs near 0 whose residuals pass the test), and converged roots
-      // must be unit vectors
+      // pass for a block once the tracked loss of orthogonality of the basis
+      // would exceed eta_tol (append_block: preconditioned residuals lying
+      // largely in the basis compound it, and the Rayleigh-Ritz step of a
+      // nearly dependent basis returns spurious Ritz pairs near 0 whose
+      // residuals pass the test), and converged roots must be unit vectors
       res = davidson(c, n, n, max_iter, tol, PREC_FP64, false, 0, h_evals, h_err, 0, 0, nullptr, 0.5, 1e300, false,
                      0.0, true, true);
       res64 = res;
