@@ -52,6 +52,14 @@ def test_adjoint_vs_oracle(mol, op):
         dq = got["c_qov"][k][:, None] * x[:, k][None, :]
         assert np.abs(dq - ref["d_qov"]).max() < 1e-12 * max(np.abs(ref["d_qov"]).max(), 1e-3)
         assert np.array_equal(got_x["d_gamma"][k], got["d_gamma"][k])
+        # without the MO-charge derivatives the AO operator takes its own
+        # route (transition charges and gamma_lr derivatives from the AO
+        # transition densities, no q_oo / q_vv): the same closed form
+        for key in ("d_gamma", "d_gamma_lr"):
+            scale = max(np.abs(ref[key]).max(), 1e-3)
+            assert np.abs(got_x[key][k] - ref[key]).max() < 1e-12 * scale, (key, k)
+        dq = got_x["c_qov"][k][:, None] * x[:, k][None, :]
+        assert np.abs(dq - ref["d_qov"]).max() < 1e-12 * max(np.abs(ref["d_qov"]).max(), 1e-3)
     # sum rule: the Coulomb energy 2 u^T gamma u = sum_AB gamma_AB dw/dgamma_AB
     u = eng.q_ov @ x[:, 0]
     assert abs((got["d_gamma"][0] * m["gamma"]).sum() - 2.0 * u @ m["gamma"] @ u) < 1e-12

@@ -9,8 +9,10 @@
   the FP64 and the FP16-ladder solves agree per state to 1 meV (f reported
   and bounded); the returned X are orthonormal to 1e-10 and an independent
   FP64 product of them (tda_compute_products) gives |A x_k - w_k x_k| below
-  the tolerance; the AO-factored FP64 operator equals the per-atom-charges
-  operator (q_oo 51.5 GB, q_vv 12.9 GB, G = gamma_lr q_oo 51.5 GB on the
+  the tolerance; the adjoint of the FP64 solve (AO transition densities,
+  no q_oo / q_vv on the device) satisfies omega_k = sum de x^2 +
+  gamma : dw/dgamma + gamma_lr : dw/dgamma_lr; the AO-factored FP64 operator
+  equals the per-atom-charges operator (q_oo 51.5 GB, q_vv 12.9 GB, G = gamma_lr q_oo 51.5 GB on the
   device) on two vectors to 1e-12 relative -- two contractions of different
   structure, each pinned to the oracle on the fixture molecules.
 * The batched FMO workload at full size: 1024 fragments + 2048 CT pairs,
@@ -95,6 +97,19 @@ def test_c2_water512_ladder_vs_fp64_and_residuals():
     res = np.linalg.norm(ax - x * w[None, :], axis=0)
     assert res.max() < tol
     assert np.allclose(np.diag(x.T @ ax), w, rtol=0, atol=1e-9)
+    # the adjoint at full size, from the AO transition densities (no q_oo,
+    # q_vv, G): omega is homogeneous of degree one in (de, gamma, gamma_lr)
+    # at fixed X, so omega_k = sum de x_k^2 + sum gamma o dw/dgamma +
+    # sum gamma_lr o dw/dgamma_lr (x_k^T A x_k = omega_k for the unit
+    # eigenvectors, to the residual)
+    print("  C2 adjoint", flush=True)
+    adj = s64.adjoint(n, X=x64)
+    orbe = np.ravel(inp["orbe"])
+    de = (orbe[inp["virt_indices"]][None, :] - orbe[inp["occ_indices"]][:, None]).ravel()
+    euler = (de[:, None] * x64 * x64).sum(0) + np.einsum("kab,ab->k", adj["d_gamma"], inp["gamma"]) + \
+        np.einsum("kab,ab->k", adj["d_gamma_lr"], inp["gamma_lr"])
+    print(f"  max |omega - euler sum| {np.abs(euler - w64).max():.3e}", flush=True)
+    assert np.abs(euler - w64).max() < 1e-9
     s64.close()
     slp.close()
 

@@ -104,6 +104,15 @@ void launch_ao_operand(const double* C, const double* SC, int n_ao, int base, in
 void launch_ao_charges(const void* M, int dtype, int k, int n_ao, int64_t ld, const int* ao_off, int n_atoms,
                        const double* gamma_lr, double* u, hipStream_t s,
                        int64_t lo_off = 0);
+// transition charges without q_ov: XR [k][2n_ao][ld] (XR_c = R x_c^T), L
+// [2n_ao][ld]; u[A*ldu + c] = 1/2 sum_{mu in A} sum_{i<o} (L[(0,mu)][i]
+// XR_c[(1,mu)][i] + L[(1,mu)][i] XR_c[(0,mu)][i])
+void launch_ao_dens_charges(const double* L, const double* XR, int k, int n_ao, int o, int64_t ld,
+                            const int* ao_off, int n_atoms, double* u, int64_t ldu, hipStream_t s);
+// out[A*N + B] = -1/2 sum_{mu in A, nu in B} (D[mu][nu] D[n_ao+mu][n_ao+nu] +
+// D[mu][n_ao+nu] D[n_ao+mu][nu]) for D [2n_ao][ld] = L x R^T of one vector
+void launch_ao_dglr(const double* D, int64_t ld, int n_ao, const int* ao_off, int n_atoms, double* out,
+                    hipStream_t s);
 // M[c][(0,mu),(1,mu)] += w[atom(mu)*k + c] and M[c][(1,mu),(0,mu)] += same
 void launch_ao_add_coulomb(void* M, int dtype, int k, int n_ao, int64_t ld, const int* atom, const double* w,
                            hipStream_t s,

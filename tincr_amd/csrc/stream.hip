@@ -799,6 +799,71 @@ void launch_ao_charges(const void* M, int dtype, int k, int n_ao, int64_t ld, co
   TDA_LAUNCH_CHECK();
 }
 
+// Transition charges of k vectors from the AO factors, without the MO
+// charges q_ov: with D = L x R^T and XR_c = R x_c^T (the first GEMM of the
+// AO-factored product), q_ov^A x_c = 1/2 sum_{mu in A} (D[(0,mu),(1,mu)] +
+// D[(1,mu),(0,mu)]) = 1/2 sum_{mu in A} sum_i (L[(0,mu)][i] XR_c[(1,mu)][i] +
+// L[(1,mu)][i] XR_c[(0,mu)][i]).  One workgroup per (atom, vector); the sum
+// runs in a fixed order (deterministic).
+__global__ void __launch_bounds__(256) ao_dens_charges_kernel(const double* __restrict__ L,
+                                                              const double* __restrict__ XR, int n_ao, int o,
+                                                              int64_t ld, const int* __restrict__ ao_off,
+                                                              double* __restrict__ u, int64_t ldu) {
+  const int A = blockIdx.x;
+  const int c = blockIdx.y;
+  const double* X = XR + (int64_t)c * 2 * n_ao * ld;
+  double acc = 0.0;
+  for (int mu = ao_off[A]; mu < ao_off[A + 1]; ++mu) {
+    const double* l0 = L + (int64_t)mu * ld;
+    const double* l1 = L + (int64_t)(n_ao + mu) * ld;
+    const double* x0 = X + (int64_t)mu * ld;
+    const double* x1 = X + (int64_t)(n_ao + mu) * ld;
+    for (int i = threadIdx.x; i < o; i += 256) acc += l0[i] * x1[i] + l1[i] * x0[i];
+  }
+  __shared__ double red[4];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) u[(int64_t)A * ldu + c] = 0.5 * (red[0] + red[1] + red[2] + red[3]);
+}
+
+void launch_ao_dens_charges(const double* L, const double* XR, int k, int n_ao, int o, int64_t ld,
+                            const int* ao_off, int n_atoms, double* u, int64_t ldu, hipStream_t s) {
+  if (k == 0 || n_atoms == 0) return;
+  dim3 grid(n_atoms, k);
+  ao_dens_charges_kernel<<<grid, 256, 0, s>>>(L, XR, n_ao, o, ld, ao_off, u, ldu);
+  TDA_LAUNCH_CHECK();
+}
+
+// Derivative of w = x^T A x with respect to gamma_lr through the AO-factored
+// exchange: w holds -1/4 sum_pq D'_pq gamma~_pq D_pq with D = L x R^T and
+// D' = L' x R'^T = D with both index halves exchanged, so
+// dw / dgamma_lr[A][B] = -1/2 sum_{mu in A, nu in B} (D[mu][nu] D[n+mu][n+nu] +
+// D[mu][n+nu] D[n+mu][nu])   (n = n_ao; D rows with stride ld).
+__global__ void __launch_bounds__(256) ao_dglr_kernel(const double* __restrict__ D, int64_t ld, int n_ao,
+                                                      const int* __restrict__ ao_off, int n_atoms,
+                                                      double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (int64_t)n_atoms * n_atoms) return;
+  const int A = (int)(t / n_atoms), B = (int)(t % n_atoms);
+  double acc = 0.0;
+  for (int mu = ao_off[A]; mu < ao_off[A + 1]; ++mu) {
+    const double* d0 = D + (int64_t)mu * ld;
+    const double* d1 = D + (int64_t)(n_ao + mu) * ld;
+    for (int nu = ao_off[B]; nu < ao_off[B + 1]; ++nu)
+      acc += d0[nu] * d1[n_ao + nu] + d0[n_ao + nu] * d1[nu];
+  }
+  out[t] = -0.5 * acc;
+}
+
+void launch_ao_dglr(const double* D, int64_t ld, int n_ao, const int* ao_off, int n_atoms, double* out,
+                    hipStream_t s) {
+  const int64_t total = (int64_t)n_atoms * n_atoms;
+  if (total == 0) return;
+  ao_dglr_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(D, ld, n_ao, ao_off, n_atoms, out);
+  TDA_LAUNCH_CHECK();
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) ao_add_coulomb_kernel(T* __restrict__ M, int k, int n_ao, int64_t ld,
                                                              const int* __restrict__ atom,

@@ -146,6 +146,9 @@ struct tda_ctx {
   // operator form: 0 = AO-factored exchange, 1 = per-atom charges (three-stage)
   int op_kind = 0;
   bool mo_built = false;  // q_oo / q_vv present
+  // q_ov present; the AO operator (LC, not a CT pair) does not build it:
+  // its products, transition charges and adjoint work from the AO factors
+  bool qov_built = false;
   // AO-factored operands: Rt [2n_ao][v_p], Lm [2n_ao][o_p], Rpt [v_p][2n_ao],
   // Lpt [o_p][2n_ao]; the AO-expanded gamma_lr is gathered through ao_atom
   DevBuf ao_atom, aoRt, aoLm, aoRpt, aoLpt, aoXR, aoM, aoF;
@@ -494,15 +497,20 @@ void prepare(tda_ctx* c) {
   const int N = c->N, nao = c->nao;
   c->SC.ensure((size_t)nao * nao * 8);
   gemm_sc(c, c->S.as<double>(), c->C.as<double>(), c->SC.as<double>(), nao);
-  c->qov.ensure((size_t)N * c->ovp * 8);
-  if (!c->ct) {
+  const bool ao = c->has_lc && c->op_kind == 0 && !c->ct;
+  if (ao) {
+    c->qov.release();  // built on request only (tda_get_charges)
+  } else if (!c->ct) {
+    c->qov.ensure((size_t)N * c->ovp * 8);
     build_charges(c, c->C.as<double>(), c->SC.as<double>(), c->aooff.as<int>(), N, nao, c->ob, c->o, c->vb, c->v,
                   c->op, c->vp, c->qov.as<double>(), nullptr, nullptr);
   } else {
+    c->qov.ensure((size_t)N * c->ovp * 8);
     c->SCl.ensure((size_t)c->nao_l * c->nao_l * 8);
     gemm_sc(c, c->Sl.as<double>(), c->Cl.as<double>(), c->SCl.as<double>(), c->nao_l);
     build_ct_qov(c);
   }
+  c->qov_built = !ao;
   c->de.ensure((size_t)c->ovp * 8);
   launch_delta_eps(c->orbe.as<double>(), c->ob, c->o, c->vb, c->v, c->vp, c->de.as<double>(), c->stream);
   c->mo_built = false;
@@ -742,6 +750,57 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   }
 }
 
+// Transition charges u[A][ldu] (column c0 + c for vector c) of n vectors X
+// (device [n][o][v_p]) on the AO operands, without q_ov:
+// q_ov^A x = 1/2 sum_{mu in A} (D[(0,mu),(1,mu)] + D[(1,mu),(0,mu)]) with
+// D = L x R^T, from the first GEMM of the AO product (XR = R x^T) and a
+// per-atom dot over the occupied index.  FP64.
+void ao_trans_charges(tda_ctx* c, const double* X, int n, double* u, int64_t ldu) {
+  const int o = c->o, op = c->op, vp = c->vp, nao = c->nao;
+  const int64_t n2 = 2 * (int64_t)nao, per = n2 * op * 8;
+  const int64_t room = std::min<int64_t>((int64_t)c->budget, 8ll << 30) / per;
+  const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n, std::min<int64_t>(64, room)));
+  c->aoXR.ensure((size_t)chunk * n2 * op * 8);
+  for (int c0 = 0; c0 < n; c0 += chunk) {
+    const int kc = std::min(chunk, n - c0);
+    GemmArgs g = gargs(c->aoRt.p, X + (size_t)c0 * c->ovp, c->aoXR.p, n2, (int64_t)kc * o, vp, plain(vp), plain(1),
+                       plain(1), plain(vp), plain(op), split(o, n2 * op, 1));
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
+    launch_ao_dens_charges(c->aoLm.as<double>(), c->aoXR.as<double>(), kc, nao, o, op, c->aooff.as<int>(), c->N,
+                           u + c0, ldu, c->stream);
+  }
+}
+
+// d w_k / d gamma_lr [n][N][N] (device) of the AO operator from the AO
+// transition densities D = L x_k R^T (G1 and G2 of the product without the
+// Hadamard factor), without q_oo / q_vv / G: w_k holds -x^T K x =
+// -1/4 sum_pq D'_pq gamma~_pq D_pq, D' = L' x R'^T is D with both halves of
+// its index exchanged (L', R' are L, R with their halves swapped), so
+// d w_k / d gamma_lr[A][B] = -1/2 sum_{mu in A, nu in B} (D00 D11 + D01 D10).
+void ao_dglr(tda_ctx* c, const double* X, int n, double* dglr) {
+  const int o = c->o, op = c->op, vp = c->vp, nao = c->nao, N = c->N;
+  const int64_t n2 = 2 * (int64_t)nao, ld = ao_ld(c), per = (n2 * op + n2 * ld) * 8;
+  const int64_t room = std::min<int64_t>((int64_t)c->budget, 8ll << 30) / per;
+  const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n, std::min<int64_t>(64, room)));
+  c->aoXR.ensure((size_t)chunk * n2 * op * 8);
+  c->aoM.ensure((size_t)chunk * n2 * ld * 8);
+  for (int c0 = 0; c0 < n; c0 += chunk) {
+    const int kc = std::min(chunk, n - c0);
+    GemmArgs g1 = gargs(c->aoRt.p, X + (size_t)c0 * c->ovp, c->aoXR.p, n2, (int64_t)kc * o, vp, plain(vp), plain(1),
+                        plain(1), plain(vp), plain(op), split(o, n2 * op, 1));
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g1);
+    GemmArgs g2 = gargs(c->aoLm.p, c->aoXR.p, c->aoM.p, n2, n2, o, plain(op), plain(1), plain(1), plain(op),
+                        plain(ld), plain(1));
+    g2.batch = kc;
+    g2.b_batch = n2 * op;
+    g2.c_batch = n2 * ld;
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g2);
+    for (int j = 0; j < kc; ++j)
+      launch_ao_dglr(c->aoM.as<double>() + (size_t)j * n2 * ld, ld, nao, c->aooff.as<int>(), N,
+                     dglr + (size_t)(c0 + j) * N * N, c->stream);
+  }
+}
+
 void products(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   if (c->dense) {
     // AX[c][r] = sum_s H[r][s] X[c][s]  (solvers/mod.rs:41-43), pads of H zero
@@ -793,12 +852,23 @@ void hf_derivatives(tda_ctx* c, const double* Xd, int n, double* dgam, double* d
   const int64_t ovp = c->ovp, loo = (int64_t)op * op;
   c->u.ensure((size_t)N * n * 8);
   c->uu.ensure((size_t)N * n * 8);
-  GemmArgs gu = gargs(c->qov.p, Xd, c->u.p, N, n, ovp, plain(ovp), plain(1), plain(1), plain(ovp), plain(n),
-                      plain(1));
-  run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, gu);
+  if (c->qov_built) {
+    GemmArgs gu = gargs(c->qov.p, Xd, c->u.p, N, n, ovp, plain(ovp), plain(1), plain(1), plain(ovp), plain(n),
+                        plain(1));
+    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, gu);
+  } else {
+    ao_trans_charges(c, Xd, n, c->u.as<double>(), n);
+  }
   gamma_times(c, N, n);
   if (dgam) launch_hf_coulomb(c->u.as<double>(), c->uu.as<double>(), N, n, dgam, cq, c->stream);
   if (!c->has_lc || (!dglr && !dqoo && !dqvv)) return;
+  if (c->has_lc && c->op_kind == 0 && !c->ct && !dqoo && !dqvv) {
+    // AO operator: gamma_lr derivatives from the AO transition densities,
+    // no q_oo / q_vv / G (N (o^2 + v^2) words) on the device
+    ao_dglr(c, Xd, n, dglr);
+    TDA_HIP(hipStreamSynchronize(c->stream));
+    return;
+  }
   build_mo_charges(c);
   const int64_t NBop = (int64_t)N * op;
   c->T.ensure((size_t)vp * NBop * 8);
@@ -1829,6 +1899,13 @@ int tda_get_charges(tda_ctx* c, double* q_ov, double* q_oo, double* q_vv, double
     prepare(c);
     const int N = c->N, o = c->o, v = c->v, op = c->op, vp = c->vp;
     if (q_ov) {
+      if (!c->qov_built) {  // the AO operator does not keep q_ov: build it for the caller
+        c->qov.ensure((size_t)N * c->ovp * 8);
+        build_charges(c, c->C.as<double>(), c->SC.as<double>(), c->aooff.as<int>(), N, c->nao, c->ob, c->o, c->vb,
+                      c->v, c->op, c->vp, c->qov.as<double>(), nullptr, nullptr);
+        TDA_HIP(hipStreamSynchronize(c->stream));
+        c->qov_built = true;
+      }
       std::vector<double> h((size_t)N * c->ovp);
       TDA_HIP(hipMemcpy(h.data(), c->qov.p, h.size() * 8, hipMemcpyDeviceToHost));
       copy_unpad_rows(h.data(), N * o, v, vp, q_ov);
@@ -2175,9 +2252,13 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
     if (!c->dense) {
       // q_trans = q_ov X (tda/mod.rs:91), mu = xyz^T q_trans, f = 2/3 w |mu|^2 (moments.rs:7-24)
       c->u.ensure((size_t)N * n * 8);
-      GemmArgs g = gargs(c->qov.p, c->RZ.p, c->u.p, N, n, L, plain(L), plain(1), plain(1), plain(L), plain(n),
-                         plain(1));
-      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
+      if (c->qov_built) {
+        GemmArgs g = gargs(c->qov.p, c->RZ.p, c->u.p, N, n, L, plain(L), plain(1), plain(1), plain(L), plain(n),
+                           plain(1));
+        run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
+      } else {
+        ao_trans_charges(c, c->RZ.as<double>(), n, c->u.as<double>(), n);  // AO operator: no q_ov
+      }
       c->dxyz.ensure((size_t)N * 3 * 8);
       c->post.ensure((size_t)5 * n * 8);
       double* d_w = c->post.as<double>();
