@@ -17,6 +17,18 @@ extern "C" {
 int tda_debug_gemm(int in_prec, int a_kc, int b_kc, int64_t M, int64_t N, int64_t K, const double* A,
                    const double* B, double* C, double alpha, double beta, int ksplit);
 
+/* As tda_debug_gemm with FP64 products, the operands flagged a32 / b32
+ * stored on the device as FP32 (rounded from the given doubles) and widened
+ * to FP64 inside the kernel: C = alpha * fl32(A) . fl32(B) + beta * C. */
+int tda_debug_gemm_f32store(int a_kc, int b_kc, int64_t M, int64_t N, int64_t K, const double* A, const double* B,
+                            double* C, double alpha, double beta, int ksplit, int a32, int b32);
+
+/* Dot-product stream of the Davidson basis (FP64 arithmetic; a32 / b32: that
+ * operand stored on the device as FP32, rounded from the doubles):
+ * C[M][N] = alpha * A[M][K] . B[N][K]^T + beta * C */
+int tda_debug_rowdot(int M, int N, int64_t K, const double* A, int a32, const double* B, int b32, double* C,
+                     double alpha, double beta);
+
 /* The K-contiguous 16-bit GEMM path for batch entries z < batch:
  *   C[z][m][n] = epi(alpha * sum_k A[m][k] B[z][n][k])
  * A [M][lda], B [batch][N][ldb] (row strides >= K; elements past K are never

@@ -62,6 +62,49 @@ def test_gemm_split_k(ksplit):
     assert np.abs(out - a @ b).max() < 1e-11
 
 
+@pytest.mark.parametrize("shape", [(128, 16, 4096), (37, 13, 1031), (100, 30, 500), (16, 3000, 128),
+                                   (64, 64, 300), (300, 300, 64), (1, 5, 7)])
+@pytest.mark.parametrize("a32,b32", [(True, True), (True, False), (False, True)])
+@pytest.mark.parametrize("a_kc,b_kc", [(True, True), (True, False), (False, False), (False, True)])
+def test_gemm_f32_stored_operands(shape, a32, b32, a_kc, b_kc):
+    """FP64 products of FP32-stored operands (the FP32 Davidson basis): equal
+    to the FP64 product of the FP32-rounded inputs to FP64 rounding, so a
+    wrong widening index or vector width moves values by O(1)."""
+    nat = _nat()
+    m, n, k = shape
+    print(f"gemm f32-stored a32={a32} b32={b32} a_kc={a_kc} b_kc={b_kc} shape={shape}", flush=True)
+    rng = np.random.default_rng(m * 7919 + n * 31 + k)
+    a = rng.standard_normal((m, k))
+    b = rng.standard_normal((k, n))
+    c0 = rng.standard_normal((m, n))
+    ar = a.astype(np.float32).astype(np.float64) if a32 else a
+    br = b.astype(np.float32).astype(np.float64) if b32 else b
+    ref = -1.0 * ar @ br + 1.0 * c0
+    out = nat.debug_gemm_f32store(a, b, a32, b32, a_kc, b_kc, alpha=-1.0, beta=1.0, c=c0)
+    scale = (np.abs(ar) @ np.abs(br)).max() + 1.0
+    assert np.abs(out - ref).max() / scale < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 7), (16, 16, 64), (37, 13, 1031), (128, 16, 20000), (70, 45, 3001),
+                                   (200, 130, 2048)])
+@pytest.mark.parametrize("a32,b32", [(False, False), (True, True), (True, False)])
+def test_rowdot(shape, a32, b32):
+    """Basis dot products V P^T (direct-from-HBM MFMA stream, slices summed in
+    a second kernel) against numpy on the same rounded inputs, to FP64
+    rounding; row / column / length tails and several slices."""
+    nat = _nat()
+    m, n, k = shape
+    print(f"rowdot {shape} a32={a32} b32={b32}", flush=True)
+    rng = np.random.default_rng(m * 131 + n * 7 + k)
+    a, b, c0 = rng.standard_normal((m, k)), rng.standard_normal((n, k)), rng.standard_normal((m, n))
+    ar = a.astype(np.float32).astype(float) if a32 else a
+    br = b.astype(np.float32).astype(float) if b32 else b
+    out = nat.debug_rowdot(a, b, a32, b32, alpha=-0.5, beta=2.0, c=c0)
+    ref = -0.5 * ar @ br.T + 2.0 * c0
+    scale = (np.abs(ar) @ np.abs(br).T).max() + 1.0
+    assert np.abs(out - ref).max() / scale < 1e-14
+
+
 def test_mfma_layout_identity():
     """16x16 identity times a matrix of distinct values: any lane-layout slip
     in the MFMA operand or result mapping moves values."""

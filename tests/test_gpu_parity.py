@@ -255,6 +255,29 @@ def test_run_tda_mixed_precision(prec, mol, op):
     slp.close()
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("mol", ["uracil", "benzene"])
+def test_fp32_basis_of_the_16bit_phase(prec, mol, monkeypatch):
+    """The FP16 / BF16 phase keeps its basis V and products AV as FP32
+    (TINCR_TDA_BASIS32, default on; every product with them in FP64): the
+    solve lands within 1 meV of FP64 with the FP32 basis and with the FP64
+    one, and the 16-bit phase did run."""
+    m = molecule(mol)
+    s64 = _system(m)
+    w64, _ = s64.run_tda(6, 300, 1e-6)
+    s64.close()
+    for b32 in ("1", "0"):
+        monkeypatch.setenv("TINCR_TDA_BASIS32", b32)
+        print(f"fp32 basis={b32} {mol} {prec}", flush=True)
+        slp = _system(m, precision=prec)
+        w, _ = slp.run_tda(6, 300, 1e-6)
+        st = slp.last_stats
+        slp.close()
+        assert st["converged"] == 1 and st["max_residual"] < 1e-6
+        assert st["n_iter"] - st["n_iter_fp32"] - st["n_iter_fp64"] > 0
+        assert np.abs(w - w64).max() * HARTREE_TO_EV < 1e-3
+
+
 def test_small_system_mixed_context_runs_fp64(monkeypatch):
     """A mixed-precision context on a system whose products cost less than
     TINCR_TDA_LADDER_MIN_FLOPS runs the reference FP64 path: same omega as an

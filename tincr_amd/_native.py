@@ -237,6 +237,10 @@ def load_debug():
     i, i64 = ctypes.c_int, ctypes.c_int64
     lib.tda_debug_gemm.argtypes = [i, i, i, i64, i64, i64, D, D, D, ctypes.c_double, ctypes.c_double, i]
     lib.tda_debug_gemm.restype = ctypes.c_int
+    lib.tda_debug_gemm_f32store.argtypes = [i, i, i64, i64, i64, D, D, D, ctypes.c_double, ctypes.c_double, i, i, i]
+    lib.tda_debug_gemm_f32store.restype = ctypes.c_int
+    lib.tda_debug_rowdot.argtypes = [i, i, i64, D, i, D, i, D, ctypes.c_double, ctypes.c_double]
+    lib.tda_debug_rowdot.restype = ctypes.c_int
     I = ctypes.POINTER(ctypes.c_int)
     lib.tda_debug_gemm16.argtypes = [i, i, i64, i64, i64, i, i64, i64, D, D, ctypes.c_double, D, i, I, i, D, D, D, i,
                                      I]
@@ -262,6 +266,34 @@ def debug_gemm(a, b, precision="fp64", a_kc=True, b_kc=False, alpha=1.0, beta=0.
                             alpha, beta, ksplit)
     if rc != TDA_OK:
         raise TdaError(rc, "tda_debug_gemm failed")
+    return out
+
+
+def debug_gemm_f32store(a, b, a32=True, b32=True, a_kc=True, b_kc=False, alpha=1.0, beta=0.0, c=None, ksplit=0):
+    """alpha * fl32(A) @ fl32(B) (+ beta * C) as an FP64 product whose flagged
+    operands are stored as FP32 on the device.  a: [M, K]; b: [K, N]."""
+    lib = load_debug()
+    m, k = a.shape
+    n = b.shape[1]
+    a_st = f64(a if a_kc else a.T)
+    b_st = f64(b.T if b_kc else b)
+    out = np.zeros((m, n)) if c is None else f64(c).copy()
+    rc = lib.tda_debug_gemm_f32store(int(a_kc), int(b_kc), m, n, k, dptr(a_st), dptr(b_st), dptr(out), alpha, beta,
+                                     ksplit, int(a32), int(b32))
+    if rc != TDA_OK:
+        raise TdaError(rc, "tda_debug_gemm_f32store failed")
+    return out
+
+
+def debug_rowdot(a, b, a32=False, b32=False, alpha=1.0, beta=0.0, c=None):
+    """alpha * A @ B^T (+ beta * C) through the basis stream kernel.  a: [M, K]; b: [N, K]."""
+    lib = load_debug()
+    m, k = a.shape
+    n = b.shape[0]
+    out = np.zeros((m, n)) if c is None else f64(c).copy()
+    rc = lib.tda_debug_rowdot(m, n, k, dptr(f64(a)), int(a32), dptr(f64(b)), int(b32), dptr(out), alpha, beta)
+    if rc != TDA_OK:
+        raise TdaError(rc, "tda_debug_rowdot failed")
     return out
 
 

@@ -63,6 +63,67 @@ extern "C" int tda_debug_gemm(int in_prec, int a_kc, int b_kc, int64_t M, int64_
   });
 }
 
+extern "C" int tda_debug_gemm_f32store(int a_kc, int b_kc, int64_t M, int64_t N, int64_t K, const double* A,
+                                       const double* B, double* C, double alpha, double beta, int ksplit, int a32,
+                                       int b32) {
+  return guard([&] {
+    TDA_CHECK(M > 0 && N > 0 && K > 0 && (a32 || b32), TDA_ERR_BAD_ARG, "bad arguments");
+    Buf a64(M * K * 8), b64(K * N * 8), a(M * K * 4), b(K * N * 4), c(M * N * 8);
+    TDA_HIP(hipMemcpy(a64.p, A, M * K * 8, hipMemcpyHostToDevice));
+    TDA_HIP(hipMemcpy(b64.p, B, K * N * 8, hipMemcpyHostToDevice));
+    TDA_HIP(hipMemcpy(c.p, C, M * N * 8, hipMemcpyHostToDevice));
+    if (a32) launch_cast_vectors((const double*)a64.p, M * K, a.p, TDA_PREC_FP32, 0);
+    if (b32) launch_cast_vectors((const double*)b64.p, K * N, b.p, TDA_PREC_FP32, 0);
+    GemmArgs g;
+    std::memset(&g, 0, sizeof(g));
+    g.A = a32 ? a.p : a64.p;
+    g.B = b32 ? b.p : b64.p;
+    g.C = c.p;
+    g.M = M; g.N = N; g.K = K;
+    g.am = a_kc ? plain(K) : plain(1);
+    g.ak = a_kc ? plain(1) : plain(M);
+    g.bk = b_kc ? plain(1) : plain(N);
+    g.bn = b_kc ? plain(K) : plain(1);
+    g.cm = plain(N); g.cn = plain(1);
+    g.batch = 1; g.alpha = alpha; g.beta = beta;
+    g.dm = plain(0); g.dmap = plain(1); g.xm = plain(0); g.xn = plain(0);
+    g.hm = plain(0); g.hn = plain(0);
+    g.a_f32 = a32 != 0;
+    g.b_f32 = b32 != 0;
+    GemmPlan plan = gemm_plan(DT_F64, M, N, K, 1, a_kc, b_kc, ksplit);
+    Buf ws(gemm_workspace_bytes(DT_F64, plan, M, N, 1));
+    g.ws = ws.p;
+    gemm_launch(DT_F64, DT_F64, EPI_STORE, a_kc, b_kc, g, plan, 0);
+    TDA_HIP(hipDeviceSynchronize());
+    TDA_HIP(hipMemcpy(C, c.p, M * N * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+extern "C" int tda_debug_rowdot(int M, int N, int64_t K, const double* A, int a32, const double* B, int b32,
+                                double* C, double alpha, double beta) {
+  return guard([&] {
+    TDA_CHECK(M > 0 && N > 0 && K > 0, TDA_ERR_BAD_ARG, "bad arguments");
+    const int64_t ld = (K + 3) / 4 * 4;  // 16-byte aligned rows
+    Buf a64(M * K * 8), b64(N * K * 8), a((size_t)M * ld * 8), b((size_t)N * ld * 8), c((size_t)M * N * 8);
+    Buf ws(rowdot_ws_bytes(M, N, K));
+    TDA_HIP(hipMemcpy(a64.p, A, M * K * 8, hipMemcpyHostToDevice));
+    TDA_HIP(hipMemcpy(b64.p, B, N * K * 8, hipMemcpyHostToDevice));
+    TDA_HIP(hipMemcpy(c.p, C, (size_t)M * N * 8, hipMemcpyHostToDevice));
+    // rows of K elements at stride ld (FP32 or FP64)
+    for (int i = 0; i < M; ++i) {
+      if (a32) launch_cast_vectors((const double*)a64.p + i * K, K, (float*)a.p + i * ld, TDA_PREC_FP32, 0);
+      else TDA_HIP(hipMemcpy((double*)a.p + i * ld, (const double*)a64.p + i * K, K * 8, hipMemcpyDeviceToDevice));
+    }
+    for (int j = 0; j < N; ++j) {
+      if (b32) launch_cast_vectors((const double*)b64.p + j * K, K, (float*)b.p + j * ld, TDA_PREC_FP32, 0);
+      else TDA_HIP(hipMemcpy((double*)b.p + j * ld, (const double*)b64.p + j * K, K * 8, hipMemcpyDeviceToDevice));
+    }
+    launch_rowdot(a.p, a32 != 0, ld, b.p, b32 != 0, ld, M, N, K, alpha, beta, (double*)c.p, N, (double*)ws.p, 0);
+    TDA_HIP(hipDeviceSynchronize());
+    TDA_HIP(hipMemcpy(C, c.p, (size_t)M * N * 8, hipMemcpyDeviceToHost));
+  });
+}
+
 extern "C" int tda_debug_gemm16(int in_prec, int epi, int64_t M, int64_t N, int64_t K, int batch, int64_t lda,
                                 int64_t ldb, const double* A, const double* B, double alpha, const double* H, int nh,
                                 const int* hidx, int hmod, const double* diag, const double* X, double* C,

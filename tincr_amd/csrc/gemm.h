@@ -54,6 +54,12 @@ struct GemmArgs {
   // gamma_lr expanded to AO pairs) instead of H gathered through hidx.
   const float* hfull;
   int64_t hld;
+  // FP64 products of FP32-stored operands (a_f32 / b_f32 = 1: A / B hold
+  // floats, widened to FP64 on their way into LDS; generic kernel only).
+  int a_f32, b_f32;
+  // EPI_DIAG of the 16-bit kernel: the x of the diagonal term read as FP32
+  // (xsrc32) instead of FP64 (xsrc); the result may then be FP32 as well.
+  const float* xsrc32;
 };
 
 // Row / column index into H of the Hadamard epilogue.

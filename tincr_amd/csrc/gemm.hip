@@ -146,6 +146,7 @@ void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g,
     return;
   }
   TDA_CHECK(!g.a_lo && !g.b_lo && !g.c_lo, 2, "gemm: split operands need the 16-bit K-contiguous kernel");
+  TDA_CHECK(!(g.a_f32 || g.b_f32) || in == DT_F64, 2, "gemm: FP32-stored operands need FP64 products");
   GemmPlan pl = plan;
   // 16-bit inputs on the big tile: 128-deep K slices when the K maps allow
   // (TDA_BIG_K; off by default: it spills registers at two workgroups per CU)
@@ -174,7 +175,10 @@ void gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, GemmArgs g,
   else TDA_CHECK(g.bn.s0 == 1 && g.bn.div >= g.N, 2, "gemm: B N-contiguous needs plain unit bn");
   if (pl.ksplit > 1) TDA_CHECK(g.ws != nullptr, 2, "gemm: split-K needs a workspace");
   switch (in) {
-    case DT_F64: gemm_dispatch_f64(epi, out, pl.cfg, a_kc, b_kc, g, stream); break;
+    case DT_F64:
+      if (g.a_f32 || g.b_f32) gemm_dispatch_f64mix(epi, out, pl.cfg, a_kc, b_kc, g, stream);
+      else gemm_dispatch_f64(epi, out, pl.cfg, a_kc, b_kc, g, stream);
+      break;
     case DT_F32: gemm_dispatch_f32(epi, out, pl.cfg, a_kc, b_kc, g, stream); break;
     case DT_F16: gemm_dispatch_f16(epi, out, pl.cfg, a_kc, b_kc, g, stream); break;
     default: gemm_dispatch_bf16(epi, out, pl.cfg, a_kc, b_kc, g, stream); break;

@@ -403,8 +403,10 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
         put4(cmo, c_al, j, v);
       }
     } else if constexpr (EPI == EPI_DIAG) {
-      // FP64 output: C = alpha acc + diag[dm(m) + dmap(n)] * X[z][xm(m) + xn(n)]
-      const double* xs = g.xsrc + zb * g.x_batch;
+      // FP64 (or FP32) output: C = alpha acc + diag[dm(m) + dmap(n)] * X[z][xm(m) + xn(n)],
+      // X FP64 (xsrc) or FP32 (xsrc32), the sum formed in FP64
+      const double* xs = g.xsrc32 ? nullptr : g.xsrc + zb * g.x_batch;
+      const float* xs32 = g.xsrc32 ? g.xsrc32 + zb * g.x_batch : nullptr;
       const int64_t dmo = g.dm(m), xmo = g.xm(m);
 #pragma unroll
       for (int j = 0; j < NI; ++j) {
@@ -413,7 +415,8 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
           const int64_t n = n_first[j] + r;
           if (n >= g.N) continue;
           double v = (double)alpha * (double)acc[i][j][r];
-          v += g.diag[dmo + g.dmap(n)] * xs[xmo + g.xn(n)];
+          const int64_t xo = xmo + g.xn(n);
+          v += g.diag[dmo + g.dmap(n)] * (xs32 ? (double)xs32[xo] : xs[xo]);
           C[cmo + g.cn(n)] = to_type<TO>(v);
         }
       }
@@ -449,7 +452,7 @@ inline bool gemm16_ok(DType in, DType out, Epi epi, bool a_kc, bool b_kc, const 
   if ((g.a_lo || g.c_lo) && in != DT_F16) return false;
   if (g.a_lo && (g.a_lo % 8 || g.b_lo % 8 || g.a_lo < g.K || g.b_lo < g.K)) return false;
   if (g.c_lo && epi == EPI_DIAG) return false;
-  if (epi == EPI_DIAG) return out == DT_F64;
+  if (epi == EPI_DIAG) return out == DT_F64 || (out == DT_F32 && g.xsrc32);
   // EPI_HADAMARD: a full factor table with 32-bit offsets, or plain H maps
   // with 32-bit offsets
   if (epi == EPI_HADAMARD && g.hfull &&
@@ -503,6 +506,7 @@ void launch16_t(GemmArgs& g, hipStream_t s, int tile) {
 template <typename TI>
 void launch16_in(Epi epi, GemmArgs& g, hipStream_t s, int tile) {
   if (epi == EPI_HADAMARD) launch16_t<TI, TI, EPI_HADAMARD>(g, s, tile);
+  else if (epi == EPI_DIAG && g.xsrc32) launch16_t<TI, float, EPI_DIAG>(g, s, tile);
   else if (epi == EPI_DIAG) launch16_t<TI, double, EPI_DIAG>(g, s, tile);
   else launch16_t<TI, TI, EPI_STORE>(g, s, tile);
 }

@@ -144,7 +144,10 @@ __device__ inline double epi_value(const GemmArgs& g, int zb, int64_t m, int64_t
   return v;
 }
 
-template <typename TI, typename TO, typename Cfg, bool A_KC, bool B_KC, int EPI>
+// LA / LB: element types of A and B in memory when they differ from the MFMA
+// type TI (FP32-stored operands of an FP64 product: each element is widened
+// when the slice is written to LDS, so the arithmetic is TI's).
+template <typename TI, typename TO, typename Cfg, bool A_KC, bool B_KC, int EPI, typename LA = TI, typename LB = TI>
 __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs g) {
   using MF = Mfma<TI>;
   using TA = typename MF::TA;
@@ -153,7 +156,10 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
   constexpr int E = MF::E, KSTEP = MF::KSTEP;
   constexpr int WTM = BM / WGM, WTN = BN / WGN;
   constexpr int MI = WTM / 16, NI = WTN / 16;
-  constexpr int VEC = 16 / sizeof(TI);
+  // 16-byte global loads: VA / VB elements of each operand's memory type
+  constexpr int VA = 16 / sizeof(LA), VB = 16 / sizeof(LB);
+  static_assert(E == 1 || (std::is_same<LA, TI>::value && std::is_same<LB, TI>::value),
+                "widened operands exist for the 1-element-fragment types only");
   constexpr int PAD = (E == 1) ? (sizeof(TI) == 8 ? 2 : 4) : 8;
   // LDS geometry: E==1 -> [k][m] ; E==8 -> [m][k]
   constexpr int A_OUT = (E == 1) ? BK : BM, A_IN = (E == 1) ? BM + PAD : BK + PAD;
@@ -197,24 +203,24 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
   const int64_t m0 = (int64_t)blockIdx.y * BM;
   const int64_t n0 = (int64_t)blockIdx.x * BN;
 #endif
-  const TI* A = (const TI*)g.A + zb * g.a_batch;
-  const TI* B = (const TI*)g.B + zb * g.b_batch;
+  const LA* A = (const LA*)g.A + zb * g.a_batch;
+  const LB* B = (const LB*)g.B + zb * g.b_batch;
 
   const int64_t kbeg = (int64_t)zs * g.k_chunk;
   const int64_t kend = min(g.K, kbeg + g.k_chunk);
   const int nkt = kend > kbeg ? (int)((kend - kbeg + BK - 1) / BK) : 0;
 
   // ---- loader geometry (compile-time counts) ----
-  constexpr int A_VECS = BM * BK / VEC;  // vectors per A tile
-  constexpr int B_VECS = BN * BK / VEC;
+  constexpr int A_VECS = BM * BK / VA;  // vectors per A tile
+  constexpr int B_VECS = BN * BK / VB;
   constexpr int A_PER = (A_VECS + 255) / 256;
   constexpr int B_PER = (B_VECS + 255) / 256;
-  VecN<TI, VEC> ra[A_PER];
-  VecN<TI, VEC> rb[B_PER];
+  VecN<LA, VA> ra[A_PER];
+  VecN<LB, VB> rb[B_PER];
 
   // Precompute the (tile-invariant) row/col parts of addresses.
-  // For A_KC: vector idx -> (m = idx / (BK/VEC), kv = idx % (BK/VEC))
-  // else      : vector idx -> (k = idx / (BM/VEC), mv = idx % (BM/VEC))
+  // For A_KC: vector idx -> (m = idx / (BK/VA), kv = idx % (BK/VA))
+  // else      : vector idx -> (k = idx / (BM/VA), mv = idx % (BM/VA))  (B likewise with VB)
   // Row / column parts of the addresses and the per-vector eligibility for
   // the 16-byte load path (tile-invariant; the K part is checked per tile).
   const bool a_base_al = (((uintptr_t)A) & 15) == 0;
@@ -225,12 +231,12 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
   for (int i = 0; i < A_PER; ++i) {
     int idx = tid + i * 256;
     int64_t m;
-    if (A_KC) m = m0 + idx / (BK / VEC);
-    else m = m0 + (idx % (BM / VEC)) * VEC;
+    if (A_KC) m = m0 + idx / (BK / VA);
+    else m = m0 + (idx % (BM / VA)) * VA;
     a_row_ok[i] = (idx < A_VECS) && (m < g.M);
     a_row_off[i] = a_row_ok[i] ? g.am(m) : 0;
-    if (A_KC) a_vec_ok[i] = a_row_ok[i] && a_base_al && (a_row_off[i] % VEC == 0);
-    else a_vec_ok[i] = a_row_ok[i] && a_base_al && (m + VEC <= g.M) && (g.ak.s0 % VEC == 0);
+    if (A_KC) a_vec_ok[i] = a_row_ok[i] && a_base_al && (a_row_off[i] % VA == 0);
+    else a_vec_ok[i] = a_row_ok[i] && a_base_al && (m + VA <= g.M) && (g.ak.s0 % VA == 0);
   }
   int64_t b_col_off[B_PER];
   bool b_col_ok[B_PER], b_vec_ok[B_PER];
@@ -238,12 +244,12 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
   for (int i = 0; i < B_PER; ++i) {
     int idx = tid + i * 256;
     int64_t n;
-    if (B_KC) n = n0 + idx / (BK / VEC);
-    else n = n0 + (idx % (BN / VEC)) * VEC;
+    if (B_KC) n = n0 + idx / (BK / VB);
+    else n = n0 + (idx % (BN / VB)) * VB;
     b_col_ok[i] = (idx < B_VECS) && (n < g.N);
     b_col_off[i] = b_col_ok[i] ? g.bn(n) : 0;
-    if (B_KC) b_vec_ok[i] = b_col_ok[i] && b_base_al && (b_col_off[i] % VEC == 0);
-    else b_vec_ok[i] = b_col_ok[i] && b_base_al && (n + VEC <= g.N) && (g.bk.s0 % VEC == 0);
+    if (B_KC) b_vec_ok[i] = b_col_ok[i] && b_base_al && (b_col_off[i] % VB == 0);
+    else b_vec_ok[i] = b_col_ok[i] && b_base_al && (n + VB <= g.N) && (g.bk.s0 % VB == 0);
   }
 
   auto load_tile = [&](int kt) {
@@ -251,32 +257,32 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
     // K-map evaluated once per tile (tiles never straddle a K-map boundary).
     const int64_t ak0 = g.ak(k0), bk0 = g.bk(k0);
     const bool full_k = k0 + BK <= kend;
-    const bool a_k_al = (ak0 % VEC) == 0, b_k_al = (bk0 % VEC) == 0;
+    const bool a_k_al = (ak0 % VA) == 0, b_k_al = (bk0 % VB) == 0;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
       int idx = tid + i * 256;
       if (A_KC) {
-        const int kk = (idx % (BK / VEC)) * VEC;
-        const TI* p = A + a_row_off[i] + ak0 + kk;  // ak.s0 == 1 (host check)
+        const int kk = (idx % (BK / VA)) * VA;
+        const LA* p = A + a_row_off[i] + ak0 + kk;  // ak.s0 == 1 (host check)
         if (a_vec_ok[i] && full_k && a_k_al) {
-          ra[i] = *(const VecN<TI, VEC>*)p;
+          ra[i] = *(const VecN<LA, VA>*)p;
         } else {
-          VecN<TI, VEC> v;
+          VecN<LA, VA> v;
 #pragma unroll
-          for (int e = 0; e < VEC; ++e) v.v[e] = (a_row_ok[i] && k0 + kk + e < kend) ? p[e] : (TI)0;
+          for (int e = 0; e < VA; ++e) v.v[e] = (a_row_ok[i] && k0 + kk + e < kend) ? p[e] : (LA)0;
           ra[i] = v;
         }
       } else {
-        const int kk = idx / (BM / VEC);
-        const int64_t m = m0 + (idx % (BM / VEC)) * VEC;
-        const TI* p = A + a_row_off[i] + ak0 + g.ak.s0 * kk;
+        const int kk = idx / (BM / VA);
+        const int64_t m = m0 + (idx % (BM / VA)) * VA;
+        const LA* p = A + a_row_off[i] + ak0 + g.ak.s0 * kk;
         const bool krow = (idx < A_VECS) && (k0 + kk < kend);
         if (a_vec_ok[i] && krow && a_k_al) {
-          ra[i] = *(const VecN<TI, VEC>*)p;
+          ra[i] = *(const VecN<LA, VA>*)p;
         } else {
-          VecN<TI, VEC> v;
+          VecN<LA, VA> v;
 #pragma unroll
-          for (int e = 0; e < VEC; ++e) v.v[e] = (krow && a_row_ok[i] && m + e < g.M) ? p[e] : (TI)0;
+          for (int e = 0; e < VA; ++e) v.v[e] = (krow && a_row_ok[i] && m + e < g.M) ? p[e] : (LA)0;
           ra[i] = v;
         }
       }
@@ -285,27 +291,27 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
     for (int i = 0; i < B_PER; ++i) {
       int idx = tid + i * 256;
       if (B_KC) {
-        const int kk = (idx % (BK / VEC)) * VEC;
-        const TI* p = B + b_col_off[i] + bk0 + kk;  // bk.s0 == 1 (host check)
+        const int kk = (idx % (BK / VB)) * VB;
+        const LB* p = B + b_col_off[i] + bk0 + kk;  // bk.s0 == 1 (host check)
         if (b_vec_ok[i] && full_k && b_k_al) {
-          rb[i] = *(const VecN<TI, VEC>*)p;
+          rb[i] = *(const VecN<LB, VB>*)p;
         } else {
-          VecN<TI, VEC> v;
+          VecN<LB, VB> v;
 #pragma unroll
-          for (int e = 0; e < VEC; ++e) v.v[e] = (b_col_ok[i] && k0 + kk + e < kend) ? p[e] : (TI)0;
+          for (int e = 0; e < VB; ++e) v.v[e] = (b_col_ok[i] && k0 + kk + e < kend) ? p[e] : (LB)0;
           rb[i] = v;
         }
       } else {
-        const int kk = idx / (BN / VEC);
-        const int64_t n = n0 + (idx % (BN / VEC)) * VEC;
-        const TI* p = B + b_col_off[i] + bk0 + g.bk.s0 * kk;
+        const int kk = idx / (BN / VB);
+        const int64_t n = n0 + (idx % (BN / VB)) * VB;
+        const LB* p = B + b_col_off[i] + bk0 + g.bk.s0 * kk;
         const bool krow = (idx < B_VECS) && (k0 + kk < kend);
         if (b_vec_ok[i] && krow && b_k_al) {
-          rb[i] = *(const VecN<TI, VEC>*)p;
+          rb[i] = *(const VecN<LB, VB>*)p;
         } else {
-          VecN<TI, VEC> v;
+          VecN<LB, VB> v;
 #pragma unroll
-          for (int e = 0; e < VEC; ++e) v.v[e] = (krow && b_col_ok[i] && n + e < g.N) ? p[e] : (TI)0;
+          for (int e = 0; e < VB; ++e) v.v[e] = (krow && b_col_ok[i] && n + e < g.N) ? p[e] : (LB)0;
           rb[i] = v;
         }
       }
@@ -318,21 +324,26 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
       int idx = tid + i * 256;
       if (idx >= A_VECS) continue;
       int m, k;
-      if (A_KC) { m = idx / (BK / VEC); k = (idx % (BK / VEC)) * VEC; }
-      else { k = idx / (BM / VEC); m = (idx % (BM / VEC)) * VEC; }
-      if (E == 1) {  // LDS [k][m]
+      if (A_KC) { m = idx / (BK / VA); k = (idx % (BK / VA)) * VA; }
+      else { k = idx / (BM / VA); m = (idx % (BM / VA)) * VA; }
+      if constexpr (E == 1) {  // LDS [k][m]
         if (A_KC) {
 #pragma unroll
-          for (int e = 0; e < VEC; ++e) As[(k + e) * A_IN + m] = ra[i].v[e];
+          for (int e = 0; e < VA; ++e) As[(k + e) * A_IN + m] = (TI)ra[i].v[e];
         } else {
-          *(VecN<TI, VEC>*)&As[k * A_IN + m] = ra[i];
+          if constexpr (std::is_same<LA, TI>::value) {
+            *(VecN<TI, VA>*)&As[k * A_IN + m] = ra[i];
+          } else {
+#pragma unroll
+            for (int e = 0; e < VA; ++e) As[k * A_IN + m + e] = (TI)ra[i].v[e];
+          }
         }
       } else {  // LDS [m][k]
         if (A_KC) {
-          *(VecN<TI, VEC>*)&As[m * A_IN + k] = ra[i];
+          *(VecN<TI, VA>*)&As[m * A_IN + k] = ra[i];
         } else {
 #pragma unroll
-          for (int e = 0; e < VEC; ++e) As[(m + e) * A_IN + k] = ra[i].v[e];
+          for (int e = 0; e < VA; ++e) As[(m + e) * A_IN + k] = ra[i].v[e];
         }
       }
     }
@@ -341,21 +352,26 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
       int idx = tid + i * 256;
       if (idx >= B_VECS) continue;
       int n, k;
-      if (B_KC) { n = idx / (BK / VEC); k = (idx % (BK / VEC)) * VEC; }
-      else { k = idx / (BN / VEC); n = (idx % (BN / VEC)) * VEC; }
-      if (E == 1) {  // LDS [k][n]
+      if (B_KC) { n = idx / (BK / VB); k = (idx % (BK / VB)) * VB; }
+      else { k = idx / (BN / VB); n = (idx % (BN / VB)) * VB; }
+      if constexpr (E == 1) {  // LDS [k][n]
         if (B_KC) {
 #pragma unroll
-          for (int e = 0; e < VEC; ++e) Bs[(k + e) * B_IN + n] = rb[i].v[e];
+          for (int e = 0; e < VB; ++e) Bs[(k + e) * B_IN + n] = (TI)rb[i].v[e];
         } else {
-          *(VecN<TI, VEC>*)&Bs[k * B_IN + n] = rb[i];
+          if constexpr (std::is_same<LB, TI>::value) {
+            *(VecN<TI, VB>*)&Bs[k * B_IN + n] = rb[i];
+          } else {
+#pragma unroll
+            for (int e = 0; e < VB; ++e) Bs[k * B_IN + n + e] = (TI)rb[i].v[e];
+          }
         }
       } else {  // LDS [n][k]
         if (B_KC) {
-          *(VecN<TI, VEC>*)&Bs[n * B_IN + k] = rb[i];
+          *(VecN<TI, VB>*)&Bs[n * B_IN + k] = rb[i];
         } else {
 #pragma unroll
-          for (int e = 0; e < VEC; ++e) Bs[(n + e) * B_IN + k] = rb[i].v[e];
+          for (int e = 0; e < VB; ++e) Bs[(n + e) * B_IN + k] = rb[i].v[e];
         }
       }
     }
@@ -564,32 +580,33 @@ inline int bk_of(DType in, int cfg = CFG_BIG) {
 }
 inline size_t acc_bytes(DType in) { return in == DT_F64 ? 8 : 4; }
 
-template <typename TI, typename TO, typename Cfg, int EPI>
+template <typename TI, typename TO, typename Cfg, int EPI, typename LA = TI, typename LB = TI>
 void launch_cfg(bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s) {
   dim3 grid((unsigned)ceil_div(g.N, Cfg::BN), (unsigned)ceil_div(g.M, Cfg::BM),
             (unsigned)(g.batch * g.ksplit));
-  if (a_kc && b_kc) gemm_kernel<TI, TO, Cfg, true, true, EPI><<<grid, 256, 0, s>>>(g);
-  else if (a_kc && !b_kc) gemm_kernel<TI, TO, Cfg, true, false, EPI><<<grid, 256, 0, s>>>(g);
-  else if (!a_kc && b_kc) gemm_kernel<TI, TO, Cfg, false, true, EPI><<<grid, 256, 0, s>>>(g);
-  else gemm_kernel<TI, TO, Cfg, false, false, EPI><<<grid, 256, 0, s>>>(g);
+  if (a_kc && b_kc) gemm_kernel<TI, TO, Cfg, true, true, EPI, LA, LB><<<grid, 256, 0, s>>>(g);
+  else if (a_kc && !b_kc) gemm_kernel<TI, TO, Cfg, true, false, EPI, LA, LB><<<grid, 256, 0, s>>>(g);
+  else if (!a_kc && b_kc) gemm_kernel<TI, TO, Cfg, false, true, EPI, LA, LB><<<grid, 256, 0, s>>>(g);
+  else gemm_kernel<TI, TO, Cfg, false, false, EPI, LA, LB><<<grid, 256, 0, s>>>(g);
   TDA_LAUNCH_CHECK();
 }
 
-template <typename TI, typename TO, int EPI>
+template <typename TI, typename TO, int EPI, typename LA = TI, typename LB = TI>
 void launch_t(int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s) {
   switch (cfg) {
-    case CFG_BIG: launch_cfg<TI, TO, CfgBig, EPI>(a_kc, b_kc, g, s); break;
+    case CFG_BIG: launch_cfg<TI, TO, CfgBig, EPI, LA, LB>(a_kc, b_kc, g, s); break;
     case CFG_BIG_K:
-      if constexpr (Mfma<TI>::E == 8) launch_cfg<TI, TO, CfgBigK, EPI>(a_kc, b_kc, g, s);
-      else launch_cfg<TI, TO, CfgBig, EPI>(a_kc, b_kc, g, s);
+      if constexpr (Mfma<TI>::E == 8) launch_cfg<TI, TO, CfgBigK, EPI, LA, LB>(a_kc, b_kc, g, s);
+      else launch_cfg<TI, TO, CfgBig, EPI, LA, LB>(a_kc, b_kc, g, s);
       break;
-    case CFG_SKINNY_N: launch_cfg<TI, TO, CfgSkinnyN, EPI>(a_kc, b_kc, g, s); break;
-    case CFG_SKINNY_M: launch_cfg<TI, TO, CfgSkinnyM, EPI>(a_kc, b_kc, g, s); break;
+    case CFG_SKINNY_N: launch_cfg<TI, TO, CfgSkinnyN, EPI, LA, LB>(a_kc, b_kc, g, s); break;
+    case CFG_SKINNY_M: launch_cfg<TI, TO, CfgSkinnyM, EPI, LA, LB>(a_kc, b_kc, g, s); break;
     case CFG_SKINNY_N16:
-      if constexpr (std::is_same<TI, double>::value) launch_cfg<TI, TO, CfgSkinnyN16, EPI>(a_kc, b_kc, g, s);
-      else launch_cfg<TI, TO, CfgSkinnyN, EPI>(a_kc, b_kc, g, s);
+      if constexpr (std::is_same<TI, double>::value)
+        launch_cfg<TI, TO, CfgSkinnyN16, EPI, LA, LB>(a_kc, b_kc, g, s);
+      else launch_cfg<TI, TO, CfgSkinnyN, EPI, LA, LB>(a_kc, b_kc, g, s);
       break;
-    default: launch_cfg<TI, TO, CfgSmall, EPI>(a_kc, b_kc, g, s); break;
+    default: launch_cfg<TI, TO, CfgSmall, EPI, LA, LB>(a_kc, b_kc, g, s); break;
   }
 }
 
@@ -613,6 +630,8 @@ void launch_reduce(GemmArgs& g, hipStream_t s) {
 // Per-type entry points, each defined in its own translation unit
 // (gemm_f64.hip, gemm_f32.hip, gemm_f16.hip, gemm_bf16.hip).
 void gemm_dispatch_f64(int epi, DType out, int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s);
+// FP64 products with one or both operands stored as FP32 (gemm_f64mix.hip)
+void gemm_dispatch_f64mix(int epi, DType out, int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s);
 void gemm_dispatch_f32(int epi, DType out, int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s);
 void gemm_dispatch_f16(int epi, DType out, int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s);
 void gemm_dispatch_bf16(int epi, DType out, int cfg, bool a_kc, bool b_kc, GemmArgs& g, hipStream_t s);

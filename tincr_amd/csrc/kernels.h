@@ -1,4 +1,4 @@
-// Launchers of the non-GEMM kernels (qbuild.hip, stream.hip, eigh.hip).
+// Launchers of the non-GEMM kernels (stream.hip, eigh.hip, basis.hip).
 #pragma once
 #include "common.h"
 
@@ -34,6 +34,16 @@ void launch_unpack_x(const double* src, int o, int v, int vp, int k, double* dst
 void launch_delta_eps(const double* orbe, int ob, int o, int vb, int v, int vp, double* de, hipStream_t s);
 // dst[k][L] (double) -> (float/half/bf16)
 void launch_cast_vectors(const double* src, int64_t n, void* dst, int dtype, hipStream_t s);
+// FP32 -> FP64 element copy
+void launch_widen(const float* src, int64_t n, double* dst, hipStream_t s);
+
+// ---- dot products of the Davidson basis (basis.hip) ----
+// C[i][j] (ldc) = alpha sum_{l<K} A[i][l] B[j][l] + beta C[i][j]; A [M][lda],
+// B [N][ldb], each FP32 (a32 / b32) or FP64, rows 16-byte aligned; FP64
+// products.  ws: rowdot_ws_bytes(M, N, K) bytes (may be NULL when 0).
+size_t rowdot_ws_bytes(int M, int N, int64_t K);
+void launch_rowdot(const void* A, bool a32, int64_t lda, const void* B, bool b32, int64_t ldb, int M, int N,
+                   int64_t K, double alpha, double beta, double* C, int64_t ldc, double* ws, hipStream_t s);
 
 // ---- Davidson helpers (davidson.rs:53-185, solvers/mod.rs:373-380) ----
 // out = r / d,  d = w - de,  |d| < 1e-4 -> 1.0

@@ -251,6 +251,15 @@ __global__ void __launch_bounds__(256) cast_kernel(const double* __restrict__ sr
     dst[t] = to_type<T>(src[t]);
 }
 
+__global__ void __launch_bounds__(256) widen_kernel(const float* __restrict__ src, int64_t n, double* __restrict__ dst) {
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) dst[t] = src[t];
+}
+void launch_widen(const float* src, int64_t n, double* dst, hipStream_t s) {
+  if (n == 0) return;
+  widen_kernel<<<grid_for(n), 256, 0, s>>>(src, n, dst);
+  TDA_LAUNCH_CHECK();
+}
+
 void launch_cast_vectors(const double* src, int64_t n, void* dst, int dtype, hipStream_t s) {
   if (n == 0) return;
   const int g = grid_for(n);

@@ -207,7 +207,21 @@ struct tda_ctx {
   // loss of orthogonality of the basis would exceed eta_tol
   int reorth_mode = 2;
   double eta_tol = 1e-8;
+  double eta_tol32 = 1e-5;  // the same for an FP32 basis (whose rounding alone is 6e-8)
+  // near the precision floor of a low-precision phase (max residual below
+  // near_k * delta) the phase hands over after near_stall iterations without
+  // a 20 % improvement: there the products' error, not the subspace, limits
+  // the residual
+  double near_k = 4.0;
+  int near_stall = 2;
   bool lp_phase = true;  // the FP16 / BF16 phase of the ladder (off: start at the FP32-accuracy phase)
+  // FP16 / BF16 phase of the AO operator: the basis V and its products AV
+  // stored as FP32 (the phase's product error, ~1e-4 per unit vector, lies
+  // far above FP32 rounding); every product with them is formed in FP64.
+  // Halves the bytes of the subspace GEMMs that stream V and AV.
+  bool basis32 = true;
+  DevBuf qstage;  // FP64 staging of the new basis vectors of an FP32 basis
+  DevBuf ws_basis;  // partial sums of the basis dot products (launch_rowdot)
   // instrumentation
   bool timing = true;
   bool verbose = false;
@@ -241,7 +255,7 @@ struct tda_ctx {
                       &stage, &stage2, &V, &AV, &H, &Y, &evals, &RZ, &R, &PW, &norms, &npart, &idx, &count,
                       &gs_scratch, &eigw, &guess, &ev_e, &ev_info, &gfull, &ao_atom, &aoRt, &aoLm, &aoRpt, &aoLpt, &aoXR, &aoM, &aoF,
                       &Cl, &Sl, &SCl, &Shl, &aooff_l, &ctW, &ctZ, &Hd, &dxyz, &post,
-                      &adjP, &adjQ, &adjX, &adjD, &one, &zvw})
+                      &adjP, &adjQ, &adjX, &adjD, &one, &zvw, &qstage, &ws_basis})
       b->release();
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -311,6 +325,38 @@ void run_gemm(tda_ctx* c, DType in, DType out, Epi epi, bool a_kc, bool b_kc, Ge
     g.ws = c->ws_gemm.p;
   }
   gemm_launch(in, out, epi, a_kc, b_kc, g, plan, c->stream, after_main);
+}
+
+// Davidson basis products on rows of length L = ovp (FP64 arithmetic; the
+// basis may be stored as FP32).
+// C[i][j] (ldc) = alpha A_i . B_j + beta C[i][j] for i < M, j < N: the
+// register-streaming dot kernel (basis.hip) for FP32 rows and for up to 96
+// FP64 rows, the LDS-tiled GEMM for more FP64 rows (measured faster there)
+void bdot(tda_ctx* c, const void* A, bool a32, int M, const void* B, bool b32, int N, double* C, int64_t ldc,
+          double alpha = 1.0, double beta = 0.0) {
+  const int64_t L = c->ovp;
+  if (a32 || b32 || M <= 96) {
+    const size_t wsb = rowdot_ws_bytes(M, N, L);
+    if (wsb) c->ws_basis.ensure(wsb);
+    launch_rowdot(A, a32, L, B, b32, L, M, N, L, alpha, beta, C, ldc, wsb ? c->ws_basis.as<double>() : nullptr,
+                  c->stream);
+    return;
+  }
+  GemmArgs g = gargs(A, B, C, M, N, L, plain(L), plain(1), plain(1), plain(L), plain(ldc), plain(1));
+  g.alpha = alpha;
+  g.beta = beta;
+  run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
+}
+// C_r = alpha sum_j Y[r ysr + j ysj] V_j + beta C_r for r < R (rows of C and
+// V of length and stride L), on the LDS-tiled GEMM
+void bcomb(tda_ctx* c, const double* Y, int64_t ysr, int64_t ysj, int R, int J, const void* V, bool v32, double* C,
+           double alpha = 1.0, double beta = 0.0) {
+  const int64_t L = c->ovp;
+  GemmArgs g = gargs(Y, V, C, R, L, J, plain(ysr), plain(ysj), plain(L), plain(1), plain(L), plain(1));
+  g.b_f32 = v32;
+  g.alpha = alpha;
+  g.beta = beta;
+  run_gemm(c, DT_F64, DT_F64, EPI_STORE, ysj == 1, false, g);
 }
 
 // ---------------------------------------------------------------------------
@@ -644,7 +690,10 @@ void apply_A(tda_ctx* c, const double* X, int k, double* AX, int prec) {
 //   G4  AX_c = de o x_c + L'^T F_c^T             [o][v_p]
 // M, F, R'^T and L'^T store their (s, mu) rows with the padded stride ao_ld().
 // ---------------------------------------------------------------------------
-void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
+// x32: X and AX are FP32 vectors (the FP32 basis of the FP16 / BF16 phase;
+// 16-bit products only): X is narrowed from FP32, the diagonal term reads it
+// as FP32 and AX is written as FP32.
+void apply_A_ao(tda_ctx* c, const void* X, int k, void* AX, int prec, bool x32 = false) {
   const int o = c->o, v = c->v, op = c->op, vp = c->vp, nao = c->nao, N = c->N;
   const int64_t ovp = c->ovp, n2 = 2 * (int64_t)nao, ld = ao_ld(c);
   const bool lp = prec != PREC_FP64;
@@ -655,14 +704,16 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
   const DType dt = lp ? dtype_of(prec) : DT_F64;
   const size_t esz = lp ? prec_bytes(prec) : 8;
   const int mprec = lp ? (sp ? PREC_FP16 : prec) : PREC_FP64;  // element type of M
+  TDA_CHECK(!x32 || (lp && !sp), TDA_ERR_BAD_ARG, "apply_A_ao: FP32 vectors need 16-bit products");
   const void* Xin = X;
   if (sp) {
     c->xlp.ensure((size_t)k * ovp * 2 * esz);
-    launch_convert_split(X, (int64_t)k * o, vp, vp, c->xlp.p, 2 * vp, vp, c->stream);
+    launch_convert_split((const double*)X, (int64_t)k * o, vp, vp, c->xlp.p, 2 * vp, vp, c->stream);
     Xin = c->xlp.p;
   } else if (lp) {
     c->xlp.ensure((size_t)k * ovp * esz);
-    launch_cast_vectors(X, (int64_t)k * ovp, c->xlp.p, prec, c->stream);
+    if (x32) launch_convert_f32_to((const float*)X, (int64_t)k * ovp, c->xlp.p, prec, c->stream);
+    else launch_cast_vectors((const double*)X, (int64_t)k * ovp, c->xlp.p, prec, c->stream);
     Xin = c->xlp.p;
   }
   if (lp) prepare_lowprec(c, prec);
@@ -736,11 +787,12 @@ void apply_A_ao(tda_ctx* c, const double* X, int k, double* AX, int prec) {
     g.diag = c->de.as<double>();
     g.dm = plain(vp);
     g.dmap = plain(1);
-    g.xsrc = X;
+    if (x32) g.xsrc32 = (const float*)X;
+    else g.xsrc = (const double*)X;
     g.xm = plain(vp);
     g.xn = plain(1);
     g.x_batch = ovp;
-    run_gemm(c, dt, DT_F64, EPI_DIAG, true, true, g);
+    run_gemm(c, dt, x32 ? DT_F32 : DT_F64, EPI_DIAG, true, true, g);
   }
   if (c->timing) {
     // MFMA flops of the launches (three products per split GEMM)
@@ -801,8 +853,12 @@ void ao_dglr(tda_ctx* c, const double* X, int n, double* dglr) {
   }
 }
 
-void products(tda_ctx* c, const double* X, int k, double* AX, int prec) {
+// x32: X and AX are FP32 vectors (AO operator, FP16 / BF16 products only).
+void products(tda_ctx* c, const void* Xv, int k, void* AXv, int prec, bool x32 = false) {
+  const double* X = (const double*)Xv;
+  double* AX = (double*)AXv;
   if (c->dense) {
+    TDA_CHECK(!x32, TDA_ERR_BAD_ARG, "products: FP32 vectors need the AO operator");
     // AX[c][r] = sum_s H[r][s] X[c][s]  (solvers/mod.rs:41-43), pads of H zero
     const int64_t L = c->ovp;
     GemmArgs g = gargs(X, c->Hd.p, AX, k, L, L, plain(L), plain(1), plain(1), plain(L), plain(L), plain(1));
@@ -821,9 +877,11 @@ void products(tda_ctx* c, const double* X, int k, double* AX, int prec) {
                        (prec == PREC_F16X ? 2.0 : 1.0);
     chunk = (int)std::max(1.0, std::min(64.0, std::min((double)c->budget, 8.0 * (1ull << 30)) / per));
   }
+  TDA_CHECK(!x32 || ao, TDA_ERR_BAD_ARG, "products: FP32 vectors need the AO operator");
   for (int c0 = 0; c0 < k; c0 += chunk) {
     const int kc = std::min(chunk, k - c0);
-    if (ao) apply_A_ao(c, X + (size_t)c0 * c->ovp, kc, AX + (size_t)c0 * c->ovp, prec);
+    if (x32) apply_A_ao(c, (const float*)Xv + (size_t)c0 * c->ovp, kc, (float*)AXv + (size_t)c0 * c->ovp, prec, true);
+    else if (ao) apply_A_ao(c, X + (size_t)c0 * c->ovp, kc, AX + (size_t)c0 * c->ovp, prec);
     else apply_A(c, X + (size_t)c0 * c->ovp, kc, AX + (size_t)c0 * c->ovp, prec);
   }
   c->n_products += k;
@@ -1006,9 +1064,8 @@ int gs_block(tda_ctx* c, double* P, double* W, double* Q, int nc, double* wn) {
   if (c->gs_gram && nc <= 64) {
     double* Gt = scr;                         // [2 nc][nc]
     double* Tq = scr + (size_t)2 * nc * nc;   // [nc][nc]
-    GemmArgs g = gargs(P, W, Gt, 2 * nc, nc, L, split(nc, c->pw_rows * L, L), plain(1), plain(1), plain(L),
-                       plain(nc), plain(1));
-    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
+    bdot(c, P, false, nc, W, false, nc, Gt, nc);                     // rows [0, nc): P W^T
+    bdot(c, W, false, nc, W, false, nc, Gt + (size_t)nc * nc, nc);   // rows [nc, 2 nc): W W^T
     launch_gs_gram(Gt, nc, Tq, c->norms.as<double>(), d_count, c->stream);
     int* hc = (int*)(c->pinned() + 320);
     double* hwn = c->pinned() + 192;
@@ -1019,8 +1076,7 @@ int gs_block(tda_ctx* c, double* P, double* W, double* Q, int nc, double* wn) {
     if (!hc[1]) {
       const int nq = hc[0];
       if (nq == 0) return 0;
-      GemmArgs q = gargs(Tq, W, Q, nq, L, nc, plain(nc), plain(1), plain(L), plain(1), plain(L), plain(1));
-      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, q);
+      bcomb(c, Tq, nc, 1, nq, nc, W, false, Q);
       // The Gram estimate of |u_j| is a difference of large terms when the
       // block's vectors are nearly dependent among themselves (large
       // coefficients T): the accepted q_j must come out unit vectors, else
@@ -1093,27 +1149,30 @@ void subspace_eigh(tda_ctx* c, const double* H, int ldh, int m, int n, double* e
 // second pass ("twice is enough": Daniel, Gragg, Kaufman and Stewart) once
 // the estimate would exceed eta_tol; after it the new vectors are orthogonal
 // to the basis to ~eps and eta stays where it was.
-int append_block(tda_ctx* c, int m, int nc, bool reorth, double& eta) {
+// v32: the basis is stored as FP32 (V as floats; the products with it are
+// FP64): the new vectors are formed in FP64 staging and rounded to FP32 when
+// appended, so the basis is orthogonal to ~FP32 rounding (eps32 = 6e-8) and
+// eta is tracked from there.
+int append_block(tda_ctx* c, int m, int nc, bool reorth, double& eta, bool v32 = false) {
   const int64_t L = c->ovp;
   double* V = c->V.as<double>();
+  float* V32 = (float*)c->V.p;
+  const void* Vb = v32 ? (const void*)V32 : (const void*)V;
   double* P = c->Pb();
   double* W = c->Wb();
   double* C0 = c->stage2.as<double>();
   launch_copy(P, W, (int64_t)nc * L, c->stream);
   if (m > 0) {
-    GemmArgs g0 = gargs(V, P, C0, m, nc, L, plain(L), plain(1), plain(1), plain(L), plain(nc), plain(1));
-    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g0);
-    GemmArgs g1 = gargs(C0, V, W, nc, L, m, plain(1), plain(nc), plain(L), plain(1), plain(L), plain(1));
-    g1.alpha = -1.0;
-    g1.beta = 1.0;
-    run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, g1);
+    bdot(c, Vb, v32, m, P, false, nc, C0, nc);              // C0 = V P^T  [m][nc]
+    bcomb(c, C0, 1, nc, nc, m, Vb, v32, W, -1.0, 1.0);      // W -= C0^T V
   }
   std::vector<double> pn(nc), wn(nc);
   if (reorth) {
     launch_norms(P, nc, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
     TDA_HIP(hipMemcpyAsync(c->pinned() + 128, c->norms.p, nc * 8, hipMemcpyDeviceToHost, c->stream));
   }
-  double* Q = V + (size_t)m * L;
+  if (v32) c->qstage.ensure((size_t)nc * L * 8);
+  double* Q = v32 ? c->qstage.as<double>() : V + (size_t)m * L;
   int h_count = gs_block(c, P, W, Q, nc, reorth ? wn.data() : nullptr);
   if (reorth) std::memcpy(pn.data(), c->pinned() + 128, nc * 8);  // gs_block synchronized the stream
   bool need = false;
@@ -1121,22 +1180,22 @@ int append_block(tda_ctx* c, int m, int nc, bool reorth, double& eta) {
     double grow = 0.0;
     for (int j = 0; j < nc; ++j)
       if (wn[j] > 1e-8) grow = std::max(grow, pn[j] / wn[j]);
-    const double eta_new = (eta + 2.3e-16) * grow;
-    need = c->reorth_mode == 1 || eta_new > c->eta_tol;
+    const double eps = v32 ? 6e-8 : 2.3e-16;
+    const double eta_new = (eta + eps) * grow;
+    // an FP32 basis cannot be orthogonal beyond its rounding: the second
+    // pass is for losses well above it
+    need = c->reorth_mode == 1 || eta_new > (v32 ? c->eta_tol32 : c->eta_tol);
     if (!need) eta = std::max(eta, eta_new);
   }
   if (need && h_count > 0) {
     ++c->reorth_passes;
-    GemmArgs r0 = gargs(V, Q, C0, m, h_count, L, plain(L), plain(1), plain(1), plain(L), plain(h_count), plain(1));
-    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, r0);
-    GemmArgs r1 = gargs(C0, V, Q, h_count, L, m, plain(1), plain(h_count), plain(L), plain(1), plain(L), plain(1));
-    r1.alpha = -1.0;
-    r1.beta = 1.0;
-    run_gemm(c, DT_F64, DT_F64, EPI_STORE, false, false, r1);
+    bdot(c, Vb, v32, m, Q, false, h_count, C0, h_count);
+    bcomb(c, C0, 1, h_count, h_count, m, Vb, v32, Q, -1.0, 1.0);
     launch_copy(Q, P, (int64_t)h_count * L, c->stream);
     launch_copy(Q, W, (int64_t)h_count * L, c->stream);
     h_count = gs_block(c, P, W, Q, h_count, nullptr);
   }
+  if (v32 && h_count > 0) launch_cast_vectors(Q, (int64_t)h_count * L, V32 + (size_t)m * L, PREC_FP32, c->stream);
   return h_count;
 }
 
@@ -1181,8 +1240,15 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
                    int stop_stall, std::vector<double>& h_evals, std::vector<double>& h_err, int cap_in = 0,
                    int keep = 0, int* kk_out = nullptr, double stall_factor = 0.5,
                    double stall_floor = 1e300, bool thick_restart = true, double floor_k = 0.0,
-                   bool reorth = false, bool verify = false) {
+                   bool reorth = false, bool verify = false, bool v32 = false) {
   const int64_t L = c->ovp;
+  // v32: V and AV hold FP32 vectors (V32 / AV32 over the same allocations);
+  // the start vectors V[0, m0) arrive as FP64 and are rounded here.  RZ, R,
+  // P, W and every exported vector stay FP64.
+  TDA_CHECK(!v32 || (!verify && prec != PREC_FP64 && prec != PREC_F16X && !c->dense), TDA_ERR_BAD_ARG,
+            "Davidson: an FP32 basis is for the FP16 / BF16 phase only");
+  float* V32 = (float*)c->V.p;
+  float* AV32 = (float*)c->AV.p;
   const int cap = cap_in > 0 ? cap_in : 15 * n;  // max_space (davidson.rs:74)
   const int ldh = c->space(n);                    // leading dimension of H and Y
   TDA_CHECK(cap <= ldh, TDA_ERR_BAD_ARG,
@@ -1199,7 +1265,11 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
   double* W = c->Wb();
   int m = m0, n_done = 0, dim_sub = m0;
   const int dim_sub_origin = m0;
-  double eta = 1e-15;  // estimated loss of orthogonality of the basis
+  double eta = v32 ? 6e-8 : 1e-15;  // estimated loss of orthogonality of the basis
+  if (v32) {
+    launch_copy(V, W, (int64_t)m0 * L, c->stream);  // FP64 copy first: V32 overlays V
+    launch_cast_vectors(W, (int64_t)m0 * L, V32, PREC_FP32, c->stream);
+  }
   DavResult res;
   std::vector<int> idx;
   double best = 1e300, prev_ritz_sum = 1e300;
@@ -1219,8 +1289,7 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       subspace_eigh(c, H, ldh, m_last, kk, c->evals.as<double>(), Y);
     }
     if (kk <= n) return;
-    GemmArgs gw = gargs(Y, V, W, kk, L, m_last, plain(ldh), plain(1), plain(L), plain(1), plain(L), plain(1));
-    run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, gw);
+    bcomb(c, Y, ldh, 1, kk, m_last, v32 ? (const void*)V32 : (const void*)V, v32, W);
     *kk_out = kk;
   };
   for (int it = 0; it < max_iter; ++it) {
@@ -1234,27 +1303,28 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       n_done = 0;
       proj_from = 0;
     }
-    const int n_eig = thick ? std::min(keep, m) : n;
+    // n eigenpairs per iteration; a thick restart (and the export at the
+    // end of a thick phase) solves the projection again for its keep
+    // lowest pairs
+    const int n_eig = n;
     n_eig_last = n_eig;
     m_last = m;
     {
       Timer t;
       TDA_HIP(hipEventRecord(t.a, c->stream));
-      products(c, V + (size_t)n_done * L, m - n_done, AV + (size_t)n_done * L, prec);
+      if (v32) products(c, V32 + (size_t)n_done * L, m - n_done, AV32 + (size_t)n_done * L, prec, true);
+      else products(c, V + (size_t)n_done * L, m - n_done, AV + (size_t)n_done * L, prec);
       TDA_HIP(hipEventRecord(t.b, c->stream));
       // 1.1 projection H[p][q] = V_p . AV_q for the new q (davidson.rs:89)
-      GemmArgs g = gargs(V, AV + (size_t)proj_from * L, H + proj_from, m, m - proj_from, L, plain(L), plain(1),
-                         plain(1), plain(L), plain(ldh), plain(1));
-      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, true, g);
+      if (v32) bdot(c, V32, true, m, AV32 + (size_t)proj_from * L, true, m - proj_from, H + proj_from, ldh);
+      else bdot(c, V, false, m, AV + (size_t)proj_from * L, false, m - proj_from, H + proj_from, ldh);
       n_done = m;
       // 2. subspace eigenproblem (davidson.rs:93-97)
       subspace_eigh(c, H, ldh, m, n_eig, c->evals.as<double>(), Y);
       // 3.1 Ritz vectors RZ = Y V, 3.2 residuals R = Y AV - u RZ, 3.3 norms
       // (davidson.rs:101-107)
-      GemmArgs gr = gargs(Y, V, RZ, n, L, m, plain(ldh), plain(1), plain(L), plain(1), plain(L), plain(1));
-      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, gr);
-      GemmArgs ga = gargs(Y, AV, R, n, L, m, plain(ldh), plain(1), plain(L), plain(1), plain(L), plain(1));
-      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, ga);
+      bcomb(c, Y, ldh, 1, n, m, v32 ? (const void*)V32 : (const void*)V, v32, RZ);
+      bcomb(c, Y, ldh, 1, n, m, v32 ? (const void*)AV32 : (const void*)AV, v32, R);
       launch_residual(R, RZ, c->evals.as<double>(), n, L, c->stream);
       launch_norms(R, n, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
       h_evals.resize(n);
@@ -1288,6 +1358,10 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       // vectors: only products and what is derived from them went bad)
       res.nonfinite = true;
       if (kk_out) *kk_out = n;
+      if (v32) {  // the restart vectors back to FP64 in V[0, n)
+        launch_widen(V32, (int64_t)n * L, W, c->stream);
+        launch_copy(W, V, (int64_t)n * L, c->stream);
+      }
       return res;
     }
     if (!finite)
@@ -1324,9 +1398,14 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       return res;
     }
     if (floor_k > 0 && prec != PREC_FP64 && !c->dense && it == 0) {
-      // product error of this precision on the lowest Ritz vector
+      // product error of this precision on the lowest Ritz vector, against
+      // FP64 products or, for FP16 / BF16 products of the AO operator, the
+      // FP16 hi / lo split ones (error ~1e-7, three orders below theirs;
+      // a third of an FP64 product's time)
+      const bool ao = c->has_lc && c->op_kind == 0 && !c->ct;
+      const int ref_prec = (prec == PREC_FP16 || prec == PREC_BF16) && ao && c->split32 ? PREC_F16X : PREC_FP64;
       products(c, RZ, 1, P, prec);
-      products(c, RZ, 1, W, PREC_FP64);
+      products(c, RZ, 1, W, ref_prec);
       c->one.ensure(8);
       launch_fill(c->one.as<double>(), 1.0, 1, c->stream);
       launch_residual(W, P, c->one.as<double>(), 1, L, c->stream);  // W <- W - P
@@ -1349,10 +1428,11 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
     const bool ritz_moving = prev_ritz_sum - ritz_sum > n * std::max(res.delta, tol);
     prev_ritz_sum = ritz_sum;
     if (stop_stall > 0 && max_err < stall_floor) {
-      if (max_err < stall_factor * best || ritz_moving) {
+      const bool near = res.delta > 0 && max_err < c->near_k * res.delta;
+      if (max_err < stall_factor * best || (ritz_moving && !near)) {
         best = std::min(best, max_err);
         stall = 0;
-      } else if (++stall >= stop_stall) {
+      } else if (++stall >= (near ? std::min(stop_stall, c->near_stall) : stop_stall)) {
         finish(false);
         return res;
       }
@@ -1375,25 +1455,30 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
           const int nb = std::min(64, nc - b0);
           launch_precondition_many(R, c->evals.as<double>(), c->idx.as<int>() + b0, nb, c->de.as<double>(), L,
                                    c->pre_thr, P, c->stream);
-          m += append_block(c, m, nb, reorth, eta);
+          m += append_block(c, m, nb, reorth, eta, v32);
         }
       }
     } else if (thick) {
       // thick restart: rotate basis and products onto the kk lowest Ritz vectors
-      const int kk = n_eig;
-      GemmArgs gv = gargs(Y, V, W, kk, L, m, plain(ldh), plain(1), plain(L), plain(1), plain(L), plain(1));
-      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, gv);
-      GemmArgs gav = gargs(Y, AV, P, kk, L, m, plain(ldh), plain(1), plain(L), plain(1), plain(L), plain(1));
-      run_gemm(c, DT_F64, DT_F64, EPI_STORE, true, false, gav);
-      launch_copy(W, V, (int64_t)kk * L, c->stream);
-      launch_copy(P, AV, (int64_t)kk * L, c->stream);
+      const int kk = std::min(keep, m);
+      if (kk > n_eig) subspace_eigh(c, H, ldh, m, kk, c->evals.as<double>(), Y);
+      bcomb(c, Y, ldh, 1, kk, m, v32 ? (const void*)V32 : (const void*)V, v32, W);
+      bcomb(c, Y, ldh, 1, kk, m, v32 ? (const void*)AV32 : (const void*)AV, v32, P);
+      if (v32) {
+        launch_cast_vectors(W, (int64_t)kk * L, V32, PREC_FP32, c->stream);
+        launch_cast_vectors(P, (int64_t)kk * L, AV32, PREC_FP32, c->stream);
+      } else {
+        launch_copy(W, V, (int64_t)kk * L, c->stream);
+        launch_copy(P, AV, (int64_t)kk * L, c->stream);
+      }
       m = n_done = dim_sub = kk;
       rotated = true;
     } else {
       // collapse to the Ritz vectors (davidson.rs:172-176); RZ holds n of
       // them, so a phase started from more than n vectors collapses to n
       dim_sub = std::min(dim_sub_origin, n);
-      launch_copy(RZ, V, (int64_t)dim_sub * L, c->stream);
+      if (v32) launch_cast_vectors(RZ, (int64_t)dim_sub * L, V32, PREC_FP32, c->stream);
+      else launch_copy(RZ, V, (int64_t)dim_sub * L, c->stream);
       m = dim_sub;
       n_done = 0;
     }
@@ -1523,6 +1608,14 @@ int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx**
     if (er && (atoi(er) == 0 || atoi(er) == 1)) c->reorth_mode = atoi(er);
     const char* ee = getenv("TINCR_TDA_ETA_TOL");
     if (ee && atof(ee) > 0) c->eta_tol = atof(ee);
+    const char* eb = getenv("TINCR_TDA_BASIS32");
+    if (eb) c->basis32 = atoi(eb) != 0;
+    const char* e32 = getenv("TINCR_TDA_ETA_TOL32");
+    if (e32 && atof(e32) > 0) c->eta_tol32 = atof(e32);
+    const char* enk = getenv("TINCR_TDA_NEAR_K");
+    if (enk && atof(enk) >= 0) c->near_k = atof(enk);
+    const char* ens = getenv("TINCR_TDA_NEAR_STALL");
+    if (ens && atoi(ens) >= 1) c->near_stall = atoi(ens);
   }
   if (hbm_budget_bytes) c->budget = hbm_budget_bytes;
   int rc = guarded(c, [&] {
@@ -2197,8 +2290,9 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       bool done = false, lp_overflow = false;
       int start = n, kk = n;
       if ((c->prec == PREC_FP16 || c->prec == PREC_BF16) && c->lp_phase) {
+        const bool v32 = c->basis32 && c->has_lc && c->op_kind == 0 && !c->ct;
         res = davidson(c, n, n, max_iter, tol, c->prec, false, lp_stall_window(c), h_evals, h_err, c->lp_cap(n),
-                       c->lp_keep(n), &kk, 0.8, 1e300, true, c->floor_k, true);
+                       c->lp_keep(n), &kk, 0.8, 1e300, true, c->floor_k, true, false, v32);
         used += res.iters;
         done = res.converged;
         lp_overflow = res.nonfinite;
