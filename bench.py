@@ -218,9 +218,9 @@ def _frag(d):
                                           "orbs", "s", "orbe", "gamma")})
 
 
-def fmo_batch(prec, mons, cts, n_states, max_iter, tol, operator, workers):
+def fmo_batch(prec, mons, cts, n_states, max_iter, tol, operator, workers, mode="grouped"):
     from tincr_amd.tda import TdaBatch
-    b = TdaBatch(prec, n_workers=workers, operator=operator)
+    b = TdaBatch(prec, n_workers=workers, operator=operator, mode=mode)
     for m in mons:
         o, v = len(m["occ_indices"]), len(m["virt_indices"])
         b.add_system(m["n_atoms"], m["orbs_per_atom"], m["xyz"], m["occ_indices"], m["virt_indices"], m["orbs"],
@@ -294,11 +294,14 @@ def main_fmo(args):
     prec = args.prec if args.prec_given else "fp32"
     tol, max_iter = args.tol, 150
 
+    totals = {}
+
     def run(p):
-        b = fmo_batch(p, mons, cts, n, max_iter, tol, args.operator, args.workers)
+        b = fmo_batch(p, mons, cts, n, max_iter, tol, args.operator, args.workers, args.batch_mode)
         t = time.perf_counter()
         res = b.run()
         t = time.perf_counter() - t
+        totals[p] = b.totals
         b.close()
         return t, res
 
@@ -316,10 +319,16 @@ def main_fmo(args):
     dw = max(float(np.abs(a["ci_eigenvalues"] - c["ci_eigenvalues"]).max()) for a, c in zip(res, r64))
     df = max(float(np.abs(a["oscillator_strengths"] - c["oscillator_strengths"]).max()) for a, c in zip(res, r64))
     tot_ms, tot_fl, tot_n = np.zeros((2, 3)), np.zeros((2, 3)), np.zeros((2, 3))
-    for r in res:
-        tot_ms += np.array(r["stats"]["exch_ms"])
-        tot_fl += np.array(r["stats"]["exch_flops"])
-        tot_n += np.array(r["stats"]["exch_launches"])
+    if args.batch_mode == "grouped":  # the grouped exchange launches of the last timed batch
+        t = totals[prec]
+        tot_ms += np.array(t["exch_ms"])
+        tot_fl += np.array(t["exch_flops"])
+        tot_n += np.array(t["exch_launches"])
+    else:
+        for r in res:
+            tot_ms += np.array(r["stats"]["exch_ms"])
+            tot_fl += np.array(r["stats"]["exch_flops"])
+            tot_n += np.array(r["stats"]["exch_launches"])
     si, pi = np.unravel_index(np.argmax(tot_ms), tot_ms.shape)
     split = any(r["stats"].get("split_fp32", 0) for r in res)
     kprec = class_prec(pi, prec, split)
@@ -333,7 +342,8 @@ def main_fmo(args):
         "config": {"workload": (f"batched FMO: {len(mons)} fragments (H2O, (H2O)2, C4-C40 polyenes, >= 3 A "
                                 f"apart) + {len(cts)} nearest CT pairs, lowest {n} states each (clamped to ov), "
                                 f"tol {tol}"),
-                   "precision": prec, "operator": args.operator, "workers": args.workers,
+                   "precision": prec, "batch_mode": args.batch_mode,
+                   "operator": "charges" if args.batch_mode == "grouped" else args.operator, "workers": args.workers,
                    "n_items": len(res), "n_states": n_states},
         "fp64": {"states_per_s": round(n_states / t64, 3), "ms_per_batch": round(1e3 * t64, 1)},
         "speedup_vs_fp64": round(value / (n_states / t64), 3),
@@ -341,7 +351,10 @@ def main_fmo(args):
         "input_generation_s": round(t_gen, 1),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[kprec],
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[kprec], 5), "traffic": None,
-                     "kernel": KERNEL_NAMES[args.operator][si] + f" ({kprec} MFMA), summed over fragments",
+                     "kernel": (KERNEL_NAMES["charges" if args.batch_mode == "grouped" else args.operator][si]
+                                + f" ({kprec} MFMA), " + ("grouped launches over all fragments"
+                                                          if args.batch_mode == "grouped" else
+                                                          "summed over fragments")),
                      "launches": int(tot_n[si, pi])},
     }
     if not args.no_cpu:
@@ -367,6 +380,8 @@ def main():
                     help="scc: converged LC-SCC orbitals of the geometry (default); synthetic: seeded random "
                          "orthonormal orbitals with S = I (throughput only)")
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--batch-mode", default="grouped", choices=["grouped", "pool"],
+                    help="--workload fmo: one grouped Davidson for all fragments, or a worker pool of contexts")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 256 MiB calibration copy first (reference for counter runs)")
     args = ap.parse_args()

@@ -230,18 +230,32 @@ int tda_adjoint_nuclear(tda_ctx* ctx, int n_states, const double* X, const doubl
 /* ------------------------------------------------------------------------
  * Batched fragments: the FMO monomer loop (src/fmo/lcmo/basis.rs:462-465,
  * prepare_tda + run_tda per monomer) and the CT-pair loop (basis.rs:172-238,
- * run_ct_tda per pair) in one call.  Worker threads, one context and HIP
- * stream each, take the fragments largest estimated cost first; each fragment
- * converges on its own.  Input arrays are read during tda_batch_run only and
- * must stay valid until it returns; outputs are written there.
+ * run_ct_tda per pair) in one call; each fragment converges on its own.
+ * TDA_BATCH_GROUPED (default): the fragments' operators (per-atom charges
+ * form) are prepared by worker threads, then one Davidson runs for all of
+ * them together: every step is one grouped launch per operation over the
+ * fragments still active (products, projections, subspace eigensolves,
+ * residuals, Gram-Schmidt), converged fragments drop out.  FP64 batches use
+ * FP64 products; the other precisions FP32 products with an FP64
+ * verification of each fragment's roots (subspace algebra always FP64).
+ * TDA_BATCH_POOL: worker threads, one context and HIP stream each, solve
+ * whole fragments (tda_run), largest estimated cost first.
+ * Input arrays are read during tda_batch_run only and must stay valid until
+ * it returns; outputs are written there.
  * ------------------------------------------------------------------------ */
 typedef struct tda_batch tda_batch;
+#define TDA_BATCH_GROUPED 0
+#define TDA_BATCH_POOL 1
 
 int tda_batch_create(int device, int n_workers, size_t hbm_budget_bytes, int precision, tda_batch** out);
 void tda_batch_destroy(tda_batch* batch);
 const char* tda_batch_last_error(const tda_batch* batch);
-/* operator form of the monomer solves (TDA_OP_*); CT pairs use the charges form */
+/* operator form of the monomer solves (TDA_OP_*); CT pairs use the charges
+ * form, and so does the grouped driver for every fragment */
 int tda_batch_set_operator(tda_batch* batch, int op);
+/* TDA_BATCH_GROUPED or TDA_BATCH_POOL (no counterpart in the reference, whose
+ * loops are rayon iterators over the fragments) */
+int tda_batch_set_mode(tda_batch* batch, int mode);
 /* A monomer (tda_set_system arguments) solved as run_tda(n_roots, max_iter,
  * tol) with tda_run's outputs.  Returns the item index (>= 0) or -status. */
 int tda_batch_add_system(tda_batch* batch, int n_atoms, int n_ao, const int32_t* orbs_per_atom, const double* orbs,
@@ -259,6 +273,11 @@ int tda_batch_add_ct_pair(tda_batch* batch, int n_atoms_h, int n_ao_h, const int
                           double* omega, double* X, double* q_trans, double* tr_dipoles, double* f);
 /* Forget the added items. */
 int tda_batch_clear(tda_batch* batch);
+/* Totals of the last grouped tda_batch_run (no counterpart in the reference):
+ * exch_ms / exch_launches / exch_flops of the grouped exchange launches,
+ * n_iter = the largest fragment iteration count, n_products summed,
+ * t_total_ms the run's wall time; zero after a pool run. */
+int tda_batch_last_totals(const tda_batch* batch, tda_stats* out);
 /* Solves every item.  status [n_items] (optional) receives each item's
  * tda_run status, stats [n_items] (optional) its statistics.  Returns TDA_OK,
  * TDA_ERR_NOT_CONVERGED if some item did not converge (outputs hold its last

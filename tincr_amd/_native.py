@@ -27,7 +27,7 @@ EXPORTS = [
     "tda_trans_charges", "tda_trans_charges_noalloc", "tda_orbe_differences", "tda_set_system", "tda_set_ct_pair", "tda_set_dense", "tda_prepare",
     "tda_clear_charges", "tda_get_charges", "tda_compute_products", "tda_precondition", "tda_run",
     "tda_adjoint", "tda_adjoint_nuclear", "tda_zvector",
-    "tda_batch_create", "tda_batch_destroy", "tda_batch_last_error", "tda_batch_set_operator",
+    "tda_batch_create", "tda_batch_destroy", "tda_batch_last_error", "tda_batch_set_operator", "tda_batch_set_mode", "tda_batch_last_totals",
     "tda_batch_add_system", "tda_batch_add_ct_pair", "tda_batch_clear", "tda_batch_run",
 ]
 
@@ -112,6 +112,8 @@ def load():
     lib.tda_batch_last_error.argtypes = [P]
     lib.tda_batch_last_error.restype = ctypes.c_char_p
     lib.tda_batch_set_operator.argtypes = [P, i]
+    lib.tda_batch_set_mode.argtypes = [P, i]
+    lib.tda_batch_last_totals.argtypes = [P, ctypes.POINTER(TdaStats)]
     lib.tda_batch_add_system.argtypes = [P, i, i, I32, D, D, D, i, i, i, i, D, D, D, i, i, ctypes.c_double,
                                          D, D, D, D, D]
     lib.tda_batch_add_ct_pair.argtypes = [P, i, i, I32, D, D, D, i, i, i, i, I32, D, D, D, i, i, D, D, D, D, D, D,

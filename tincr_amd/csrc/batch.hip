@@ -1,10 +1,12 @@
 // Batched excited-state solves of many fragments: the FMO monomer loop
 // (src/fmo/lcmo/basis.rs:462-465, prepare_tda + run_tda per monomer on a rayon
 // pool) and the charge-transfer pair loop (basis.rs:172-238, run_ct_tda per
-// ordered pair).  Each worker thread owns one context (one HIP stream) and
+// ordered pair).  The C entry points and the worker-pool driver
+// (TDA_BATCH_POOL): each worker thread owns one context (one HIP stream) and
 // takes the next unsolved fragment, largest estimated cost first, so that the
 // fragments' kernels overlap on the GPU and the long solves start early.
-// Every fragment converges (or fails) on its own; the others are unaffected.
+// The default driver is batch_grouped.hip.  Every fragment converges (or
+// fails) on its own; the others are unaffected.
 #include <algorithm>
 #include <atomic>
 #include <cstring>
@@ -15,39 +17,10 @@
 #include <vector>
 
 #include "../../include/tincr_tda.h"
+#include "batch.h"
 
-namespace {
+using tda::BatchItem;
 
-struct BatchItem {
-  bool ct = false;
-  // monomer / system (tda_set_system arguments)
-  int n_atoms = 0, n_ao = 0, ob = 0, o = 0, vb = 0, v = 0;
-  const int32_t* opa = nullptr;
-  const double *orbs = nullptr, *s = nullptr, *orbe = nullptr, *gamma = nullptr, *gamma_lr = nullptr;
-  // CT pair (tda_set_ct_pair arguments)
-  int n_atoms_l = 0, n_ao_l = 0;
-  const int32_t* opa_l = nullptr;
-  const double *orbs_l = nullptr, *s_l = nullptr, *orbe_l = nullptr, *s_hl = nullptr, *gamma_l = nullptr,
-               *gamma_hl = nullptr;
-  const double* xyz = nullptr;
-  // solve and outputs
-  int n_roots = 0, max_iter = 0;
-  double tol = 0;
-  double *omega = nullptr, *X = nullptr, *q_trans = nullptr, *tr_dipoles = nullptr, *f = nullptr;
-  double cost = 0;
-};
-
-}  // namespace
-
-struct tda_batch {
-  int device = 0;
-  int n_workers = 1;
-  size_t budget = 0;
-  int precision = TDA_PREC_FP64;
-  int op = TDA_OP_AO;
-  std::vector<BatchItem> items;
-  std::string err;
-};
 
 namespace {
 
@@ -92,6 +65,18 @@ const char* tda_batch_last_error(const tda_batch* b) { return b ? b->err.c_str()
 int tda_batch_set_operator(tda_batch* b, int op) {
   if (!b || (op != TDA_OP_AO && op != TDA_OP_CHARGES)) return TDA_ERR_BAD_ARG;
   b->op = op;
+  return TDA_OK;
+}
+
+int tda_batch_last_totals(const tda_batch* b, tda_stats* out) {
+  if (!b || !out) return TDA_ERR_BAD_ARG;
+  *out = b->last;
+  return TDA_OK;
+}
+
+int tda_batch_set_mode(tda_batch* b, int mode) {
+  if (!b || (mode != TDA_BATCH_GROUPED && mode != TDA_BATCH_POOL)) return TDA_ERR_BAD_ARG;
+  b->mode = mode;
   return TDA_OK;
 }
 
@@ -174,6 +159,19 @@ int tda_batch_run(tda_batch* b, int32_t* status, tda_stats* stats) {
   if (!b) return TDA_ERR_BAD_ARG;
   const int n = (int)b->items.size();
   if (n == 0) return TDA_OK;
+  if (b->mode == TDA_BATCH_GROUPED) {
+    std::vector<int32_t> st(n, TDA_ERR_STATE);
+    const int rc =
+        tda::batch_run_grouped(b->device, b->precision, b->budget, b->items, st.data(), stats, &b->last, b->err);
+    if (status) std::memcpy(status, st.data(), (size_t)n * sizeof(int32_t));
+    if (rc != TDA_OK) return rc;
+    for (int i = 0; i < n; ++i)
+      if (st[i] != TDA_OK) {
+        b->err = "fragment " + std::to_string(i) + ": Davidson Algorithm did not converge!";
+        return st[i];
+      }
+    return TDA_OK;
+  }
   std::vector<int> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->items[a].cost > b->items[c].cost; });

@@ -152,10 +152,9 @@ __device__ inline double wave_max(double v) {
 }
 
 template <int MAXM, int POOL>
-__global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __restrict__ H, int ldh, int m, int n,
-                                                            double* __restrict__ work, double* __restrict__ evals,
-                                                            double* __restrict__ Y, int ldy,
-                                                            unsigned long long* __restrict__ clk) {
+__device__ __forceinline__ void eigh_body(const double* __restrict__ H, int ldh, int m, int n,
+                                          double* __restrict__ work, double* __restrict__ evals,
+                                          double* __restrict__ Y, int ldy, unsigned long long* __restrict__ clk) {
 #define EIGH_STAMP(i) \
   if (clk && threadIdx.x == 0) clk[i] = wall_clock64();
   __shared__ double pool[POOL];
@@ -555,6 +554,30 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
   __syncthreads();
   EIGH_STAMP(5)
 #undef EIGH_STAMP
+}
+
+template <int MAXM, int POOL>
+__global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __restrict__ H, int ldh, int m, int n,
+                                                            double* __restrict__ work, double* __restrict__ evals,
+                                                            double* __restrict__ Y, int ldy,
+                                                            unsigned long long* __restrict__ clk) {
+  eigh_body<MAXM, POOL>(H, ldh, m, n, work, evals, Y, ldy, clk);
+}
+
+// Many independent subspace problems (the fragments of a batch): workgroup b
+// solves descriptor b < *count; the grid is fixed (a captured launch replays
+// with new descriptors).
+__global__ void __launch_bounds__(EIGH_THREADS) eigh_batched_kernel(const EighDesc* __restrict__ d,
+                                                                    const int* __restrict__ count) {
+  if ((int)blockIdx.x >= *count) return;
+  const EighDesc e = d[blockIdx.x];
+  eigh_body<EIGH_MAXM_SMALL, EIGH_POOL_SMALL>(e.H, e.ldh, e.m, e.n, e.work, e.evals, e.Y, e.ldy, nullptr);
+}
+
+void launch_eigh_batched(const EighDesc* d, const int* count, int grid, hipStream_t s) {
+  if (grid <= 0) return;
+  eigh_batched_kernel<<<grid, EIGH_THREADS, 0, s>>>(d, count);
+  TDA_LAUNCH_CHECK();
 }
 
 void launch_eigh(const double* H, int ldh, int m, int n, double* work, double* evals, double* Y, int ldy,

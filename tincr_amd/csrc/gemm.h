@@ -71,6 +71,12 @@ __host__ __device__ inline int64_t hada_index(const GemmArgs& g, int64_t x) {
   return g.hidx[r];
 }
 
+// Tile configurations of the generic kernel (gemm_impl.h): 128x128, 64x32,
+// 32x128, 64x64, 128x128 with 128-deep K slices (16-bit inputs), 64x16 with
+// 64-deep K slices.  CFG_SKINNY_N16 is instantiated for FP64 inputs only
+// (other inputs fall back to CFG_SKINNY_N).
+enum CfgId : int { CFG_BIG = 0, CFG_SKINNY_N = 1, CFG_SKINNY_M = 2, CFG_SMALL = 3, CFG_BIG_K = 4, CFG_SKINNY_N16 = 5 };
+
 // Element type of an operand / result buffer.
 enum DType : int { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2, DT_F16 = 3 };
 

@@ -148,7 +148,8 @@ __device__ inline double epi_value(const GemmArgs& g, int zb, int64_t m, int64_t
 // type TI (FP32-stored operands of an FP64 product: each element is widened
 // when the slice is written to LDS, so the arithmetic is TI's).
 template <typename TI, typename TO, typename Cfg, bool A_KC, bool B_KC, int EPI, typename LA = TI, typename LB = TI>
-__global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs g) {
+__device__ __forceinline__ void gemm_tile(const GemmArgs& g, const int zb, const int zs, const int64_t m0,
+                                          const int64_t n0) {
   using MF = Mfma<TI>;
   using TA = typename MF::TA;
   constexpr int BM = Cfg::BM, BN = Cfg::BN, WGM = Cfg::WGM, WGN = Cfg::WGN;
@@ -173,36 +174,7 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
   const int wm0 = (wave / WGN) * WTM;
   const int wn0 = (wave % WGN) * WTN;
 
-#if TDA_XCD_REMAP
-  // Workgroup w of the launch is dispatched to XCD w % 8, each XCD with its
-  // own L2.  Give every XCD a contiguous run of the tile sequence, and walk
-  // the tiles in groups of 8 tile rows, so that the tiles sharing A rows / B
-  // columns run on the same L2 at about the same time.
-  int64_t w = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
-  {
-    const int64_t nwg = (int64_t)gridDim.x * gridDim.y * gridDim.z;
-    const int64_t q = nwg / 8, r = nwg % 8, x = w % 8, j = w / 8;
-    w = (x < r) ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
-  }
-  const int64_t per_z = (int64_t)gridDim.x * gridDim.y;
-  const int zz = (int)(w / per_z);
-  const int64_t t = w - (int64_t)zz * per_z;
-  constexpr int GM = 8;
-  const int64_t gsize = (int64_t)GM * gridDim.x;
-  const int64_t first_m = (t / gsize) * GM;
-  const int64_t gm = min((int64_t)gridDim.y - first_m, (int64_t)GM);
-  const int64_t tile_m = first_m + (t % gsize) % gm;
-  const int64_t tile_n = (t % gsize) / gm;
-  const int zb = zz / g.ksplit;
-  const int zs = zz % g.ksplit;
-  const int64_t m0 = tile_m * BM;
-  const int64_t n0 = tile_n * BN;
-#else
-  const int zb = blockIdx.z / g.ksplit;
-  const int zs = blockIdx.z % g.ksplit;
-  const int64_t m0 = (int64_t)blockIdx.y * BM;
-  const int64_t n0 = (int64_t)blockIdx.x * BN;
-#endif
+
   const LA* A = (const LA*)g.A + zb * g.a_batch;
   const LB* B = (const LB*)g.B + zb * g.b_batch;
 
@@ -503,6 +475,61 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
     }
 }
 
+template <typename TI, typename TO, typename Cfg, bool A_KC, bool B_KC, int EPI, typename LA = TI, typename LB = TI>
+__global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs g) {
+  constexpr int BM = Cfg::BM, BN = Cfg::BN;
+#if TDA_XCD_REMAP
+  // Workgroup w of the launch is dispatched to XCD w % 8, each XCD with its
+  // own L2.  Give every XCD a contiguous run of the tile sequence, and walk
+  // the tiles in groups of 8 tile rows, so that the tiles sharing A rows / B
+  // columns run on the same L2 at about the same time.
+  int64_t w = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+  {
+    const int64_t nwg = (int64_t)gridDim.x * gridDim.y * gridDim.z;
+    const int64_t q = nwg / 8, r = nwg % 8, x = w % 8, j = w / 8;
+    w = (x < r) ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
+  }
+  const int64_t per_z = (int64_t)gridDim.x * gridDim.y;
+  const int zz = (int)(w / per_z);
+  const int64_t t = w - (int64_t)zz * per_z;
+  constexpr int GM = 8;
+  const int64_t gsize = (int64_t)GM * gridDim.x;
+  const int64_t first_m = (t / gsize) * GM;
+  const int64_t gm = min((int64_t)gridDim.y - first_m, (int64_t)GM);
+  const int64_t tile_m = first_m + (t % gsize) % gm;
+  const int64_t tile_n = (t % gsize) / gm;
+  const int zb = zz / g.ksplit;
+  const int zs = zz % g.ksplit;
+  const int64_t m0 = tile_m * BM;
+  const int64_t n0 = tile_n * BN;
+#else
+  const int zb = blockIdx.z / g.ksplit;
+  const int zs = blockIdx.z % g.ksplit;
+  const int64_t m0 = (int64_t)blockIdx.y * BM;
+  const int64_t n0 = (int64_t)blockIdx.x * BN;
+#endif
+  gemm_tile<TI, TO, Cfg, A_KC, B_KC, EPI, LA, LB>(g, zb, zs, m0, n0);
+}
+
+// Grouped launch: many independent GEMMs (one GemmArgs each, e.g. one per
+// fragment of a batch) in one persistent launch.  tiles[t] = {descriptor,
+// batch index z, tile row, tile column}; the workgroups stride over the
+// *count tiles (no split-K).  The grid is fixed, so a captured launch
+// replays with new descriptors and tile lists.
+template <typename TI, typename TO, typename Cfg, bool A_KC, bool B_KC, int EPI, typename LA = TI, typename LB = TI>
+__global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS)
+    gemm_grouped_kernel(const GemmArgs* __restrict__ descs, const int4* __restrict__ tiles,
+                        const int* __restrict__ count) {
+  const int nt = *count;
+  for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+    const int4 tl = tiles[t];
+    gemm_tile<TI, TO, Cfg, A_KC, B_KC, EPI, LA, LB>(descs[tl.x], tl.y, 0, (int64_t)tl.z * Cfg::BM,
+                                                     (int64_t)tl.w * Cfg::BN);
+    __syncthreads();  // LDS reuse by the next tile
+  }
+}
+
+
 // Sum split-K partials and apply the epilogue.
 // Many partials per output (ksplit >= 16): one wave per output element, the
 // 64 lanes split the partials and combine them with a wave sum.
@@ -552,9 +579,6 @@ __global__ void __launch_bounds__(256) splitk_reduce_kernel(GemmArgs g) {
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-// CFG_SKINNY_N16 is instantiated for FP64 inputs only (other inputs fall
-// back to CFG_SKINNY_N).
-enum CfgId : int { CFG_BIG = 0, CFG_SKINNY_N = 1, CFG_SKINNY_M = 2, CFG_SMALL = 3, CFG_BIG_K = 4, CFG_SKINNY_N16 = 5 };
 
 inline int cfg_bm(int c) {
   switch (c) {

@@ -148,5 +148,15 @@ void launch_dipoles(const double* xyz, const double* qt, const double* w, int N,
 // (copy, tridiagonalisation, bisection, inverse iteration, back-transform).
 void launch_eigh(const double* H, int ldh, int m, int n, double* work, double* evals, double* Y, int ldy,
                  hipStream_t s, unsigned long long* phase_clk = nullptr);
+// Batched form for the fragments of a batch: descriptor b (m <= 256,
+// n <= min(m, 64), work m*m + m doubles) solved by workgroup b < *count.
+struct EighDesc {
+  const double* H;
+  double* work;
+  double* evals;
+  double* Y;
+  int ldh, m, n, ldy;
+};
+void launch_eigh_batched(const EighDesc* d, const int* count, int grid, hipStream_t s);
 
 }  // namespace tda

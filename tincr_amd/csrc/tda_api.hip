@@ -23,6 +23,7 @@
 
 #include "../../include/tincr_tda.h"
 #include "gemm.h"
+#include "internal.h"
 #include "kernels.h"
 
 using namespace tda;
@@ -2395,3 +2396,46 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// library-internal (internal.h): the prepared charges-form operator of a
+// context, for the grouped batch driver
+// ---------------------------------------------------------------------------
+namespace tda {
+
+int ctx_charges_dims(tda_ctx* c, ChargesDims* d) {
+  if (!c || !d) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] {
+    TDA_CHECK(c->has_system && !c->dense, TDA_ERR_STATE, "no system loaded");
+    TDA_CHECK(c->ct || c->op_kind == 1 || !c->has_lc, TDA_ERR_STATE, "charges form expected");
+    d->N = c->N;
+    d->NL = c->has_lc ? c->Nx : 0;
+    d->o = c->o;
+    d->v = c->v;
+    d->op = c->op;
+    d->vp = c->vp;
+    d->lc = c->has_lc;
+  });
+}
+
+int ctx_export_charges(tda_ctx* c, double* qov, double* gamma, double* G, double* qvv, double* de) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] {
+    TDA_CHECK(c->ct || c->op_kind == 1 || !c->has_lc, TDA_ERR_STATE, "charges form expected");
+    prepare(c);
+    TDA_CHECK(c->qov_built && (!c->has_lc || c->mo_built), TDA_ERR_STATE, "operator not prepared");
+    const size_t N = c->N;
+    if (qov) TDA_HIP(hipMemcpyAsync(qov, c->qov.p, N * c->ovp * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (gamma) TDA_HIP(hipMemcpyAsync(gamma, c->gamma.p, N * N * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (c->has_lc && G)
+      TDA_HIP(hipMemcpyAsync(G, c->G.p, (size_t)c->Nx * c->op * c->op * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (c->has_lc && qvv)
+      TDA_HIP(hipMemcpyAsync(qvv, c->qvv.p, (size_t)c->Nx * c->vp * c->vp * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (de) TDA_HIP(hipMemcpyAsync(de, c->de.p, (size_t)c->ovp * 8, hipMemcpyDeviceToDevice, c->stream));
+    TDA_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+const char* ctx_error(tda_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+}  // namespace tda

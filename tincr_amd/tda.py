@@ -381,9 +381,16 @@ class TdaBatch:
     ``run`` returns one dict per item (``ci_eigenvalues``, ``ci_coefficients``,
     ``q_trans``, ``tr_dipoles``, ``oscillator_strengths``, ``stats``,
     ``status``) and raises ``DavidsonError`` if an item did not converge
-    (after every item has been solved)."""
+    (after every item has been solved).
 
-    def __init__(self, precision="fp64", n_workers=8, device=0, hbm_budget_bytes=0, operator="ao"):
+    ``mode="grouped"`` (default): one Davidson for all fragments, every step
+    one grouped launch per operation (charges-form operator; FP32 products
+    with an FP64 verification unless ``precision="fp64"``).  ``mode="pool"``:
+    worker threads with one context each solve whole fragments."""
+
+    MODES = {"grouped": 0, "pool": 1}
+
+    def __init__(self, precision="fp64", n_workers=8, device=0, hbm_budget_bytes=0, operator="ao", mode="grouped"):
         self.lib = nat.load()
         self.handle = nat.ctypes.c_void_p()
         rc = self.lib.tda_batch_create(device, n_workers, hbm_budget_bytes, nat.PREC[precision],
@@ -391,6 +398,7 @@ class TdaBatch:
         if rc != nat.TDA_OK:
             raise nat.TdaError(rc, "tda_batch_create failed")
         self._check(self.lib.tda_batch_set_operator(self.handle, nat.OPERATOR[operator]))
+        self._check(self.lib.tda_batch_set_mode(self.handle, self.MODES[mode]))
         self.items = []
 
     def _check(self, rc):
@@ -455,6 +463,9 @@ class TdaBatch:
         status = np.zeros(n, dtype=np.int32)
         stats = (nat.TdaStats * max(n, 1))()
         rc = self.lib.tda_batch_run(self.handle, nat.iptr(status), stats)
+        tot = nat.TdaStats()
+        self._check(self.lib.tda_batch_last_totals(self.handle, nat.ctypes.byref(tot)))
+        self.totals = tot.as_dict()
         results = []
         for i, (_, out) in enumerate(self.items):
             r = dict(out)
