@@ -78,9 +78,10 @@ __device__ inline double block_sum256(double v, double* red) {
 }
 
 template <int OP>
-__global__ void __launch_bounds__(256) vec_kernel(const VecUnit* __restrict__ units, const int* __restrict__ count) {
+__global__ void __launch_bounds__(256) vec_kernel(const GroupSlot* __restrict__ slot) {
   __shared__ double red[4];
-  const int nu = *count;
+  const int nu = slot->count;
+  const VecUnit* __restrict__ units = (const VecUnit*)slot->descs;
   for (int u = blockIdx.x; u < nu; u += gridDim.x) {
     const VecUnit e = units[u];
     if constexpr (OP == VO_RESIDUAL) {
@@ -124,10 +125,11 @@ struct GsDesc {
   int nc;
 };
 
-__global__ void __launch_bounds__(256) gs_inblock_kernel(const GsDesc* __restrict__ d, const int* __restrict__ count) {
+__global__ void __launch_bounds__(256) gs_inblock_kernel(const GroupSlot* __restrict__ slot) {
   __shared__ double red[4][8];
   __shared__ double coef[8];
-  const int nd = *count;
+  const int nd = slot->count;
+  const GsDesc* __restrict__ d = (const GsDesc*)slot->descs;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int b = blockIdx.x; b < nd; b += gridDim.x) {
     const GsDesc e = d[b];
@@ -191,9 +193,9 @@ struct DipDesc {
   double* f;          // [n]
   int N, n;
 };
-__global__ void __launch_bounds__(256) dipoles_grouped_kernel(const DipDesc* __restrict__ d,
-                                                              const int* __restrict__ count) {
-  const int nd = *count;
+__global__ void __launch_bounds__(256) dipoles_grouped_kernel(const GroupSlot* __restrict__ slot) {
+  const int nd = slot->count;
+  const DipDesc* __restrict__ d = (const DipDesc*)slot->descs;
   for (int b = blockIdx.x; b < nd; b += gridDim.x) {
     const DipDesc e = d[b];
     for (int c = threadIdx.x; c < e.n; c += 256) {
@@ -212,9 +214,9 @@ struct UnpackDesc {
   double* dst;
   int o, v, vp, n;
 };
-__global__ void __launch_bounds__(256) unpack_grouped_kernel(const UnpackDesc* __restrict__ d,
-                                                             const int* __restrict__ count) {
-  const int nd = *count;
+__global__ void __launch_bounds__(256) unpack_grouped_kernel(const GroupSlot* __restrict__ slot) {
+  const int nd = slot->count;
+  const UnpackDesc* __restrict__ d = (const UnpackDesc*)slot->descs;
   for (int b = blockIdx.x; b < nd; b += gridDim.x) {
     const UnpackDesc e = d[b];
     const int64_t total = (int64_t)e.o * e.v * e.n;
@@ -288,19 +290,24 @@ struct PinMem {
   }
 };
 
-// The descriptors, tile lists and counts of one segment of a step, packed in
-// one pinned buffer and uploaded with one copy.
+// The work lists of one segment of a step (descriptors, tile lists),
+// packed in one pinned buffer and uploaded with one copy.  The device buffer
+// starts with MAX_SLOTS slot headers (GroupSlot: absolute pointers + count),
+// which the grouped kernels read; the buffer is reallocated only when it must
+// grow (gen counts reallocations: captured graphs hold its address).
 struct Plan {
+  static constexpr int MAX_SLOTS = 32;
   std::vector<char> host;
   PinMem pin;
   DevMem dev;
+  uint64_t gen = 0;
   struct Op {
-    size_t desc_off, tile_off, count_off;
+    size_t desc_off, tile_off;
     int count;
   };
   std::vector<Op> ops;
   void reset() {
-    host.clear();
+    host.assign(MAX_SLOTS * sizeof(GroupSlot), 0);
     ops.clear();
   }
   size_t put(const void* p, size_t bytes) {
@@ -308,6 +315,11 @@ struct Plan {
     host.resize(off + bytes);
     if (bytes) std::memcpy(host.data() + off, p, bytes);
     return off;
+  }
+  int add(size_t desc_off, size_t tile_off, int count) {
+    TDA_CHECK((int)ops.size() < MAX_SLOTS, 2, "batch plan: too many operations in one segment");
+    ops.push_back(Op{desc_off, tile_off, count});
+    return (int)ops.size() - 1;
   }
   // a grouped GEMM: descriptors + tiles of configuration cfg
   int gemm(const std::vector<GemmArgs>& d, int cfg) {
@@ -319,39 +331,81 @@ struct Plan {
         for (int a = 0; a < tm; ++a)
           for (int c = 0; c < tn; ++c) tiles.push_back(int4{i, z, a, c});
     }
-    Op op;
-    op.desc_off = put(d.data(), d.size() * sizeof(GemmArgs));
-    op.tile_off = put(tiles.data(), tiles.size() * sizeof(int4));
-    op.count = (int)tiles.size();
-    op.count_off = put(&op.count, sizeof(int));
-    ops.push_back(op);
-    return (int)ops.size() - 1;
+    const size_t doff = put(d.data(), d.size() * sizeof(GemmArgs));
+    const size_t toff = put(tiles.data(), tiles.size() * sizeof(int4));
+    return add(doff, toff, (int)tiles.size());
   }
   template <typename T>
   int units(const std::vector<T>& u) {
-    Op op;
-    op.desc_off = put(u.data(), u.size() * sizeof(T));
-    op.tile_off = 0;
-    op.count = (int)u.size();
-    op.count_off = put(&op.count, sizeof(int));
-    ops.push_back(op);
-    return (int)ops.size() - 1;
+    return add(put(u.data(), u.size() * sizeof(T)), 0, (int)u.size());
   }
   void upload(hipStream_t s) {
-    if (host.empty()) return;
+    if (host.empty()) reset();
     // the previous upload's copy and the kernels reading its device copy
     // must be done before either buffer is reused
     TDA_HIP(hipStreamSynchronize(s));
+    if (dev.bytes < host.size()) {
+      dev.ensure(std::max(host.size(), (size_t)64 << 20));
+      ++gen;
+    }
+    GroupSlot* hs = (GroupSlot*)host.data();
+    for (size_t i = 0; i < ops.size(); ++i) {
+      hs[i].descs = (const char*)dev.p + ops[i].desc_off;
+      hs[i].tiles = (const int4*)((const char*)dev.p + ops[i].tile_off);
+      hs[i].count = ops[i].count;
+    }
     pin.ensure(host.size());
-    dev.ensure(host.size());
     std::memcpy(pin.p, host.data(), host.size());
     TDA_HIP(hipMemcpyAsync(dev.p, pin.p, host.size(), hipMemcpyHostToDevice, s));
   }
-  template <typename T>
-  const T* dptr(size_t off) const {
-    return (const T*)((const char*)dev.p + off);
-  }
+  const GroupSlot* slot(int op) const { return (const GroupSlot*)dev.p + op; }
+  int count(int op) const { return ops[op].count; }
 };
+
+// A segment of launches (between two host decisions) replayed from a
+// captured graph once its launch sequence (the key: tile configurations,
+// grids) and the plan buffer are unchanged; the work lists come from the
+// uploaded plan.  TINCR_TDA_BATCH_GRAPH=0: direct launches.
+struct SegGraph {
+  hipGraphExec_t exec = nullptr;
+  std::vector<int> key;
+  uint64_t gen = 0;
+  void drop() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+  }
+  ~SegGraph() { drop(); }
+};
+bool graphs_on() {
+  const char* e = getenv("TINCR_TDA_BATCH_GRAPH");
+  return !(e && e[0] == '0');
+}
+template <typename F>
+void run_segment(hipStream_t s, SegGraph& sg, const std::vector<int>& key, uint64_t gen, bool use_graph, F&& launches) {
+  if (!use_graph) {
+    launches();
+    return;
+  }
+  if (!(sg.exec && sg.key == key && sg.gen == gen)) {
+    sg.drop();
+    hipGraph_t graph = nullptr;
+    TDA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      launches();
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    TDA_HIP(hipStreamEndCapture(s, &graph));
+    const hipError_t e = hipGraphInstantiate(&sg.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    TDA_HIP(e);
+    sg.key = key;
+    sg.gen = gen;
+  }
+  TDA_HIP(hipGraphLaunch(sg.exec, s));
+}
 
 // operation classes of the grouped GEMMs (tile autotuner key)
 enum OpClass : int {
@@ -449,6 +503,8 @@ struct Group {
   };
   std::vector<Rec> recs;
   tda_stats* totals = nullptr;
+  bool graphs = false;
+  SegGraph g_prod[2], g_sub, g_exp;
   explicit Group(BatchMem& m)
       : arena(m.arena), scal(m.scal), outb(m.outb), scal_h(m.scal_h), out_h(m.out_h), plan(m.plan), t64(m.t64),
         t32(m.t32) {}
@@ -458,14 +514,17 @@ struct Group {
 size_t t_bytes(const GItem& it, int k, size_t esz) {
   return rup((int64_t)k * it.d.vp * it.d.NL * it.d.op * esz, 256);
 }
-// pool of T: the largest fragment's n vectors, at least min(64 GiB, all of them)
+// pool of T: the largest fragment's n vectors, at least min(budget, all of
+// them); budget TINCR_TDA_BATCH_TPOOL_GB (default 16)
 void ensure_t_pool(DevMem& pool, const std::vector<GItem*>& items, size_t esz, hipStream_t s) {
   size_t mx = 0, all = 0;
   for (GItem* it : items) {
     mx = std::max(mx, t_bytes(*it, it->n, esz));
     all += t_bytes(*it, it->n, esz);
   }
-  const size_t want = std::max(mx, std::min(all, (size_t)64 << 30));
+  const char* e = getenv("TINCR_TDA_BATCH_TPOOL_GB");
+  const double gb = e && atof(e) > 0 ? atof(e) : 16.0;
+  const size_t want = std::max(mx, std::min(all, (size_t)(gb * (1 << 30))));
   if (want <= pool.bytes) return;
   pool.ensure(want);
   TDA_HIP(hipMemsetAsync(pool.p, 0, pool.bytes, s));
@@ -608,19 +667,20 @@ void prepare_operators(Group& g, int n_workers, size_t budget) {
   if (!first_err.empty()) throw Error{TDA_ERR_HIP, first_err};
 }
 
-// run one grouped GEMM op of the plan
+// One grouped launch of a plan op.  With graphs every launch is made (empty
+// work lists included) on a fixed grid, so a captured segment fits any step;
+// otherwise empty ops are skipped and small ones get a smaller grid.
+int vec_grid(const Group& g, int op) {
+  return g.graphs ? GRID_VEC : std::max(1, std::min(GRID_VEC, g.plan.count(op)));
+}
 void run_gemm_op(Group& g, int op, DType in, DType out, Epi epi, bool a_kc, bool b_kc, int cfg) {
-  const Plan::Op& o = g.plan.ops[op];
-  if (o.count == 0) return;
-  grouped_gemm_launch(in, out, epi, a_kc, b_kc, cfg, g.plan.dptr<GemmArgs>(o.desc_off),
-                      g.plan.dptr<int4>(o.tile_off), g.plan.dptr<int>(o.count_off), GRID_GEMM, g.s);
+  if (!g.graphs && g.plan.count(op) == 0) return;
+  grouped_gemm_launch(in, out, epi, a_kc, b_kc, cfg, g.plan.slot(op), GRID_GEMM, g.s);
 }
 template <int OP>
 void run_vec_op(Group& g, int op) {
-  const Plan::Op& o = g.plan.ops[op];
-  if (o.count == 0) return;
-  vec_kernel<OP><<<std::min(GRID_VEC, o.count), 256, 0, g.s>>>(g.plan.dptr<VecUnit>(o.desc_off),
-                                                               g.plan.dptr<int>(o.count_off));
+  if (!g.graphs && g.plan.count(op) == 0) return;
+  vec_kernel<OP><<<vec_grid(g, op), 256, 0, g.s>>>(g.plan.slot(op));
   TDA_LAUNCH_CHECK();
 }
 
@@ -675,14 +735,13 @@ int pick_cfg(Group& g, const GemmSpec& sp, const std::vector<GemmArgs>& d, doubl
   for (auto& x : ev) TDA_HIP(hipEventCreate(&x));
   float ms[3];
   Plan tp;
+  tp.reset();
   std::vector<int> ops(3);
   for (int c = 0; c < 3; ++c) ops[c] = tp.gemm(t, cands[c]);
   tp.upload(g.s);
   for (int c = 0; c < 3; ++c) {
-    const Plan::Op& o = tp.ops[ops[c]];
     auto launch = [&]() {
-      grouped_gemm_launch(sp.in, sp.out, sp.epi, sp.a_kc, sp.b_kc, cands[c], tp.dptr<GemmArgs>(o.desc_off),
-                          tp.dptr<int4>(o.tile_off), tp.dptr<int>(o.count_off), GRID_GEMM, g.s);
+      grouped_gemm_launch(sp.in, sp.out, sp.epi, sp.a_kc, sp.b_kc, cands[c], tp.slot(ops[c]), GRID_GEMM, g.s);
     };
     launch();  // warm-up
     TDA_HIP(hipEventRecord(ev[2 * c], g.s));
@@ -818,32 +877,41 @@ void products_chunk(Group& g, const std::vector<GItem*>& its, bool fp32) {
   const int o_1 = g.plan.gemm(ds1, c1);
   const int o_3 = c3 >= 0 ? g.plan.gemm(ds3, c3) : -1;
   g.plan.upload(g.s);
-  run_vec_op<VO_CAST>(g, o_cx);
-  run_gemm_op(g, o_u, s_u.in, s_u.out, s_u.epi, s_u.a_kc, s_u.b_kc, default_cfg(OC_U));
-  run_gemm_op(g, o_uu, s_uu.in, s_uu.out, s_uu.epi, s_uu.a_kc, s_uu.b_kc, default_cfg(OC_UU));
-  run_vec_op<VO_CAST>(g, o_cu);
-  run_gemm_op(g, o_c, s_c.in, s_c.out, s_c.epi, s_c.a_kc, s_c.b_kc, default_cfg(OC_COUL));
-  auto timed = [&](int stage, double fl, auto&& launch) {
-    Group::Rec r{nullptr, nullptr, stage, pc, fl};
-    TDA_HIP(hipEventCreate(&r.a));
-    TDA_HIP(hipEventCreate(&r.b));
-    TDA_HIP(hipEventRecord(r.a, g.s));
-    launch();
-    TDA_HIP(hipEventRecord(r.b, g.s));
-    g.recs.push_back(r);
+  // the whole product step is timed (device events around it): its
+  // algorithmic flops are the Coulomb and exchange contractions
+  double fc = 0;
+  for (GItem* it : its) fc += 4.0 * it->k * (double)it->d.N * it->d.o * it->d.v + 2.0 * it->k * (double)it->d.N * it->d.N;
+  Group::Rec r{nullptr, nullptr, 0, pc, fc + f1 + f3};
+  TDA_HIP(hipEventCreate(&r.a));
+  TDA_HIP(hipEventCreate(&r.b));
+  TDA_HIP(hipEventRecord(r.a, g.s));
+  auto launches = [&](bool with3) {
+    run_vec_op<VO_CAST>(g, o_cx);
+    run_gemm_op(g, o_u, s_u.in, s_u.out, s_u.epi, s_u.a_kc, s_u.b_kc, default_cfg(OC_U));
+    run_gemm_op(g, o_uu, s_uu.in, s_uu.out, s_uu.epi, s_uu.a_kc, s_uu.b_kc, default_cfg(OC_UU));
+    run_vec_op<VO_CAST>(g, o_cu);
+    run_gemm_op(g, o_c, s_c.in, s_c.out, s_c.epi, s_c.a_kc, s_c.b_kc, default_cfg(OC_COUL));
+    run_gemm_op(g, o_1, s_1.in, s_1.out, s_1.epi, s_1.a_kc, s_1.b_kc, c1);
+    if (with3) run_gemm_op(g, o_3, s_3.in, s_3.out, s_3.epi, s_3.a_kc, s_3.b_kc, c3);
   };
-  if (!ds1.empty()) timed(0, f1, [&] { run_gemm_op(g, o_1, s_1.in, s_1.out, s_1.epi, s_1.a_kc, s_1.b_kc, c1); });
-  if (ds3.empty()) return;
-  if (o_3 >= 0) {
-    timed(1, f3, [&] { run_gemm_op(g, o_3, s_3.in, s_3.out, s_3.epi, s_3.a_kc, s_3.b_kc, c3); });
-    return;
+  if (o_3 >= 0 || ds3.empty()) {
+    // tuned (or no exchange): the segment is replayed from a captured graph
+    const bool with3 = o_3 >= 0;
+    run_segment(g.s, g.g_prod[pc], {c1, c3, with3 ? 1 : 0}, g.plan.gen, g.graphs, [&] { launches(with3); });
+  } else {
+    const bool gr = g.graphs;
+    g.graphs = false;
+    launches(false);
+    const int c3t = pick_cfg(g, s_3, ds3, f3);
+    Plan& p = g.plan;
+    p.reset();
+    const int o_3t = p.gemm(ds3, c3t);
+    p.upload(g.s);
+    run_gemm_op(g, o_3t, s_3.in, s_3.out, s_3.epi, s_3.a_kc, s_3.b_kc, c3t);
+    g.graphs = gr;
   }
-  const int c3t = pick_cfg(g, s_3, ds3, f3);
-  Plan& p = g.plan;
-  p.reset();
-  const int o_3t = p.gemm(ds3, c3t);
-  p.upload(g.s);
-  timed(1, f3, [&] { run_gemm_op(g, o_3t, s_3.in, s_3.out, s_3.epi, s_3.a_kc, s_3.b_kc, c3t); });
+  TDA_HIP(hipEventRecord(r.b, g.s));
+  g.recs.push_back(r);
 }
 
 void set_status(const GItem& it, int32_t* status, tda_stats* stats, bool mixed) {
@@ -1030,18 +1098,20 @@ void run_group(Group& g, int n_workers, size_t budget, int32_t* status, tda_stat
       Plan& p = g.plan;
       p.reset();
       const int o_p = p.gemm(dp, default_cfg(OC_PROJ));
-      const size_t eo = p.put(de.data(), de.size() * sizeof(EighDesc));
-      const int ne = (int)de.size();
-      const size_t eco = p.put(&ne, sizeof(int));
+      const int o_e = p.units(de);
       const int o_z = p.gemm(dz, default_cfg(OC_RITZ));
       const int o_r = p.gemm(dr, default_cfg(OC_RITZ));
       const int o_res = p.units(dres);
       p.upload(g.s);
-      run_gemm_op(g, o_p, DT_F64, DT_F64, EPI_STORE, true, true, default_cfg(OC_PROJ));
-      launch_eigh_batched(p.dptr<EighDesc>(eo), p.dptr<int>(eco), ne, g.s);
-      run_gemm_op(g, o_z, DT_F64, DT_F64, EPI_STORE, true, false, default_cfg(OC_RITZ));
-      run_gemm_op(g, o_r, DT_F64, DT_F64, EPI_STORE, true, false, default_cfg(OC_RITZ));
-      run_vec_op<VO_RESIDUAL>(g, o_res);
+      // the eigensolver's grid: every fragment of the group (graphs), or the active ones
+      const int egrid = g.graphs ? (int)g.items.size() : (int)de.size();
+      run_segment(g.s, g.g_sub, {egrid}, p.gen, g.graphs, [&] {
+        run_gemm_op(g, o_p, DT_F64, DT_F64, EPI_STORE, true, true, default_cfg(OC_PROJ));
+        launch_eigh_batched(p.slot(o_e), egrid, g.s);
+        run_gemm_op(g, o_z, DT_F64, DT_F64, EPI_STORE, true, false, default_cfg(OC_RITZ));
+        run_gemm_op(g, o_r, DT_F64, DT_F64, EPI_STORE, true, false, default_cfg(OC_RITZ));
+        run_vec_op<VO_RESIDUAL>(g, o_res);
+      });
       TDA_HIP(hipMemcpyAsync(g.scal_h.p, g.scal.p, g.items.size() * g.scal_stride * 8, hipMemcpyDeviceToHost, g.s));
       TDA_HIP(hipStreamSynchronize(g.s));
     }
@@ -1156,21 +1226,21 @@ void run_group(Group& g, int n_workers, size_t budget, int32_t* status, tda_stat
       const int o_s = p.gemm(dsub, default_cfg(OC_GS_SUB));
       const int o_c0b = p.gemm(dc0b, default_cfg(OC_GS_DOT));
       const int o_sb = p.gemm(dsubb, default_cfg(OC_GS_SUB));
-      const size_t go = p.put(dgs.data(), dgs.size() * sizeof(GsDesc));
-      const int ng = (int)dgs.size();
-      const size_t gco = p.put(&ng, sizeof(int));
+      const int o_gs = p.units(dgs);
       p.upload(g.s);
-      run_vec_op<VO_PRECOND>(g, o_pre);
-      run_vec_op<VO_COPY>(g, o_cp);
-      // two classical passes against the basis: W = P - V (V^T P), then again on W
-      run_gemm_op(g, o_c0, DT_F64, DT_F64, EPI_STORE, true, true, default_cfg(OC_GS_DOT));
-      run_gemm_op(g, o_s, DT_F64, DT_F64, EPI_STORE, false, false, default_cfg(OC_GS_SUB));
-      run_gemm_op(g, o_c0b, DT_F64, DT_F64, EPI_STORE, true, true, default_cfg(OC_GS_DOT));
-      run_gemm_op(g, o_sb, DT_F64, DT_F64, EPI_STORE, false, false, default_cfg(OC_GS_SUB));
-      if (ng > 0) {
-        gs_inblock_kernel<<<std::min(ng, GRID_VEC), 256, 0, g.s>>>(p.dptr<GsDesc>(go), p.dptr<int>(gco));
-        TDA_LAUNCH_CHECK();
-      }
+      run_segment(g.s, g.g_exp, {0}, p.gen, g.graphs, [&] {
+        run_vec_op<VO_PRECOND>(g, o_pre);
+        run_vec_op<VO_COPY>(g, o_cp);
+        // two classical passes against the basis: W = P - V (V^T P), then again on W
+        run_gemm_op(g, o_c0, DT_F64, DT_F64, EPI_STORE, true, true, default_cfg(OC_GS_DOT));
+        run_gemm_op(g, o_s, DT_F64, DT_F64, EPI_STORE, false, false, default_cfg(OC_GS_SUB));
+        run_gemm_op(g, o_c0b, DT_F64, DT_F64, EPI_STORE, true, true, default_cfg(OC_GS_DOT));
+        run_gemm_op(g, o_sb, DT_F64, DT_F64, EPI_STORE, false, false, default_cfg(OC_GS_SUB));
+        if (g.graphs || p.count(o_gs) > 0) {
+          gs_inblock_kernel<<<vec_grid(g, o_gs), 256, 0, g.s>>>(p.slot(o_gs));
+          TDA_LAUNCH_CHECK();
+        }
+      });
       if (!grown.empty()) {
         TDA_HIP(hipMemcpyAsync(g.scal_h.p, g.scal.p, g.items.size() * g.scal_stride * 8, hipMemcpyDeviceToHost,
                                g.s));
@@ -1233,17 +1303,18 @@ void run_group(Group& g, int n_workers, size_t budget, int32_t* status, tda_stat
     Plan& p = g.plan;
     p.reset();
     const int o_q = p.gemm(dq, default_cfg(OC_QT));
-    const size_t dof = p.put(dd.data(), dd.size() * sizeof(DipDesc));
+    const int o_d = p.units(dd);
+    const int o_u = p.units(du);
     const int nd = (int)dd.size();
-    const size_t dco = p.put(&nd, sizeof(int));
-    const size_t uof = p.put(du.data(), du.size() * sizeof(UnpackDesc));
-    const size_t uco = p.put(&nd, sizeof(int));
     p.upload(g.s);
+    const bool gr = g.graphs;
+    g.graphs = false;  // one-off launches
     run_gemm_op(g, o_q, DT_F64, DT_F64, EPI_STORE, true, true, default_cfg(OC_QT));
+    g.graphs = gr;
     if (nd > 0) {
-      dipoles_grouped_kernel<<<std::min(nd, GRID_VEC), 256, 0, g.s>>>(p.dptr<DipDesc>(dof), p.dptr<int>(dco));
+      dipoles_grouped_kernel<<<std::min(nd, GRID_VEC), 256, 0, g.s>>>(p.slot(o_d));
       TDA_LAUNCH_CHECK();
-      unpack_grouped_kernel<<<std::min(nd, GRID_VEC), 256, 0, g.s>>>(p.dptr<UnpackDesc>(uof), p.dptr<int>(uco));
+      unpack_grouped_kernel<<<std::min(nd, GRID_VEC), 256, 0, g.s>>>(p.slot(o_u));
       TDA_LAUNCH_CHECK();
     }
     TDA_HIP(hipMemcpyAsync(g.out_h.p, g.outb.p, out_doubles * 8, hipMemcpyDeviceToHost, g.s));
@@ -1352,6 +1423,7 @@ int batch_run_grouped(int device, int precision, size_t budget, std::vector<Batc
     while (k < order.size()) {
       Group g(mem);
       g.totals = totals;
+      g.graphs = graphs_on();
       g.device = device;
       g.mixed = mixed;
       g.s = s;

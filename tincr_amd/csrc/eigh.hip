@@ -567,16 +567,15 @@ __global__ void __launch_bounds__(EIGH_THREADS) eigh_kernel(const double* __rest
 // Many independent subspace problems (the fragments of a batch): workgroup b
 // solves descriptor b < *count; the grid is fixed (a captured launch replays
 // with new descriptors).
-__global__ void __launch_bounds__(EIGH_THREADS) eigh_batched_kernel(const EighDesc* __restrict__ d,
-                                                                    const int* __restrict__ count) {
-  if ((int)blockIdx.x >= *count) return;
-  const EighDesc e = d[blockIdx.x];
+__global__ void __launch_bounds__(EIGH_THREADS) eigh_batched_kernel(const GroupSlot* __restrict__ slot) {
+  if ((int)blockIdx.x >= slot->count) return;
+  const EighDesc e = ((const EighDesc*)slot->descs)[blockIdx.x];
   eigh_body<EIGH_MAXM_SMALL, EIGH_POOL_SMALL>(e.H, e.ldh, e.m, e.n, e.work, e.evals, e.Y, e.ldy, nullptr);
 }
 
-void launch_eigh_batched(const EighDesc* d, const int* count, int grid, hipStream_t s) {
+void launch_eigh_batched(const GroupSlot* slot, int grid, hipStream_t s) {
   if (grid <= 0) return;
-  eigh_batched_kernel<<<grid, EIGH_THREADS, 0, s>>>(d, count);
+  eigh_batched_kernel<<<grid, EIGH_THREADS, 0, s>>>(slot);
   TDA_LAUNCH_CHECK();
 }
 

@@ -77,6 +77,17 @@ __host__ __device__ inline int64_t hada_index(const GemmArgs& g, int64_t x) {
 // (other inputs fall back to CFG_SKINNY_N).
 enum CfgId : int { CFG_BIG = 0, CFG_SKINNY_N = 1, CFG_SKINNY_M = 2, CFG_SMALL = 3, CFG_BIG_K = 4, CFG_SKINNY_N16 = 5 };
 
+// Work list of one grouped launch (a batch's fragments): descriptors, tiles
+// (grouped GEMMs) and their count.  Grouped kernels read it through a pointer
+// to a slot at a fixed device address, so a captured launch replays with the
+// work list of the current step.
+struct GroupSlot {
+  const void* descs;
+  const int4* tiles;
+  int count;
+  int pad;
+};
+
 // Element type of an operand / result buffer.
 enum DType : int { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2, DT_F16 = 3 };
 

@@ -512,15 +512,16 @@ __global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_kernel(GemmArgs
 }
 
 // Grouped launch: many independent GEMMs (one GemmArgs each, e.g. one per
-// fragment of a batch) in one persistent launch.  tiles[t] = {descriptor,
-// batch index z, tile row, tile column}; the workgroups stride over the
-// *count tiles (no split-K).  The grid is fixed, so a captured launch
-// replays with new descriptors and tile lists.
+// fragment of a batch) in one persistent launch.  slot->tiles[t] =
+// {descriptor, batch index z, tile row, tile column}; the workgroups stride
+// over the slot->count tiles (no split-K).  The grid is fixed and the work
+// list is read through the slot, so a captured launch replays with new
+// descriptors and tile lists.
 template <typename TI, typename TO, typename Cfg, bool A_KC, bool B_KC, int EPI, typename LA = TI, typename LB = TI>
-__global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS)
-    gemm_grouped_kernel(const GemmArgs* __restrict__ descs, const int4* __restrict__ tiles,
-                        const int* __restrict__ count) {
-  const int nt = *count;
+__global__ void __launch_bounds__(256, TDA_GEMM_MIN_BLOCKS) gemm_grouped_kernel(const GroupSlot* __restrict__ slot) {
+  const int nt = slot->count;
+  const GemmArgs* __restrict__ descs = (const GemmArgs*)slot->descs;
+  const int4* __restrict__ tiles = slot->tiles;
   for (int t = blockIdx.x; t < nt; t += gridDim.x) {
     const int4 tl = tiles[t];
     gemm_tile<TI, TO, Cfg, A_KC, B_KC, EPI, LA, LB>(descs[tl.x], tl.y, 0, (int64_t)tl.z * Cfg::BM,

@@ -19,11 +19,12 @@ int ctx_charges_dims(tda_ctx* c, ChargesDims* d);
 int ctx_export_charges(tda_ctx* c, double* qov, double* gamma, double* G, double* qvv, double* de);
 const char* ctx_error(tda_ctx* c);
 
-// Grouped GEMM (gemm_grouped.hip): descs[] are independent problems, tiles[t]
-// = {descriptor, batch index, tile row, tile column} for the tile shape of
-// `cfg` (CfgId), *count tiles; `grid` persistent workgroups.  No split-K.
-void grouped_gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, int cfg, const GemmArgs* descs,
-                         const int4* tiles, const int* count, int grid, hipStream_t s);
+// Grouped GEMM (gemm_grouped.hip): slot->descs are independent problems
+// (GemmArgs), slot->tiles[t] = {descriptor, batch index, tile row, tile
+// column} for the tile shape of `cfg` (CfgId), slot->count tiles; `grid`
+// persistent workgroups.  No split-K.
+void grouped_gemm_launch(DType in, DType out, Epi epi, bool a_kc, bool b_kc, int cfg, const GroupSlot* slot, int grid,
+                         hipStream_t s);
 // tile shape of a CfgId (rows, columns)
 int grouped_cfg_bm(int cfg);
 int grouped_cfg_bn(int cfg);

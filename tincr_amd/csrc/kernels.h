@@ -1,6 +1,7 @@
 // Launchers of the non-GEMM kernels (stream.hip, eigh.hip, basis.hip).
 #pragma once
 #include "common.h"
+#include "gemm.h"
 
 namespace tda {
 
@@ -148,8 +149,9 @@ void launch_dipoles(const double* xyz, const double* qt, const double* w, int N,
 // (copy, tridiagonalisation, bisection, inverse iteration, back-transform).
 void launch_eigh(const double* H, int ldh, int m, int n, double* work, double* evals, double* Y, int ldy,
                  hipStream_t s, unsigned long long* phase_clk = nullptr);
-// Batched form for the fragments of a batch: descriptor b (m <= 256,
-// n <= min(m, 64), work m*m + m doubles) solved by workgroup b < *count.
+// Batched form for the fragments of a batch: EighDesc b of slot->descs (m <=
+// 256, n <= min(m, 64), work m*m + m doubles) solved by workgroup b <
+// slot->count.
 struct EighDesc {
   const double* H;
   double* work;
@@ -157,6 +159,6 @@ struct EighDesc {
   double* Y;
   int ldh, m, n, ldy;
 };
-void launch_eigh_batched(const EighDesc* d, const int* count, int grid, hipStream_t s);
+void launch_eigh_batched(const GroupSlot* slot, int grid, hipStream_t s);
 
 }  // namespace tda
