@@ -99,6 +99,7 @@ int tune16(DType in, Epi epi, GemmArgs& g, hipStream_t s) {
   // per call so that a process can switch
   const char* ft = getenv("TINCR_TDA_TILE16");
   if (ft && (ft[0] == '0' || ft[0] == '1') && ft[1] == 0) return ft[0] - '0';
+  // the diagonal epilogue runs on the 128x128 tile (the 256x256 one spills)
   if (epi == EPI_DIAG || !autotune16_on()) return -1;
   const double flops = 2.0 * (double)g.M * (double)g.N * (double)g.K * g.batch * (g.a_lo ? 3.0 : 1.0);
   if (flops < 1e10) return -1;

@@ -488,9 +488,10 @@ void launch16_cfg(GemmArgs& g, hipStream_t s) {
 }
 
 // tile: 1 = 256x256, 0 = 128x128, -1 = by tile count (gemm16_big).  The
-// FP64-output epilogue always takes the 128x128 tile (the 256x256 one spills;
-// a 256x128 tile measured 7 % faster on one 2048 x 1024 x 6144 x 8 product
-// but no faster over a solve's mix of batch sizes).
+// diagonal epilogue (FP64 or FP32 output) always takes the 128x128 tile (the
+// 256x256 one spills: with FP32 output its tuning trials wrote 12 GB of
+// scratch per launch; a 256x128 tile measured 7 % faster on one 2048 x 1024 x
+// 6144 x 8 product but no faster over a solve's mix of batch sizes).
 template <typename TI, typename TO, int EPI>
 void launch16_t(GemmArgs& g, hipStream_t s, int tile) {
   if constexpr (EPI != EPI_DIAG) {
