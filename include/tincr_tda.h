@@ -228,6 +228,41 @@ int tda_adjoint_nuclear(tda_ctx* ctx, int n_states, const double* X, const doubl
                         const double* dgamma_lr_dr, double* grad);
 
 /* ------------------------------------------------------------------------
+ * Excited-state nuclear gradient d omega / dR of one state, replacing
+ * System::tda_gradient_lc (src/gradients/excited_state.rs:259-487) and
+ * tda_gradient_nolc (:56-257, the context has no gamma_lr), with the inputs
+ * that ground_state_gradient(true) + prepare_excited_grad store in Properties
+ * (src/gradients/ground_state.rs:12-131, excited_state.rs:11-54).  Needs a
+ * System/Monomer context whose occupied MOs are 0..n_occ-1 and virtual MOs
+ * n_occ..n_ao-1 (as System::occ_indices / virt_indices).  FP64 throughout.
+ *   form: TDA_GRAD_REFERENCE follows the reference; TDA_GRAD_EXACT replaces
+ *     the two LC response terms that finite differences reject (the Z-vector
+ *     operator becomes the ground-state orbital Hessian A + B = de + 4 J - K
+ *     - K', K'_ia,jb = (ib|ja)_lr, instead of mult_av_lc's TDA A; H+[z] in
+ *     W_ij carries (kj|ib) + (kb|ij) instead of twice (kj|ib)).  Without
+ *     gamma_lr both forms are the reference's.
+ *   omega, x [n_occ * n_virt]: the state's excitation energy and CI vector
+ *     (ci_eigenvalues[state], the state's column of ci_coefficients).
+ *   dp [n_ao^2]: P - P_ref of the converged SCC (properties p, p_ref).
+ *   ds_rows, dh_rows [3][n_ao][n_ao]: dS / dH0 in row form, [k][mu][nu] =
+ *     d S[mu][nu] / d R[atom(mu)][k] (zero when mu, nu share an atom); the
+ *     reference's grad_s[3A + k][mu][nu] (h0_and_s.rs:196-300) is this entry
+ *     for atom(mu) == A and [k][nu][mu] for atom(nu) == A.
+ *   dgamma, dgamma_lr [3][n_atoms][n_atoms]: [k][A][B] = d gamma_AB / d R[A][k],
+ *     i.e. gamma_gradients_atomwise's entry [3A + k][A][B]
+ *     (gamma_approximation.rs:256-282); dgamma_lr NULL without gamma_lr.
+ *   max_iter, conv: Z-vector conjugate-gradient limits (1e4, 1e-19 on r.r in
+ *     tda_zvector_lc, helpers.rs:1146-1147).
+ * Outputs: grad [3 n_atoms] (gradExc); z [n_occ * n_virt] (optional): the
+ * Z-vector; z_iter (optional): its CG iterations.
+ * ------------------------------------------------------------------------ */
+#define TDA_GRAD_REFERENCE 0
+#define TDA_GRAD_EXACT 1
+int tda_gradient(tda_ctx* ctx, int form, double omega, const double* x, const double* dp, const double* ds_rows,
+                 const double* dh_rows, const double* dgamma, const double* dgamma_lr, int max_iter, double conv,
+                 double* grad, double* z, int32_t* z_iter);
+
+/* ------------------------------------------------------------------------
  * Batched fragments: the FMO monomer loop (src/fmo/lcmo/basis.rs:462-465,
  * prepare_tda + run_tda per monomer) and the CT-pair loop (basis.rs:172-238,
  * run_ct_tda per pair) in one call; each fragment converges on its own.

@@ -26,7 +26,7 @@ EXPORTS = [
     "tda_ctx_create", "tda_ctx_destroy", "tda_last_error", "tda_set_precision", "tda_set_operator",
     "tda_trans_charges", "tda_trans_charges_noalloc", "tda_orbe_differences", "tda_set_system", "tda_set_ct_pair", "tda_set_dense", "tda_prepare",
     "tda_clear_charges", "tda_get_charges", "tda_compute_products", "tda_precondition", "tda_run",
-    "tda_adjoint", "tda_adjoint_nuclear", "tda_zvector",
+    "tda_adjoint", "tda_adjoint_nuclear", "tda_gradient", "tda_zvector",
     "tda_batch_create", "tda_batch_destroy", "tda_batch_last_error", "tda_batch_set_operator", "tda_batch_set_mode", "tda_batch_last_totals",
     "tda_batch_add_system", "tda_batch_add_ct_pair", "tda_batch_clear", "tda_batch_run",
 ]
@@ -106,6 +106,8 @@ def load():
     lib.tda_run.argtypes = [P, i, i, ctypes.c_double, D, D, D, D, D, ctypes.POINTER(TdaStats)]
     lib.tda_adjoint.argtypes = [P, i, D, D, D, D, D, D]
     lib.tda_adjoint_nuclear.argtypes = [P, i, D, D, D, D]
+    lib.tda_gradient.argtypes = [P, i, ctypes.c_double, D, D, D, D, D, D, i, ctypes.c_double, D, D,
+                                 ctypes.POINTER(ctypes.c_int32)]
     lib.tda_batch_create.argtypes = [i, i, ctypes.c_size_t, i, ctypes.POINTER(P)]
     lib.tda_batch_destroy.argtypes = [P]
     lib.tda_batch_destroy.restype = None

@@ -14,7 +14,7 @@ OBJDIR = os.path.join(HERE, "lib", "obj")
 LIB = os.path.join(LIBDIR, "libtincr_tda.so")
 SOURCES = ["gemm_f64.hip", "gemm_f64mix.hip", "gemm_f32.hip", "gemm_f16.hip", "gemm_bf16.hip", "gemm.hip", "stream.hip",
            "eigh.hip", "basis.hip", "tda_api.hip", "batch.hip", "batch_grouped.hip", "gemm_grouped.hip",
-           "debug_api.hip"]
+           "debug_api.hip", "gradient.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [

@@ -2222,6 +2222,62 @@ int tda_adjoint_nuclear(tda_ctx* c, int n_states, const double* X, const double*
   });
 }
 
+// Excited-state nuclear gradient of one state (tda_gradient_lc,
+// src/gradients/excited_state.rs:259-487; tda_gradient_nolc :56-257); the
+// work is in gradient.hip.
+int tda_gradient(tda_ctx* c, int form, double omega, const double* x, const double* dp, const double* ds_rows,
+                 const double* dh_rows, const double* dgamma, const double* dgamma_lr, int max_iter, double conv,
+                 double* grad, double* z, int32_t* z_iter) {
+  if (!c) return TDA_ERR_BAD_ARG;
+  return guarded(c, [&] {
+    TDA_CHECK(c->has_system && !c->dense && !c->ct, TDA_ERR_STATE, "tda_gradient: no System/Monomer loaded");
+    TDA_CHECK(form == TDA_GRAD_REFERENCE || form == TDA_GRAD_EXACT, TDA_ERR_BAD_ARG, "tda_gradient: unknown form");
+    TDA_CHECK(x && dp && ds_rows && dh_rows && dgamma && grad, TDA_ERR_BAD_ARG, "tda_gradient: NULL input / output");
+    TDA_CHECK(!c->has_lc || dgamma_lr, TDA_ERR_BAD_ARG, "tda_gradient: dgamma_lr needed with gamma_lr");
+    TDA_CHECK(max_iter >= 0 && conv >= 0.0 && std::isfinite(omega), TDA_ERR_BAD_ARG, "tda_gradient: bad max_iter / conv / omega");
+    TDA_CHECK(c->ob == 0 && c->vb == c->o && c->o + c->v == c->nao, TDA_ERR_BAD_ARG,
+              "tda_gradient: the occupied and virtual ranges must cover all MOs (occupied first)");
+    prepare(c);
+    const int N = c->N, o = c->o, op = c->op, vp = c->vp, n = c->nao;
+    // device memory of the call (gradient.hip): T / U exchange intermediates,
+    // the padded q_ov, about 30 n x n matrices, the derivative inputs
+    const int64_t tmax = std::max({(int64_t)o * vp, (int64_t)op * op, (int64_t)vp * vp, (int64_t)vp * op});
+    const double need = 8.0 * (2.0 * N * tmax * (c->has_lc ? 1 : 0) + (double)N * op * vp + 40.0 * n * (double)n +
+                               6.0 * (double)N * N);
+    TDA_CHECK(need <= (double)c->budget, TDA_ERR_OOM,
+              "tda_gradient: " + std::to_string((int64_t)(need / 1e6)) + " MB of work buffers exceed the HBM budget");
+    if (!c->qov_built) {
+      c->qov.ensure((size_t)N * c->ovp * 8);
+      build_charges(c, c->C.as<double>(), c->SC.as<double>(), c->aooff.as<int>(), N, c->nao, c->ob, c->o, c->vb,
+                    c->v, c->op, c->vp, c->qov.as<double>(), nullptr, nullptr);
+      c->qov_built = true;
+    }
+    build_mo_charges(c);
+    GradView gv;
+    gv.N = N; gv.n = n; gv.o = o; gv.v = c->v; gv.op = op; gv.vp = vp; gv.ob = c->ob; gv.vb = c->vb;
+    gv.qov = c->qov.as<double>();
+    gv.qoo = c->qoo.as<double>();
+    gv.qvv = c->qvv.as<double>();
+    gv.gamma = c->gamma.as<double>();
+    gv.gamma_lr = c->has_lc ? c->gamma_lr.as<double>() : nullptr;
+    gv.C = c->C.as<double>();
+    gv.S = c->S.as<double>();
+    gv.orbe = c->orbe.as<double>();
+    gv.de = c->de.as<double>();
+    gv.off = c->aooff.as<int>();
+    gv.off_h.resize(N + 1);
+    TDA_HIP(hipMemcpyAsync(gv.off_h.data(), gv.off, (size_t)(N + 1) * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    TDA_HIP(hipStreamSynchronize(c->stream));
+    gv.stream = c->stream;
+    GradIn in{form == TDA_GRAD_EXACT ? 1 : 0, omega, x, dp, ds_rows, dh_rows, dgamma, c->has_lc ? dgamma_lr : nullptr,
+              max_iter, conv};
+    GradOut out{grad, z, 0};
+    gradient_run(gv, [&](const double* X, double* AX) { products(c, X, 1, AX, PREC_FP64); }, in, out);
+    c->harvest_events();
+    if (z_iter) *z_iter = out.z_iter;
+  });
+}
+
 int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, double* X, double* q_trans,
             double* tr_dipoles, double* f, tda_stats* stats) {
   if (!c) return TDA_ERR_BAD_ARG;

@@ -191,6 +191,88 @@ def h0_and_s(z, xyz):
     return s, h0
 
 
+def _sk_rules_grad(tab, kind, r, x, y, z, l1, m1, l2, m2):
+    """Derivative of _sk_rules w.r.t. the pair vector d = R_2 - R_1, [len(r), 3]:
+    dV/dd_k = V_r c_k + sum_j dV/dc_j (delta_jk - c_j c_k) / r, c = d / r
+    (slako_transformation_gradients, slako_transformations.rs:682, for
+    l1, l2 <= 1).  V_r uses the spline's first derivative; the angular part
+    is the tangential derivative on the unit sphere, so it does not depend on
+    how the rules are written off it."""
+    cvec = np.stack([x, y, z], axis=1)
+    kcol = {-1: 1, 0: 2, 1: 0}           # m -> Cartesian component (y, z, x)
+
+    def t(idx, der):
+        v = splev(r, tab.splines[(kind, idx)], der=der)
+        return np.where(r <= tab.dmax, v, 0.0)
+
+    n = len(r)
+    dvdc = np.zeros((n, 3))
+    if l1 == 0 and l2 == 0:
+        v_r = t(0, 1)
+    elif l1 == 0 or l2 == 0:
+        idx, m = (2, m2) if l1 == 0 else (4, m1)
+        c_m = cvec[:, kcol[m]]
+        v_r = c_m * t(idx, 1)
+        dvdc[:, kcol[m]] = t(idx, 0)
+    else:
+        d0 = t(6, 0) - t(5, 0)
+        d1 = t(6, 1) - t(5, 1)
+        c1, c2 = cvec[:, kcol[m1]], cvec[:, kcol[m2]]
+        if m1 == m2:
+            v_r = t(5, 1) + c1 ** 2 * d1
+            dvdc[:, kcol[m1]] = 2.0 * c1 * d0
+        else:
+            v_r = c1 * c2 * d1
+            dvdc[:, kcol[m1]] += c2 * d0
+            dvdc[:, kcol[m2]] += c1 * d0
+    radial = v_r[:, None] * cvec
+    tang = (dvdc - np.sum(dvdc * cvec, axis=1)[:, None] * cvec) / r[:, None]
+    return radial + tang
+
+
+def h0_and_s_gradients(z, xyz):
+    """Nuclear derivatives of S and H0 (h0_and_s_gradients,
+    src/scc/h0_and_s.rs:196-300) in row form: ds[k, mu, nu] =
+    dS[mu, nu] / dR[A(mu), k] (zero when A(mu) == A(nu)); the same for H0.
+    The reference's [3N, n, n] array is grad_s[3A + k, mu, nu] = ds[k, mu, nu]
+    for A(mu) == A, ds[k, nu, mu] for A(nu) == A, zero elsewhere."""
+    z = np.asarray(z, dtype=int)
+    xyz = np.asarray(xyz, dtype=np.float64)
+    atom, _, _, _ = ao_layout(z)
+    n_ao = len(atom)
+    first = np.searchsorted(atom, np.arange(len(z)))
+    ds = np.zeros((3, n_ao, n_ao))
+    dh = np.zeros((3, n_ao, n_ao))
+    elems = sorted(set(int(v) for v in z))
+    for ia, za in enumerate(elems):
+        for zb in elems[ia:]:
+            idx_a = np.nonzero(z == za)[0]
+            idx_b = np.nonzero(z == zb)[0]
+            A, B = np.meshgrid(idx_a, idx_b, indexing="ij")
+            A, B = A.ravel(), B.ravel()
+            if za == zb:
+                keep = A < B
+                A, B = A[keep], B[keep]
+            diff = xyz[B] - xyz[A]
+            r = np.sqrt(diff[:, 0] ** 2 + diff[:, 1] ** 2 + diff[:, 2] ** 2)
+            keep = (r < PROXIMITY_CUTOFF) & (r > 0.0)
+            A, B, diff, r = A[keep], B[keep], diff[keep], r[keep]
+            if len(r) == 0:
+                continue
+            x, y, zz = (diff[:, k] / r for k in range(3))
+            tab = slako_table(za, zb)
+            for oa, (l1, m1, _, _) in enumerate(valence_orbitals(za)):
+                for ob, (l2, m2, _, _) in enumerate(valence_orbitals(zb)):
+                    mu = first[A] + oa
+                    nu = first[B] + ob
+                    for kind, mat in (("s", ds), ("h", dh)):
+                        g = _sk_rules_grad(tab, kind, r, x, y, zz, l1, m1, l2, m2)
+                        for k in range(3):
+                            mat[k, mu, nu] = -g[:, k]       # d / dR_A, d = R_B - R_A
+                            mat[k, nu, mu] = g[:, k]        # d / dR_B
+    return ds, dh
+
+
 def gamma_ao_wise(gamma_atomwise, atom):
     """g0_ao[mu, nu] = gamma[A(mu), B(nu)] (gamma_approximation.rs:313-336)."""
     return gamma_atomwise[np.ix_(atom, atom)]
@@ -464,3 +546,40 @@ def tda_inputs_scc(mol, lc=True, verbose=False, blas_threads=8):
     return dict(n_atoms=mol.n_atoms, orbs_per_atom=mol.orbs_per_atom, xyz=mol.xyz, occ_indices=occ,
                 virt_indices=virt, orbs=r["orbs"], s=r["s"], orbe=r["orbe"], gamma=r["gamma"],
                 gamma_lr=r["gamma_lr"], scc_iter=r["n_iter"], scc_energy=r["energy"])
+
+
+# ---------------------------------------------------------------------------
+# inputs of the excited-state gradient (ground_state.rs:12-131,
+# excited_state.rs:11-54)
+# ---------------------------------------------------------------------------
+def gamma_gradients_rows(z, xyz, r_lr=0.0, hubbard=None):
+    """[3, N, N]: entry [k, A, B] = d gamma_AB / d R_A,k, the nonzero rows of
+    gamma_gradients_atomwise (gamma_approximation.rs:256-282: [3A + k, A, B]),
+    with gamma_AB = erf(c_AB r) / r."""
+    z = np.asarray(z)
+    xyz = np.asarray(xyz, dtype=np.float64)
+    sig = sy._sigmas(z, hubbard)
+    c = 1.0 / np.sqrt(2.0 * (sig[:, None] ** 2 + sig[None, :] ** 2 + 0.5 * r_lr ** 2))
+    diff = xyz[:, None, :] - xyz[None, :, :]
+    r = np.linalg.norm(diff, axis=-1)
+    np.fill_diagonal(r, 1.0)
+    from scipy.special import erf
+    dv = 2.0 * c / math.sqrt(math.pi) * np.exp(-(c * r) ** 2) / r - erf(c * r) / r ** 2
+    np.fill_diagonal(dv, 0.0)
+    return np.ascontiguousarray(np.moveaxis(diff / r[:, :, None] * dv[:, :, None], 2, 0))
+
+
+def excited_gradient_inputs(z, xyz, scc_result):
+    """What ground_state_gradient(true) and prepare_excited_grad leave in
+    Properties for tda_gradient_lc / _nolc, in the forms tda_gradient takes:
+    dp = P - P_ref [n, n]; ds_rows, dh_rows [3, n, n] (h0_and_s_gradients);
+    dgamma, dgamma_lr [3, N, N] (gamma_gradients_rows; dgamma_lr None without
+    long-range correction)."""
+    z = np.asarray(z, dtype=int)
+    _, _, _, occ_ao = ao_layout(z)
+    hub = hubbard_of(z)
+    ds, dh = h0_and_s_gradients(z, xyz)
+    lc = scc_result.get("gamma_lr") is not None
+    return dict(dp=np.ascontiguousarray(scc_result["p"] - np.diag(occ_ao)), ds_rows=ds, dh_rows=dh,
+                dgamma=gamma_gradients_rows(z, xyz, 0.0, hub),
+                dgamma_lr=gamma_gradients_rows(z, xyz, sy.LONG_RANGE_RADIUS, hub) if lc else None)

@@ -264,6 +264,54 @@ class TdaSystem(_TdaEngine):
         return g
 
 
+    # -- excited-state gradient -----------------------------------------------
+    def tda_gradient(self, state, inputs, form="reference", omega=None, x=None, max_iter=10000, conv=1.0e-19,
+                     return_z=False):
+        """d omega / dR [3N] of state `state` (System::tda_gradient_lc,
+        src/gradients/excited_state.rs:259-487, or tda_gradient_nolc :56-257
+        without gamma_lr).  inputs: scc.excited_gradient_inputs() (P - P_ref,
+        dS / dH0 rows, gamma gradients rows).  omega / x: the state's
+        excitation energy and CI vector [n_occ * n_virt] (default: from the last
+        run_tda).  form: "reference" or "exact" (the exact LC response terms,
+        see tda_gradient in include/tincr_tda.h).  Returns grad, or (grad, z,
+        Z-vector iterations) with return_z."""
+        if omega is None:
+            omega = float(self.properties["ci_eigenvalues"][state])
+        if x is None:
+            x = self.properties["ci_coefficients"][:, state]
+        x = nat.f64(np.ravel(x))
+        N, ov = self.n_atoms, self.n_occ * self.n_virt
+        assert x.size == ov
+        form_id = {"reference": 0, "exact": 1}[form]
+        dp = nat.f64(inputs["dp"])
+        ds = nat.f64(inputs["ds_rows"])
+        dh = nat.f64(inputs["dh_rows"])
+        dg = nat.f64(inputs["dgamma"])
+        dglr = nat.f64(inputs["dgamma_lr"]) if self.gammafunction_lc else None
+        grad = np.empty(3 * N)
+        z = np.empty(ov) if return_z else None
+        its = ctypes.c_int32(0)
+        self.ctx.check(self.ctx.lib.tda_gradient(
+            self.ctx.handle, form_id, float(omega), nat.dptr(x), nat.dptr(dp), nat.dptr(ds), nat.dptr(dh),
+            nat.dptr(dg), nat.dptr(dglr), int(max_iter), float(conv), nat.dptr(grad), nat.dptr(z),
+            ctypes.byref(its)))
+        if return_z:
+            return grad, z, its.value
+        return grad
+
+    def tda_gradient_lc(self, state, inputs, **kw):
+        """tda_gradient of an LC system (excited_state.rs:259-487)."""
+        if not self.gammafunction_lc:
+            raise ValueError("tda_gradient_lc needs gamma_lr")
+        return self.tda_gradient(state, inputs, **kw)
+
+    def tda_gradient_nolc(self, state, inputs, **kw):
+        """tda_gradient without long-range correction (excited_state.rs:56-257)."""
+        if self.gammafunction_lc:
+            raise ValueError("tda_gradient_nolc needs a system without gamma_lr")
+        return self.tda_gradient(state, inputs, **kw)
+
+
 class Fragment:
     """The monomer data a CT pair reads (fmo Monomer properties): atoms'
     AO counts, coordinates [n_atoms, 3] (bohr), occupied / virtual MO indices,
