@@ -15,6 +15,11 @@ warm-up solve and then --steps timed solves; states/s = n / median.
                     [--system water512|polyene200|...] [--states n]
                     [--operator ao|charges] [--workload fmo]
 
+--workload gradient: the excited-state nuclear gradient (tda_gradient) of the
+lowest state of C60H62 and (H2O)64 (converged LC-SCC, FP64 TDA solve
+untimed), --steps timed gradients per system and form; the CPU baseline is
+the numpy restatement of tda_gradient_lc (reference [3N, n, n] arrays).
+
 --workload fmo: the batched FMO case instead -- 1024 seeded fragments (H2O,
 (H2O)2, C4..C40 polyenes) placed >= 3 A apart plus their 2048 nearest pairs
 as charge-transfer pairs, lowest 8 states each (clamped to ov), tol 1e-5, one
@@ -362,6 +367,68 @@ def main_fmo(args):
     print(json.dumps(out), flush=True)
 
 
+def main_gradient(args):
+    """Gradients/s of tda_gradient (one state, FP64) on the differentiation
+    systems, both forms, beside the numpy oracle of tda_gradient_lc."""
+    from tincr_amd import scc
+    from tincr_amd import systems as sy
+    from tincr_amd import tda
+    systems = {"C60H62": sy.polyene(60, seed=0), "(H2O)64": sy.water_cluster(64, seed=0)}
+    res, cpu = {}, {}
+    for name, mol in systems.items():
+        t0 = time.perf_counter()
+        r = scc.run_scc(mol.z, mol.xyz, lc=True, charge_conv=1e-10, energy_conv=1e-10, max_iter=500)
+        occ = [i for i, f in enumerate(r["occupation"]) if f > 1.0]
+        virt = [i for i, f in enumerate(r["occupation"]) if f <= 1.0]
+        inputs = scc.excited_gradient_inputs(mol.z, mol.xyz, r)
+        t_in = time.perf_counter() - t0
+        s = tda.TdaSystem(mol.n_atoms, mol.orbs_per_atom, mol.xyz, occ, virt, r["orbs"], r["s"], r["orbe"],
+                          r["gamma"], r["gamma_lr"], operator=args.operator)
+        w, x = s.run_tda(4, 300, 1e-6)
+        entry = {"n_atoms": mol.n_atoms, "n_ao": len(occ) + len(virt), "n_occ": len(occ), "n_virt": len(virt),
+                 "omega_eV": round(float(w[0]) * HARTREE_TO_EV, 6), "input_generation_s": round(t_in, 2)}
+        for form in ("exact", "reference"):
+            for _ in range(max(args.warmup, 0)):
+                s.tda_gradient(0, inputs, form=form)
+            sync()
+            ts = []
+            for _ in range(args.steps):
+                t = time.perf_counter()
+                g, _, its = s.tda_gradient(0, inputs, form=form, return_z=True)
+                ts.append(time.perf_counter() - t)
+            entry[form] = {"ms_per_gradient_median": round(1e3 * float(np.median(ts)), 3),
+                           "ms_per_gradient": [round(1e3 * t, 3) for t in ts], "zvector_iterations": its,
+                           "max_abs_grad": float(np.abs(g).max())}
+        s.close()
+        res[name] = entry
+        if not args.no_cpu:
+            from oracle import gradient_oracle as go
+            gi = go.gradient_inputs(mol.z, mol.xyz, r, occ, virt)
+            t = time.perf_counter()
+            go.tda_gradient(gi, x[:, 0].reshape(len(occ), len(virt)), w[0])
+            cpu[name] = time.perf_counter() - t
+    head = "C60H62"
+    value = 1e3 / res[head]["exact"]["ms_per_gradient_median"]
+    out = {
+        "metric": "LC-TDA excited-state nuclear gradients/s (one state, FP64)", "value": round(value, 3),
+        "unit": "gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res[head]["exact"]["ms_per_gradient_median"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp64",
+        "data": "seeded geometries, converged LC-SCC inputs (host), FP64 TDA CI vector of the lowest state",
+        "config": {"workload": "excited-state gradient of the lowest LC-TDA state, C60H62 (value) and (H2O)64",
+                   "operator": args.operator, "form": "exact (value); reference beside it"},
+        "timing": "one tda_gradient call: host inputs in, Z-vector CG, gradient [3N] back on the host",
+        "systems": res,
+    }
+    if cpu:
+        n_thr = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {"value": round(1.0 / cpu[head], 4), "unit": "gradients/s", "cores": n_thr,
+                               "kind": "port",
+                               "sample": "one numpy tda_gradient_lc (reference [3N, n, n] arrays) per system",
+                               "seconds": {k: round(v, 2) for k, v in cpu.items()}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -375,7 +442,7 @@ def main():
     ap.add_argument("--max-iter", type=int, default=450)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="single", choices=["single", "fmo"])
+    ap.add_argument("--workload", default="single", choices=["single", "fmo", "gradient"])
     ap.add_argument("--inputs", default="scc", choices=["scc", "synthetic"],
                     help="scc: converged LC-SCC orbitals of the geometry (default); synthetic: seeded random "
                          "orthonormal orbitals with S = I (throughput only)")
@@ -395,6 +462,8 @@ def main():
         nat.debug_copy(CALIB_COPY_BYTES, 3)
     if args.workload == "fmo":
         return main_fmo(args)
+    if args.workload == "gradient":
+        return main_gradient(args)
 
     mol, inp = build_inputs(args.system, args.seed, args.inputs)
     n = args.states

@@ -20,7 +20,12 @@ Follows the reference:
   Mulliken charges and level shifting (src/scc/scc_routine.rs:73-290,
   src/scc/helpers.rs:63-256, src/scc/mixer/broyden.rs, src/scc/
   level_shifting.rs, src/scc/fermi_occupation.rs:66-86, src/scc/
-  mulliken.rs:9-24).
+  mulliken.rs:9-24);
+* the inputs of the excited-state gradient: dS/dR and dH0/dR
+  (h0_and_s_gradients, src/scc/h0_and_s.rs:196-300, with the derivative of
+  the s/p Slater-Koster rules), the nonzero rows of dgamma/dR and
+  dgamma_lr/dR (gamma_gradients_atomwise, gamma_approximation.rs:256-282)
+  and P - P_ref (ground_state.rs:96-131).
 
 Deviation: beyond the last tabulated distance (10 bohr for every table
 here) the Slater-Koster integrals are zero.  The reference evaluates its
