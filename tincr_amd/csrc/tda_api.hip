@@ -121,13 +121,18 @@ struct tda_ctx {
   // caps 6 / 7 / 8 / 10 / 12 / 15 / 20 n gave 720 / 652 / 608 / 663 / 811 /
   // 842 / 873 ms per solve: the subspace work of a large basis costs more
   // than the extra restarts)
-  int lp_cap_mult = 8, lp_keep_mult = 2;
+  // Small systems (product below 5e10 flops, e.g. C200H202: 1e10) use cap 6 n:
+  // there the subspace eigensolver at m = 8 n (1.4 ms at m = 160) costs as
+  // much as an iteration's products; measured on SCC C200H202, 20 states:
+  // caps 4 / 5 / 6 / 8 n gave 64.5 / 61.0 / 60.1 / 62.7 ms per solve.
+  // 0: by product cost (lp_cap_mult_for), else the TINCR_TDA_LP_CAP value.
+  int lp_cap_mult = 0, lp_keep_mult = 2;
   int fp32_cap_mult = 15;  // basis cap of the FP32-accuracy phase, multiples of n
   int lp_keep(int n) const { return std::max(n, std::min(64, lp_keep_mult * n)); }
   // rows of the basis / subspace buffers: the reference's 15 n (davidson.rs:74)
   // or a larger cap of the low-precision phases
   int space(int n) const { return std::max(15, std::max(lp_cap_mult, fp32_cap_mult)) * n; }
-  int lp_cap(int n) const { return std::min(space(n), std::max(lp_cap_mult * n, lp_keep(n) + n)); }
+  int lp_cap(int n, int mult) const { return std::min(space(n), std::max(mult * n, lp_keep(n) + n)); }
   int fp32_cap(int n) const { return std::min(space(n), std::max(fp32_cap_mult * n, lp_keep(n) + n)); }
   double pre_thr = 1e-4;
   // Coulomb factor of the charges-form product: 2 (solvers/mod.rs:311); the
@@ -220,7 +225,9 @@ struct tda_ctx {
   // stored as FP32 (the phase's product error, ~1e-4 per unit vector, lies
   // far above FP32 rounding); every product with them is formed in FP64.
   // Halves the bytes of the subspace GEMMs that stream V and AV.
-  bool basis32 = true;
+  // -1: by product cost (FP32 basis for products of 5e10 flops and more: at
+  // C200H202 the FP64 basis is 1.4 ms per solve faster), else 0 / 1
+  int basis32 = -1;
   DevBuf qstage;  // FP64 staging of the new basis vectors of an FP32 basis
   DevBuf ws_basis;  // partial sums of the basis dot products (launch_rowdot)
   // instrumentation
@@ -1050,6 +1057,16 @@ int lp_stall_window(const tda_ctx* c) {
   return product_flops(c) >= 5e10 ? 10 : 4;
 }
 
+// Basis cap multiplier of the FP16 / BF16 phase (see lp_cap_mult).
+int lp_cap_mult_for(const tda_ctx* c) {
+  if (c->lp_cap_mult > 0) return c->lp_cap_mult;
+  return product_flops(c) >= 5e10 ? 8 : 6;
+}
+bool basis32_for(const tda_ctx* c) {
+  if (c->basis32 >= 0) return c->basis32 != 0;
+  return product_flops(c) >= 5e10;
+}
+
 // In-block Gram-Schmidt of the nc new vectors (davidson.rs:162-168): W[j],
 // already projected against the basis, becomes u_j; the accepted q_j go to Q
 // in order.  Returns their number and, when wn is given, |u_j| per j.  P and
@@ -1154,7 +1171,13 @@ void subspace_eigh(tda_ctx* c, const double* H, int ldh, int m, int n, double* e
 // FP64): the new vectors are formed in FP64 staging and rounded to FP32 when
 // appended, so the basis is orthogonal to ~FP32 rounding (eps32 = 6e-8) and
 // eta is tracked from there.
-int append_block(tda_ctx* c, int m, int nc, bool reorth, double& eta, bool v32 = false) {
+// prec: the phase's product precision.  A basis needs orthogonality only to
+// about its products' error: eta_tol32 in the FP16 / BF16 phase (~1e-4 per
+// unit vector), as for an FP32 basis; 1e-7 in the FP32-accuracy phase (FP32
+// MFMA or FP16 hi / lo split products, ~6e-8); eta_tol for FP64 products.
+// (At C200H202 the FP64 threshold in the low-precision phases took second
+// passes worth about 2 ms per solve that changed nothing they hand over.)
+int append_block(tda_ctx* c, int m, int nc, bool reorth, double& eta, bool v32 = false, int prec = PREC_FP64) {
   const int64_t L = c->ovp;
   double* V = c->V.as<double>();
   float* V32 = (float*)c->V.p;
@@ -1185,7 +1208,10 @@ int append_block(tda_ctx* c, int m, int nc, bool reorth, double& eta, bool v32 =
     const double eta_new = (eta + eps) * grow;
     // an FP32 basis cannot be orthogonal beyond its rounding: the second
     // pass is for losses well above it
-    need = c->reorth_mode == 1 || eta_new > (v32 ? c->eta_tol32 : c->eta_tol);
+    const double tol_p = (v32 || prec == PREC_FP16 || prec == PREC_BF16) ? c->eta_tol32
+                         : prec == PREC_FP64                              ? c->eta_tol
+                                                                          : std::max(c->eta_tol, 1e-7);
+    need = c->reorth_mode == 1 || eta_new > tol_p;
     if (!need) eta = std::max(eta, eta_new);
   }
   if (need && h_count > 0) {
@@ -1456,7 +1482,7 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
           const int nb = std::min(64, nc - b0);
           launch_precondition_many(R, c->evals.as<double>(), c->idx.as<int>() + b0, nb, c->de.as<double>(), L,
                                    c->pre_thr, P, c->stream);
-          m += append_block(c, m, nb, reorth, eta, v32);
+          m += append_block(c, m, nb, reorth, eta, v32, prec);
         }
       }
     } else if (thick) {
@@ -1610,7 +1636,7 @@ int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx**
     const char* ee = getenv("TINCR_TDA_ETA_TOL");
     if (ee && atof(ee) > 0) c->eta_tol = atof(ee);
     const char* eb = getenv("TINCR_TDA_BASIS32");
-    if (eb) c->basis32 = atoi(eb) != 0;
+    if (eb) c->basis32 = atoi(eb) != 0 ? 1 : 0;
     const char* e32 = getenv("TINCR_TDA_ETA_TOL32");
     if (e32 && atof(e32) > 0) c->eta_tol32 = atof(e32);
     const char* enk = getenv("TINCR_TDA_NEAR_K");
@@ -2347,8 +2373,9 @@ int tda_run(tda_ctx* c, int n_roots, int max_iter, double tol, double* omega, do
       bool done = false, lp_overflow = false;
       int start = n, kk = n;
       if ((c->prec == PREC_FP16 || c->prec == PREC_BF16) && c->lp_phase) {
-        const bool v32 = c->basis32 && c->has_lc && c->op_kind == 0 && !c->ct;
-        res = davidson(c, n, n, max_iter, tol, c->prec, false, lp_stall_window(c), h_evals, h_err, c->lp_cap(n),
+        const bool v32 = basis32_for(c) && c->has_lc && c->op_kind == 0 && !c->ct;
+        res = davidson(c, n, n, max_iter, tol, c->prec, false, lp_stall_window(c), h_evals, h_err,
+                       c->lp_cap(n, lp_cap_mult_for(c)),
                        c->lp_keep(n), &kk, 0.8, 1e300, true, c->floor_k, true, false, v32);
         used += res.iters;
         done = res.converged;
