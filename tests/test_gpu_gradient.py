@@ -4,7 +4,8 @@ states: against the oracle's restatement of tda_gradient_lc / _nolc
 operators, and the exact form against central differences (step 1e-4 bohr,
 the reference's src/gradients/numerical.rs:67-73 check, tolerance 1e-6
 Ha/bohr) of SCC + GPU TDA excitation energies on seeded coordinates of the
-differentiation systems C60H62 and (H2O)64."""
+differentiation systems C60H62 and (H2O)64, with the state from an FP64 and
+from an FP32-ladder solve."""
 import numpy as np
 import pytest
 
@@ -23,10 +24,10 @@ def _scc(mol, lc, conv=1e-12):
     return r, occ, virt
 
 
-def _system(mol, r, occ, virt, operator="ao"):
+def _system(mol, r, occ, virt, operator="ao", precision="fp64"):
     from tincr_amd import tda
     return tda.TdaSystem(mol.n_atoms, mol.orbs_per_atom, mol.xyz, occ, virt, r["orbs"], r["s"], r["orbe"],
-                         r["gamma"], r["gamma_lr"], operator=operator)
+                         r["gamma"], r["gamma_lr"], operator=operator, precision=precision)
 
 
 CASES = {
@@ -103,6 +104,14 @@ def test_exact_gradient_matches_central_differences(name):
     exact = s.tda_gradient(state, inputs, form="exact")
     reference = s.tda_gradient(state, inputs, form="reference")
     s.close()
+    # FP32 mode: the state solved by the FP32 ladder (FP32 products, FP64
+    # verification); the gradient itself is FP64
+    s32 = _system(mol, r, occ, virt, precision="fp32")
+    w32, _ = s32.run_tda(6, 300, 1e-7)
+    assert abs(w32[state] - w[state]) < 1e-9
+    exact32 = s32.tda_gradient(state, inputs, form="exact")
+    s32.close()
+    assert np.abs(exact32 - exact).max() < 1e-7
     rng = np.random.default_rng(7)
     coords = rng.choice(3 * mol.n_atoms, 4, replace=False)
     h = 1e-4

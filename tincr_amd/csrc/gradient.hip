@@ -49,26 +49,33 @@ inline unsigned grid1(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, TPB), 16384));
 }
 
-// Zeroed device buffers of one call, released (after the stream drains) when
-// the call returns.
+// Zeroed device buffers of one call, taken in call order from a cache that
+// outlives the call (the context's): repeated gradients of one system
+// allocate nothing.  A slot too small for its request is reallocated.
 struct Pool {
   hipStream_t s;
-  std::vector<void*> bufs;
-  explicit Pool(hipStream_t st) : s(st) {}
-  ~Pool() {
-    if (!bufs.empty()) (void)hipStreamSynchronize(s);
-    for (void* p : bufs) (void)hipFree(p);
-  }
+  std::vector<std::pair<void*, size_t>>* cache;
+  size_t next = 0;
+  Pool(hipStream_t st, std::vector<std::pair<void*, size_t>>* c) : s(st), cache(c) {}
   double* get(size_t n) {
-    void* p = nullptr;
     const size_t bytes = std::max<size_t>(n, 1) * 8;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
-      (void)hipGetLastError();
-      throw Error{TDA_ERR_OOM, "gradient: hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+    if (next == cache->size()) cache->push_back({nullptr, 0});
+    std::pair<void*, size_t>& e = (*cache)[next++];
+    if (e.second < bytes) {
+      if (e.first) {
+        TDA_HIP(hipStreamSynchronize(s));
+        (void)hipFree(e.first);
+        e = {nullptr, 0};
+      }
+      void* p = nullptr;
+      if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error{TDA_ERR_OOM, "gradient: hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+      }
+      e = {p, bytes};
     }
-    bufs.push_back(p);
-    TDA_HIP(hipMemsetAsync(p, 0, bytes, s));
-    return (double*)p;
+    TDA_HIP(hipMemsetAsync(e.first, 0, bytes, s));
+    return (double*)e.first;
   }
   int* get_int(size_t n) { return (int*)get((n + 1) / 2); }
 };
@@ -114,6 +121,15 @@ struct Mm {
 // ---------------------------------------------------------------------------
 // element-wise kernels
 // ---------------------------------------------------------------------------
+// out = num / den; then (optional) dst = src
+__global__ void cg_ratio_kernel(const double* num, const double* den, double* out, double* dst, const double* src) {
+  if (threadIdx.x == 0) {
+    const double q = num[0] / den[0];
+    if (dst) dst[0] = src[0];
+    out[0] = q;
+  }
+}
+
 // ex[i][a] = e_o[i] x[i][a], xe[i][a] = x[i][a] e_v[a]   (a < v; pads stay 0)
 __global__ void scale_x_kernel(const double* __restrict__ x, const double* __restrict__ orbe, int ob, int vb, int o,
                                int v, int vp, double* __restrict__ ex, double* __restrict__ xe) {
@@ -312,7 +328,7 @@ void gradient_run(const GradView& V, const std::function<void(const double*, dou
   const bool exact = in.exact != 0;
   const int64_t ovp = (int64_t)o * vp, oo = (int64_t)op * op, vv = (int64_t)vp * vp, vo = (int64_t)vp * op;
   const int64_t nn = (int64_t)n * n;
-  Pool pool(s);
+  Pool pool(s, V.cache);
   Mm mm{s, pool};
   // ---- inputs ----
   double* stage = pool.get(std::max<int64_t>({(int64_t)o * v, 3 * nn, 3 * (int64_t)N * N, nn}));
@@ -484,39 +500,39 @@ void gradient_run(const GradView& V, const std::function<void(const double*, dou
   double* P = pool.get(ovp);
   double* AP = pool.get(ovp);
   double* part = pool.get(256);
-  double* dots = pool.get(4);
+  double* dots = pool.get(8);
   launch_cg_init(rhs, V.de, 1, ovp, Z, s);  // z0 = r / de (:1154-1155)
   zop(Z, AP);
   launch_copy(rhs, R, ovp, s);
   axpby_kernel<<<grid1(ovp), TPB, 0, s>>>(AP, R, ovp, -1.0, 1.0);
   TDA_LAUNCH_CHECK();
   launch_copy(R, P, ovp, s);
-  double h[2];
-  auto dot = [&](const double* a, const double* b) {
-    launch_dots(a, b, 1, ovp, part, dots, s);
-    TDA_HIP(hipMemcpyAsync(h, dots, 8, hipMemcpyDeviceToHost, s));
-    TDA_HIP(hipStreamSynchronize(s));
-    return h[0];
-  };
-  double rr = dot(R, R);
+  // CG scalars stay on the device (rr, p.Ap, alpha, beta); the host reads
+  // r.r once per iteration for the stopping test
+  double* rr_d = dots;
+  double* pap_d = dots + 1;
+  double* a_d = dots + 2;
+  double* b_d = dots + 3;
+  double* rrn_d = dots + 4;
+  double rr_new = 0.0;
+  launch_dots(R, R, 1, ovp, part, rr_d, s);
   int its = in.max_iter;
   for (int it = 0; it < in.max_iter; ++it) {
     zop(P, AP);
-    const double pap = dot(P, AP);
-    const double a = rr / pap;
-    h[1] = a;
-    TDA_HIP(hipMemcpyAsync(dots + 2, &h[1], 8, hipMemcpyHostToDevice, s));
-    launch_cg_step(Z, R, P, AP, dots + 2, 1, ovp, s);
-    const double rr_new = dot(R, R);
+    launch_dots(P, AP, 1, ovp, part, pap_d, s);
+    cg_ratio_kernel<<<1, 64, 0, s>>>(rr_d, pap_d, a_d, nullptr, nullptr);  // a = r.r / p.Ap
+    TDA_LAUNCH_CHECK();
+    launch_cg_step(Z, R, P, AP, a_d, 1, ovp, s);
+    launch_dots(R, R, 1, ovp, part, rrn_d, s);
+    TDA_HIP(hipMemcpyAsync(&rr_new, rrn_d, 8, hipMemcpyDeviceToHost, s));
+    TDA_HIP(hipStreamSynchronize(s));
     if (rr_new <= in.conv) {
       its = it + 1;
       break;
     }
-    h[1] = rr_new / rr;
-    TDA_HIP(hipMemcpyAsync(dots + 3, &h[1], 8, hipMemcpyHostToDevice, s));
-    TDA_HIP(hipStreamSynchronize(s));  // h[1] is reused by the next step's copy
-    launch_cg_dir(P, R, dots + 3, 1, ovp, s);
-    rr = rr_new;
+    cg_ratio_kernel<<<1, 64, 0, s>>>(rrn_d, rr_d, b_d, rr_d, rrn_d);  // b = r.r new / old; r.r = new
+    TDA_LAUNCH_CHECK();
+    launch_cg_dir(P, R, b_d, 1, ovp, s);
   }
   out.z_iter = its;
   if (out.z) {

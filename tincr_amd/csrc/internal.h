@@ -2,6 +2,7 @@
 #pragma once
 #include "../../include/tincr_tda.h"
 #include <functional>
+#include <utility>
 #include <vector>
 
 #include "gemm.h"
@@ -49,6 +50,8 @@ struct GradView {
   const int* off;
   std::vector<int> off_h;
   hipStream_t stream;
+  // device buffers kept between calls (owned by the context, freed with it)
+  std::vector<std::pair<void*, size_t>>* cache;
 };
 struct GradIn {
   int exact;  // 0: the reference's LC response terms, 1: the exact ones

@@ -113,6 +113,7 @@ struct tda_ctx {
   DevBuf adjP, adjQ, adjX, adjD;  // adjoint work buffers
   DevBuf one;                     // device 1.0 values
   DevBuf zvw;                     // Z-vector work (R, Z, P, AP)
+  std::vector<std::pair<void*, size_t>> grad_cache;  // tda_gradient's work buffers (gradient.hip)
   int last_roots = 0;             // CI vectors of the last tda_run held in RZ
   // thick restart of the FP16 / BF16 Davidson phase, in multiples of n_roots
   // (environment TINCR_TDA_LP_CAP / TINCR_TDA_LP_KEEP); the FP32 and FP64
@@ -254,6 +255,8 @@ struct tda_ctx {
   ~tda_ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (blas) (void)rocblas_destroy_handle(blas);
+    for (auto& b : grad_cache)
+      if (b.first) (void)hipFree(b.first);
     if (hpin) (void)hipHostFree(hpin);
     for (auto e : ev_pool) (void)hipEventDestroy(e);
     for (auto& ls : lps)
@@ -2295,6 +2298,7 @@ int tda_gradient(tda_ctx* c, int form, double omega, const double* x, const doub
     TDA_HIP(hipMemcpyAsync(gv.off_h.data(), gv.off, (size_t)(N + 1) * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     TDA_HIP(hipStreamSynchronize(c->stream));
     gv.stream = c->stream;
+    gv.cache = &c->grad_cache;
     GradIn in{form == TDA_GRAD_EXACT ? 1 : 0, omega, x, dp, ds_rows, dh_rows, dgamma, c->has_lc ? dgamma_lr : nullptr,
               max_iter, conv};
     GradOut out{grad, z, 0};
