@@ -1427,11 +1427,18 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       finish(true);
       return res;
     }
-    if (floor_k > 0 && prec != PREC_FP64 && !c->dense && it == 0) {
+    if (floor_k > 0 && prec != PREC_FP64 && (prec == PREC_FP16 || prec == PREC_BF16 || tol < 1e-6) && !c->dense &&
+        it == 0) {
       // product error of this precision on the lowest Ritz vector, against
       // FP64 products or, for FP16 / BF16 products of the AO operator, the
       // FP16 hi / lo split ones (error ~1e-7, three orders below theirs;
-      // a third of an FP64 product's time)
+      // a third of an FP64 product's time).  The FP32-accuracy phases
+      // (FP32 MFMA, FP16 hi / lo split) are measured only for tolerances
+      // below 1e-6: their error, about 1e-7, otherwise lies far below tol, so
+      // every rule it enters (hand-over below floor_k delta, near-floor stall,
+      // Ritz-value progress) is decided by tol; a phase that stalls above tol
+      // still hands over by the stall window.  That saves an FP64 product per
+      // solve.
       const bool ao = c->has_lc && c->op_kind == 0 && !c->ct;
       const int ref_prec = (prec == PREC_FP16 || prec == PREC_BF16) && ao && c->split32 ? PREC_F16X : PREC_FP64;
       products(c, RZ, 1, P, prec);
