@@ -39,7 +39,10 @@ int tda_debug_rowdot(int M, int N, int64_t K, const double* A, int a32, const do
  * hi / lo pairs (hi = fp16(v), lo = fp16((v - hi) 2^11)), the product
  * evaluated as 2^-11 (A_hi B_lo + A_lo B_hi) + A_hi B_hi.  C is returned as
  * FP64 [batch][M][N] (hi + lo 2^-11 when split); *used = 1 when the
- * dedicated kernel ran (0: the generic kernel). */
+ * dedicated kernel ran (0: the generic kernel).  split is a flag word:
+ * bit 0 the split above; bit 1 (epi 2 with hidx) hands the factors over as
+ * the precomputed FP32 table hfull[(m % hmod)][(n % hmod)] = H[r(m)][r(n)]
+ * the AO operator uses, instead of the gathered H. */
 int tda_debug_gemm16(int in_prec, int epi, int64_t M, int64_t N, int64_t K, int batch, int64_t lda, int64_t ldb,
                      const double* A, const double* B, double alpha, const double* H, int nh, const int* hidx,
                      int hmod, const double* diag, const double* X, double* C, int split, int* used);

@@ -177,12 +177,23 @@ SHAPES16 = [(16, 16, 64, 1), (37, 53, 70, 2), (130, 129, 5, 3), (200, 333, 501, 
             (520, 520, 72, 60)]
 
 
+@pytest.fixture(params=["rule", "large"])
+def tile16(request, monkeypatch):
+    """The tile rule of small problems, or the 256x256 tile forced (the
+    diagonal epilogue stays on 128x128)."""
+    if request.param == "large":
+        monkeypatch.setenv("TINCR_TDA_TILE16", "1")
+    else:
+        monkeypatch.delenv("TINCR_TDA_TILE16", raising=False)
+    return request.param
+
+
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
 @pytest.mark.parametrize("shape", SHAPES16)
-def test_gemm16_store(prec, shape):
+def test_gemm16_store(prec, shape, tile16):
     nat = _nat()
     m, n, k, bt = shape
-    print(f"gemm16 store {prec} {shape}", flush=True)
+    print(f"gemm16 store {prec} {shape} tile={tile16}", flush=True)
     rng = np.random.default_rng(m + 7 * n + 13 * k + bt)
     a, b = _inputs16(rng, prec, (m, k), (bt, n, k))
     out, used = nat.debug_gemm16(a, b, prec, "store", alpha=-0.25, pad=3 + (k % 5))
@@ -194,15 +205,6 @@ def test_gemm16_store(prec, shape):
     assert np.all(np.abs(out - ref) <= ulp * (np.abs(ref) + ACC16 * scale) + ACC16 * scale + 1e-30)
 
 
-@pytest.fixture(params=["rule", "large"])
-def tile16(request, monkeypatch):
-    """The tile rule of small problems, or the 256x256 tile forced (FP64
-    outputs stay on 128x128)."""
-    if request.param == "large":
-        monkeypatch.setenv("TINCR_TDA_TILE16", "1")
-    else:
-        monkeypatch.delenv("TINCR_TDA_TILE16", raising=False)
-    return request.param
 
 
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
@@ -225,7 +227,7 @@ def test_gemm16_diag(prec, shape, tile16):
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
 @pytest.mark.parametrize("case", ["ao", "distinct", "identity"])
 @pytest.mark.parametrize("shape", [(200, 333, 501, 2), (520, 520, 72, 60)])
-def test_gemm16_hadamard(prec, case, shape):
+def test_gemm16_hadamard(prec, case, shape, tile16):
     """H[r(m)][r(n)] with r = an AO->atom table applied modulo its length
     (runs of equal keys, a wrap inside tiles: LDS-staged table), distinct
     keys (table larger than the LDS: factors read from H) and no table."""
@@ -245,6 +247,12 @@ def test_gemm16_hadamard(prec, case, shape):
     h = rng.uniform(0.1, 0.5, (nh, nh))
     out, used = nat.debug_gemm16(a, b, prec, "hadamard", alpha=-0.25, h=h, hidx=hidx, pad=5)
     assert used
+    if hidx is not None:
+        # the precomputed FP32 factor table of the AO operator gives the same result
+        out_t, used_t = nat.debug_gemm16(a, b, prec, "hadamard", alpha=-0.25, h=h, hidx=hidx, pad=5,
+                                         full_table=True)
+        assert used_t
+        assert np.array_equal(out_t, out)
     r = (lambda x: hidx[x % len(hidx)]) if hidx is not None else (lambda x: x)
     hf = h[np.ix_(r(np.arange(m)), r(np.arange(n)))]
     ref = -0.25 * _ref16(a, b, prec) * hf[None]

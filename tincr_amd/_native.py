@@ -302,7 +302,7 @@ def debug_rowdot(a, b, a32=False, b32=False, alpha=1.0, beta=0.0, c=None):
 
 
 def debug_gemm16(a, b, precision="fp16", epi="store", alpha=1.0, h=None, hidx=None, diag=None, x=None, pad=0,
-                 split=False):
+                 split=False, full_table=False):
     """The K-contiguous 16-bit GEMM: a [M, K], b [batch, N, K] -> [batch, M, N]
     (FP64 copy of the device result) and whether the dedicated kernel ran.
     Rows are stored with at least `pad` NaN elements past K (never read), the
@@ -326,7 +326,8 @@ def debug_gemm16(a, b, precision="fp16", epi="store", alpha=1.0, h=None, hidx=No
                               dptr(hh) if hh is not None else None, hh.shape[0] if hh is not None else 0,
                               hi.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) if hi is not None else None,
                               len(hi) if hi is not None else 0, dptr(dg) if dg is not None else None,
-                              dptr(xx) if xx is not None else None, dptr(out), int(split), ctypes.byref(used))
+                              dptr(xx) if xx is not None else None, dptr(out), int(split) | (2 if full_table else 0),
+                              ctypes.byref(used))
     if rc != TDA_OK:
         raise TdaError(rc, "tda_debug_gemm16 failed")
     return out, used.value == 1

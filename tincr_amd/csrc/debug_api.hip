@@ -127,7 +127,9 @@ extern "C" int tda_debug_rowdot(int M, int N, int64_t K, const double* A, int a3
 extern "C" int tda_debug_gemm16(int in_prec, int epi, int64_t M, int64_t N, int64_t K, int batch, int64_t lda,
                                 int64_t ldb, const double* A, const double* B, double alpha, const double* H, int nh,
                                 const int* hidx, int hmod, const double* diag, const double* X, double* C,
-                                int split, int* used) {
+                                int flags, int* used) {
+  const int split = flags & 1;
+  const bool full = (flags & 2) != 0;
   return guard([&] {
     TDA_CHECK(in_prec == 2 || in_prec == 3, TDA_ERR_BAD_ARG, "16-bit precision expected");
     TDA_CHECK(epi >= 0 && epi <= 2 && M > 0 && N > 0 && K > 0 && batch > 0 && lda >= K && ldb >= K,
@@ -148,6 +150,7 @@ extern "C" int tda_debug_gemm16(int in_prec, int epi, int64_t M, int64_t N, int6
     }
     Buf h(epi == EPI_HADAMARD ? (size_t)nh * nh * 8 : 8), hi(epi == EPI_HADAMARD && hidx ? (size_t)hmod * 4 : 8);
     Buf dg(epi == EPI_DIAG ? (size_t)M * N * 8 : 8), xs(epi == EPI_DIAG ? nc * 8 : 8);
+    Buf hf(full && epi == EPI_HADAMARD && hmod > 0 ? (size_t)hmod * ((hmod + 3) / 4 * 4) * 4 : 16);
     GemmArgs g;
     std::memset(&g, 0, sizeof(g));
     g.A = a.p; g.B = b.p; g.C = c.p;
@@ -170,6 +173,16 @@ extern "C" int tda_debug_gemm16(int in_prec, int epi, int64_t M, int64_t N, int6
       g.hidx = hidx ? (const int*)hi.p : nullptr;
       g.hmod = hmod;
       g.hm = plain(nh); g.hn = plain(1);
+      if (full) {
+        TDA_CHECK(hidx && hmod > 0, TDA_ERR_BAD_ARG, "the factor table needs hidx");
+        const int64_t hld = (hmod + 3) / 4 * 4;
+        std::vector<float> tab((size_t)hmod * hld, 0.f);
+        for (int a = 0; a < hmod; ++a)
+          for (int b2 = 0; b2 < hmod; ++b2) tab[(size_t)a * hld + b2] = (float)H[(size_t)hidx[a] * nh + hidx[b2]];
+        TDA_HIP(hipMemcpy(hf.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+        g.hfull = (const float*)hf.p;
+        g.hld = hld;
+      }
     } else if (epi == EPI_DIAG) {
       TDA_HIP(hipMemcpy(dg.p, diag, (size_t)M * N * 8, hipMemcpyHostToDevice));
       TDA_HIP(hipMemcpy(xs.p, X, nc * 8, hipMemcpyHostToDevice));
