@@ -1090,22 +1090,24 @@ int gs_block(tda_ctx* c, double* P, double* W, double* Q, int nc, double* wn) {
     launch_gs_gram(Gt, nc, Tq, c->norms.as<double>(), d_count, c->stream);
     int* hc = (int*)(c->pinned() + 320);
     double* hwn = c->pinned() + 192;
+    double* hq = c->pinned() + 384;
     TDA_HIP(hipMemcpyAsync(hc, d_count, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     if (wn) TDA_HIP(hipMemcpyAsync(hwn, c->norms.p, (size_t)nc * 8, hipMemcpyDeviceToHost, c->stream));
+    // Q from all nc rows of T before the count is known (one synchronization
+    // instead of two): row j of Q depends on row j of T only, so the first
+    // count rows are those of the accepted vectors whatever T holds below.
+    // The Gram estimate of |u_j| is a difference of large terms when the
+    // block's vectors are nearly dependent among themselves (large
+    // coefficients T): the accepted q_j must come out unit vectors, else the
+    // block is redone vector by vector from the intact P and W.
+    bcomb(c, Tq, nc, 1, nc, nc, W, false, Q);
+    launch_norms(Q, nc, L, c->npart.as<double>(), c->norms.as<double>() + 64, c->stream);
+    TDA_HIP(hipMemcpyAsync(hq, c->norms.as<double>() + 64, (size_t)nc * 8, hipMemcpyDeviceToHost, c->stream));
     TDA_HIP(hipStreamSynchronize(c->stream));
     if (wn) std::memcpy(wn, hwn, (size_t)nc * 8);
     if (!hc[1]) {
       const int nq = hc[0];
       if (nq == 0) return 0;
-      bcomb(c, Tq, nc, 1, nq, nc, W, false, Q);
-      // The Gram estimate of |u_j| is a difference of large terms when the
-      // block's vectors are nearly dependent among themselves (large
-      // coefficients T): the accepted q_j must come out unit vectors, else
-      // the block is redone vector by vector from the intact P and W.
-      launch_norms(Q, nq, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
-      double* hq = c->pinned() + 384;
-      TDA_HIP(hipMemcpyAsync(hq, c->norms.p, (size_t)nq * 8, hipMemcpyDeviceToHost, c->stream));
-      TDA_HIP(hipStreamSynchronize(c->stream));
       bool unit = true;
       for (int j = 0; j < nq; ++j) unit = unit && std::fabs(hq[j] - 1.0) <= 1e-10;
       if (unit) return nq;
@@ -1441,9 +1443,14 @@ DavResult davidson(tda_ctx* c, int n, int m0, int max_iter, double tol, int prec
       // solve.
       const bool ao = c->has_lc && c->op_kind == 0 && !c->ct;
       const int ref_prec = (prec == PREC_FP16 || prec == PREC_BF16) && ao && c->split32 ? PREC_F16X : PREC_FP64;
-      products(c, RZ, 1, P, prec);
-      products(c, RZ, 1, W, ref_prec);
+      // A_prec z is already at hand: R[0] = (AV Y)[0] - theta_0 z, so
+      // P = R[0] + theta_0 z (AV holds this phase's products; an FP32-stored
+      // AV adds ~6e-8, far below the errors measured here)
       c->one.ensure(8);
+      launch_copy(R, P, L, c->stream);
+      launch_fill(c->one.as<double>(), -h_evals[0], 1, c->stream);
+      launch_residual(P, RZ, c->one.as<double>(), 1, L, c->stream);  // P <- R[0] + theta_0 z
+      products(c, RZ, 1, W, ref_prec);
       launch_fill(c->one.as<double>(), 1.0, 1, c->stream);
       launch_residual(W, P, c->one.as<double>(), 1, L, c->stream);  // W <- W - P
       launch_norms(W, 1, L, c->npart.as<double>(), c->norms.as<double>(), c->stream);
@@ -1541,7 +1548,7 @@ void alloc_davidson(tda_ctx* c, int n, int keep) {
   // that orthonormal_start passes through gs_block
   const int nb = std::max(n, keep);
   c->ensure_pinned(nb);
-  c->norms.ensure((size_t)nb * 8);
+  c->norms.ensure((size_t)(nb + 64) * 8);  // [64, 128): norms of a Gram-Schmidt block's q_j
   c->npart.ensure((size_t)nb * 256 * 8);
   c->idx.ensure((size_t)n * sizeof(int));
   c->count.ensure(sizeof(int) * 4);
