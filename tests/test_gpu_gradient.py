@@ -3,9 +3,12 @@ states: against the oracle's restatement of tda_gradient_lc / _nolc
 (src/gradients/excited_state.rs:56-487) in both forms and with both exchange
 operators, and the exact form against central differences (step 1e-4 bohr,
 the reference's src/gradients/numerical.rs:67-73 check, tolerance 1e-6
-Ha/bohr) of SCC + GPU TDA excitation energies on seeded coordinates of the
-differentiation systems C60H62 and (H2O)64, with the state from an FP64 and
-from an FP32-ladder solve."""
+Ha/bohr) of SCC + GPU TDA excitation energies on 32 seeded coordinates of
+each differentiation system, C60H62 and (H2O)64, with the state from an FP64
+and from an FP32-ladder solve.  The displaced SCCs start from the converged
+density of the undisplaced geometry (the reference's restart from kept P and
+dq) and run their BLAS on 4 threads (small matrices: more threads only add
+synchronisation)."""
 import numpy as np
 import pytest
 
@@ -16,8 +19,10 @@ from tincr_amd import systems as sy
 pytestmark = pytest.mark.gpu
 
 
-def _scc(mol, lc, conv=1e-12):
-    r = scc.run_scc(mol.z, mol.xyz, lc=lc, charge_conv=conv, energy_conv=conv, max_iter=500)
+def _scc(mol, lc, conv=1e-12, p_init=None):
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=4, user_api="blas"):
+        r = scc.run_scc(mol.z, mol.xyz, lc=lc, charge_conv=conv, energy_conv=conv, max_iter=500, p_init=p_init)
     assert r["converged"]
     occ = [i for i, f in enumerate(r["occupation"]) if f > 1.0]
     virt = [i for i, f in enumerate(r["occupation"]) if f <= 1.0]
@@ -113,7 +118,7 @@ def test_exact_gradient_matches_central_differences(name):
     s32.close()
     assert np.abs(exact32 - exact).max() < 1e-7
     rng = np.random.default_rng(7)
-    coords = rng.choice(3 * mol.n_atoms, 4, replace=False)
+    coords = rng.choice(3 * mol.n_atoms, 32, replace=False)
     h = 1e-4
     err_exact, err_ref = [], []
     for c in coords:
@@ -121,7 +126,8 @@ def test_exact_gradient_matches_central_differences(name):
         for sign in (1.0, -1.0):
             m = sy.Molecule(mol.z, mol.xyz.copy())
             m.xyz.reshape(-1)[c] += sign * h
-            rr, oc, vi = _scc(m, True, conv)
+            rr, oc, vi = _scc(m, True, conv, p_init=r["p"])
+            assert (oc, vi) == (occ, virt)
             sd = _system(m, rr, oc, vi)
             wd, _ = sd.run_tda(6, 300, 1e-7)
             sd.close()

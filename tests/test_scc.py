@@ -143,3 +143,25 @@ def test_water_cluster_scc_converges():
     o = r["n_elec"] // 2
     assert r["orbe"][o] - r["orbe"][o - 1] > 0.1
     assert abs(r["dq"].sum()) < 1e-10                  # neutral
+
+
+def test_restart_from_kept_density():
+    """Restarting from the converged density of a nearby geometry (the
+    reference's kept P and dq, scc_routine.rs:104-117) reaches the cold
+    start's fixed point in fewer iterations; restarting from a converged
+    density at its own geometry converges at once."""
+    mol = sy.water_cluster(4, seed=1)
+    conv = dict(charge_conv=1e-11, energy_conv=1e-11, max_iter=300)
+    r0 = scc.run_scc(mol.z, mol.xyz, **conv)
+    xyz = mol.xyz.copy()
+    xyz[2, 0] += 1e-3
+    cold = scc.run_scc(mol.z, xyz, **conv)
+    warm = scc.run_scc(mol.z, xyz, p_init=r0["p"], **conv)
+    assert cold["converged"] and warm["converged"]
+    assert warm["n_iter"] < cold["n_iter"]
+    assert np.abs(warm["orbe"] - cold["orbe"]).max() < 1e-9
+    assert np.abs(warm["dq"] - cold["dq"]).max() < 1e-9
+    assert abs(warm["energy"] - cold["energy"]) < 1e-10
+    again = scc.run_scc(mol.z, mol.xyz, p_init=r0["p"], **conv)
+    assert again["n_iter"] <= 3
+    assert np.abs(again["orbe"] - r0["orbe"]).max() < 1e-10

@@ -449,20 +449,28 @@ def prepare_scc(z, xyz, lc=True, r_lr=sy.LONG_RANGE_RADIUS):
 
 
 def scc_loop(s, h0, gamma, gamma_lr, atom, occ_ao, n_elec, max_iter=MAX_ITER, charge_conv=SCF_CHARGE_CONV,
-             energy_conv=SCF_ENERGY_CONV, verbose=False):
+             energy_conv=SCF_ENERGY_CONV, verbose=False, p_init=None):
     """RestrictedSCC::run_scc (scc_routine.rs:127-290) for a closed shell at
     T = 0 from given H0, S, gamma, gamma_lr (None: no long-range
     correction).  Returns orbs, orbe, p, dq, occupation, energy (electronic
-    SCC energy; the repulsive energy is not included), n_iter, converged."""
+    SCC energy; the repulsive energy is not included), n_iter, converged.
+
+    p_init (optional) starts the iteration from that density matrix and its
+    Mulliken charge differences instead of p0 and zeros, as the reference
+    does when P and dq of a previous calculation are kept
+    (scc_routine.rs:104-117,132-133; the AO charges still start at zero, so
+    the mixer's first step takes the new charges unmixed): from the converged
+    density of a nearby geometry the same fixed point is reached in fewer
+    iterations."""
     atom = np.asarray(atom)
     n_atoms = int(atom.max()) + 1
     n_orbs = len(atom)
     lc = gamma_lr is not None
     g0_lr_ao = gamma_ao_wise(gamma_lr, atom) if lc else None
     p0 = density_matrix_ref(occ_ao)
-    p = p0.copy()
-    dq = np.zeros(n_atoms)
+    p = p0.copy() if p_init is None else np.array(p_init, dtype=np.float64)
     dp = p - p0
+    dq = np.zeros(n_atoms) if p_init is None else mulliken(dp, s, atom, n_atoms)
     q_ao = np.zeros(n_orbs)
     orbs = np.zeros((n_orbs, n_orbs))
     orbe = np.zeros(n_orbs)
@@ -521,7 +529,8 @@ def scc_loop(s, h0, gamma, gamma_lr, atom, occ_ao, n_elec, max_iter=MAX_ITER, ch
 
 
 def run_scc(z, xyz, lc=True, r_lr=sy.LONG_RANGE_RADIUS, verbose=False, **kw):
-    """prepare_scc + scc_loop; the result also carries s, h0, gamma, gamma_lr."""
+    """prepare_scc + scc_loop (keyword arguments go to scc_loop, p_init
+    included); the result also carries s, h0, gamma, gamma_lr."""
     prep = prepare_scc(z, xyz, lc, r_lr)
     out = scc_loop(prep["s"], prep["h0"], prep["gamma"], prep["gamma_lr"], prep["atom"], prep["occ_ao"],
                    prep["n_elec"], verbose=verbose, **kw)
