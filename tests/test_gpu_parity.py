@@ -620,3 +620,36 @@ def test_dense_engine_rejects_tda_outputs():
     rc = eng.ctx.lib.tda_get_charges(eng.ctx.handle, None, None, None, None)
     assert rc == nat.TDA_ERR_STATE
     eng.close()
+
+
+def test_fp16_ao_product_scaled_into_range(monkeypatch):
+    """FP16 AO products on SCC orbitals ((H2O)64): the power-of-two scale of
+    the first GEMM's output (TINCR_TDA_AO_SCALE=1; off by default) keeps the
+    middle matrix in FP16's normal range.  Against the FP64 product of the
+    same unit vectors (random, localised on the lowest orbital pairs,
+    smooth in the orbital-energy gaps), the scaled product's relative error
+    is no larger than the unscaled one and below 5e-5."""
+    from tincr_amd import scc, systems as sy
+    m = scc.tda_inputs_scc(sy.water_cluster(64, seed=0), blas_threads=4)
+    o, v = len(m["occ_indices"]), len(m["virt_indices"])
+    de = m["orbe"][m["virt_indices"]][None, :] - m["orbe"][m["occ_indices"]][:, None]
+    rng = np.random.default_rng(3)
+    x = np.zeros((o * v, 3))
+    x[:, 0] = rng.standard_normal(o * v)
+    low = np.argsort(de.ravel())[:32]
+    x[low, 1] = rng.standard_normal(32)
+    x[:, 2] = (np.exp(-(de - de.min()) / 0.02) * rng.standard_normal((o, v))).ravel()
+    x /= np.linalg.norm(x, axis=0)
+    s64 = _system(m)
+    ref = s64.compute_products(x)
+    s64.close()
+    err = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("TINCR_TDA_AO_SCALE", flag)
+        s = _system(m, precision="fp16", operator="ao")
+        y = s.compute_products(x)
+        s.close()
+        err[flag] = np.linalg.norm(y - ref, axis=0) / np.linalg.norm(ref, axis=0)
+    print("relative error scaled", err["1"], "unscaled", err["0"], flush=True)
+    assert np.all(err["1"] <= err["0"] * 1.05)
+    assert err["1"].max() < 5e-5

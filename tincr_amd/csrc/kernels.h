@@ -19,6 +19,7 @@ void launch_qprod(const double* Lm, int64_t ldl, const double* Rm, int64_t ldr, 
 
 // per-atom max |x| over blocks of `len` elements -> amax[n] (float)
 void launch_block_amax(const double* x, int64_t len, int n, float* amax, hipStream_t s);
+void launch_row_norms(const double* x, int n, int64_t len, int64_t ld, double* out, hipStream_t s);
 // dst[b][e] = (T) (src[b][e] * inv_scale[b])   (inv_scale may be null => 1)
 void launch_convert(const double* src, int64_t len, int n, const float* inv_scale, void* dst, int dtype,
                     hipStream_t s);

@@ -124,6 +124,25 @@ void launch_block_amax(const double* x, int64_t len, int n, float* amax, hipStre
   TDA_LAUNCH_CHECK();
 }
 
+// Euclidean norm of each of n rows of length len (row stride ld): one wave
+// per row, the sum in a fixed order.
+__global__ void __launch_bounds__(256) row_norms_kernel(const double* __restrict__ x, int n, int64_t len,
+                                                        int64_t ld, double* __restrict__ out) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const double* p = x + (int64_t)row * ld;
+  double acc = 0.0;
+  for (int64_t e = threadIdx.x & 63; e < len; e += 64) acc += p[e] * p[e];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) out[row] = sqrt(acc);
+}
+
+void launch_row_norms(const double* x, int n, int64_t len, int64_t ld, double* out, hipStream_t s) {
+  if (n == 0) return;
+  row_norms_kernel<<<(n + 3) / 4, 256, 0, s>>>(x, n, len, ld, out);
+  TDA_LAUNCH_CHECK();
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) convert_kernel(const double* __restrict__ src, int64_t len, int n,
                                                       const float* __restrict__ inv_scale, T* __restrict__ dst) {

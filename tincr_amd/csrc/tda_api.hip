@@ -168,6 +168,9 @@ struct tda_ctx {
   struct LpSet {
     bool ready = false;
     DevBuf qvv, G, Rt, Lm, Rpt, Lpt, scale;  // scale: [s_B*g_B | 1/s_B | 1/g_B] per atom
+    // AO operator, FP16 types: power of two multiplying the first GEMM's
+    // output XR = R x^T (and divided out of the last GEMM's)
+    double xr_scale = 1.0;
   } lps[5];  // per PREC_* (index PREC_F16X: FP16 hi / lo split)
   DevBuf qov32, qvv_amax, G_amax;
   // workspaces
@@ -183,6 +186,7 @@ struct tda_ctx {
   double* Wb() { return PW.as<double>() + pw_rows * ovp; }
   bool gs_gram = true;  // block Gram-Schmidt in coefficient form (launch_gs_gram)
   bool no_gfull = false;  // TINCR_TDA_GFULL=0: 16-bit density GEMM gathers gamma_lr per tile instead
+  bool ao_scale = false;  // TINCR_TDA_AO_SCALE=1: FP16 AO products with the power-of-two XR scale
   int gs_fallbacks = 0;  // blocks that fell back to the vector-by-vector kernel
   int reorth_passes = 0;  // second passes taken (log only)
   int norm_restarts = 0;  // FP64 verifications that found non-unit Ritz vectors
@@ -422,6 +426,32 @@ void prepare_lowprec(tda_ctx* c, int prec) {
         pr.dst->ensure((size_t)pr.len * esz);
         launch_convert(pr.src->as<double>(), pr.len, 1, nullptr, pr.dst->p, prec, c->stream);
       }
+    }
+    ls.xr_scale = 1.0;
+    if ((prec == PREC_FP16 || prec == PREC_F16X) && c->ao_scale) {
+      // FP16 range.  The middle matrix of the product is small (on SCC
+      // (H2O)512 orbitals and unit x: |XR| <= 0.3 with a median of 1e-6 -
+      // 2e-4, |M| <= 2e-3 with a median of 1e-7 - 1e-6), below FP16's
+      // normal range (2^-14), where its relative precision is lost.  XR is
+      // therefore stored as s XR with s = 2^floor(log2(4096 / rho)), rho =
+      // max_row |R_row|: |XR_(s,mu),j| <= |R_(s,mu)| |x_j| <= rho for a
+      // unit x, so s XR <= 4096 and M, F (linear in XR) carry the same
+      // factor well inside the FP16 range; the last GEMM multiplies by 1/s.
+      // Both are exact powers of two.  Emulated on those orbitals: the
+      // product's relative error falls from 2.7e-5 - 6.9e-5 to 1.3e-5 -
+      // 1.7e-5 (random, localised and smooth unit vectors).  Off by
+      // default: the solves it changes got no faster (the FP16 phase runs
+      // longer to its lower floor and the FP32-accuracy phase still takes
+      // the same iterations; (H2O)512: 617 vs 558 ms, C200H202: 59.4 vs
+      // 58.1 ms per solve), and the FP64 verification fixes the results.
+      ls.scale.ensure((size_t)n2 * 8);
+      launch_row_norms(c->aoRt.as<double>(), (int)n2, c->vp, c->vp, ls.scale.as<double>(), c->stream);
+      std::vector<double> rn((size_t)n2);
+      TDA_HIP(hipMemcpyAsync(rn.data(), ls.scale.p, (size_t)n2 * 8, hipMemcpyDeviceToHost, c->stream));
+      TDA_HIP(hipStreamSynchronize(c->stream));
+      double rho = 0.0;
+      for (double r : rn) rho = std::max(rho, r);
+      if (rho > 0.0 && std::isfinite(rho)) ls.xr_scale = std::exp2(std::floor(std::log2(4096.0 / rho)));
     }
     TDA_HIP(hipStreamSynchronize(c->stream));
     ls.ready = true;
@@ -728,6 +758,7 @@ void apply_A_ao(tda_ctx* c, const void* X, int k, void* AX, int prec, bool x32 =
     Xin = c->xlp.p;
   }
   if (lp) prepare_lowprec(c, prec);
+  const double s1 = lp ? c->lps[prec].xr_scale : 1.0;  // XR stored times s1 (FP16 range)
   const void* Rt = lp ? c->lps[prec].Rt.p : c->aoRt.p;
   const void* Lm = lp ? c->lps[prec].Lm.p : c->aoLm.p;
   const void* Rpt = lp ? c->lps[prec].Rpt.p : c->aoRpt.p;
@@ -744,6 +775,7 @@ void apply_A_ao(tda_ctx* c, const void* X, int k, void* AX, int prec, bool x32 =
     GemmArgs g = gargs(Rt, Xin, c->aoXR.p, n2, (int64_t)k * o, vp, plain(w * vp), plain(1), plain(1), plain(w * vp),
                        plain(w * op), split(o, n2 * w * op, 1));
     if (sp) g.a_lo = g.b_lo = vp, g.c_lo = op;
+    g.alpha = s1;
     run_gemm(c, dt, dt, EPI_STORE, true, true, g);
   }
   // G2: M[c] = -1/4 gamma~ o (Lm XR[c]^T)
@@ -795,6 +827,7 @@ void apply_A_ao(tda_ctx* c, const void* X, int k, void* AX, int prec, bool x32 =
     g.b_batch = vp * w * ld;
     g.c_batch = ovp;
     if (sp) g.a_lo = g.b_lo = ld;
+    g.alpha = 1.0 / s1;
     g.diag = c->de.as<double>();
     g.dm = plain(vp);
     g.dmap = plain(1);
@@ -1646,6 +1679,8 @@ int tda_ctx_create(int device, size_t hbm_budget_bytes, int precision, tda_ctx**
     if (eg) c->gs_gram = atoi(eg) != 0;
     const char* egf = getenv("TINCR_TDA_GFULL");
     if (egf) c->no_gfull = atoi(egf) == 0;
+    const char* eas = getenv("TINCR_TDA_AO_SCALE");
+    if (eas) c->ao_scale = atoi(eas) != 0;
     const char* elp = getenv("TINCR_TDA_LP_PHASE");
     if (elp) c->lp_phase = atoi(elp) != 0;
     const char* er = getenv("TINCR_TDA_REORTH");
