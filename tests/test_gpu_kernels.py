@@ -117,7 +117,8 @@ def test_mfma_layout_identity():
         assert np.array_equal(out, a @ b), prec
 
 
-@pytest.mark.parametrize("m,n", [(1, 1), (2, 2), (5, 3), (40, 10), (150, 20), (300, 20), (301, 64)])
+@pytest.mark.parametrize("m,n", [(1, 1), (2, 2), (5, 3), (40, 10), (64, 8), (65, 16), (129, 20), (150, 20),
+                                 (257, 30), (300, 20), (301, 64), (700, 32), (1024, 8)])
 def test_eigh(m, n):
     nat = _nat()
     print(f"eigh m={m} n={n}", flush=True)

@@ -269,8 +269,11 @@ __device__ __forceinline__ void eigh_body(const double* __restrict__ H, int ldh,
     double acc = 0.0;
     {
       const int seg = tid / nt;
-      if (seg < segs) {
-        double* ci = Am + (i - k0) - (int64_t)k0 * lda;  // ci[j * lda] = A(j, i)
+      // The pass is written once and instantiated for the LDS block and for
+      // the global one: through the one generic pointer it compiles to flat
+      // accesses, whose LDS waits also wait for the global loads in flight.
+      auto pass = [&](double* __restrict__ base) {
+        double* ci = base + (i - k0) - (int64_t)k0 * lda;  // ci[j * lda] = A(j, i)
         const double vpi = sv[i], wpi = sw[i];
         const int len = (nt + segs - 1) / segs;
         const int j0 = k + 1 + seg * len, j1 = min(m, j0 + len);
@@ -294,6 +297,10 @@ __device__ __forceinline__ void eigh_body(const double* __restrict__ H, int ldh,
           acc += a * snv[j];
         }
         if (segs > 1) part[seg * nt + (i - k - 1)] = acc;
+      };
+      if (seg < segs) {
+        if (in_lds) pass(pool);
+        else pass(A);
       }
       if (segs > 1) {
         __syncthreads();  // B2: partial sums complete
