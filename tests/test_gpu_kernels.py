@@ -174,8 +174,10 @@ ACC16 = 4e-5
 
 # (M, N, K, batch): edge tiles in M / N, K not a multiple of 8 or of 64, K
 # below one slice, and a batch large enough for the 256x256 tile (>= 512 tiles)
+# (300, 290, 192, 2): K a multiple of 64 with edge tiles — the big tile then
+# loads its K slices straight into LDS
 SHAPES16 = [(16, 16, 64, 1), (37, 53, 70, 2), (130, 129, 5, 3), (200, 333, 501, 2), (257, 255, 130, 1),
-            (520, 520, 72, 60)]
+            (520, 520, 72, 60), (300, 290, 192, 2)]
 
 
 @pytest.fixture(params=["rule", "large"])
@@ -227,7 +229,7 @@ def test_gemm16_diag(prec, shape, tile16):
 
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
 @pytest.mark.parametrize("case", ["ao", "distinct", "identity"])
-@pytest.mark.parametrize("shape", [(200, 333, 501, 2), (520, 520, 72, 60)])
+@pytest.mark.parametrize("shape", [(200, 333, 501, 2), (520, 520, 72, 60), (300, 290, 192, 2)])
 def test_gemm16_hadamard(prec, case, shape, tile16):
     """H[r(m)][r(n)] with r = an AO->atom table applied modulo its length
     (runs of equal keys, a wrap inside tiles: LDS-staged table), distinct
@@ -264,7 +266,7 @@ def test_gemm16_hadamard(prec, case, shape, tile16):
 
 
 @pytest.mark.parametrize("epi", ["store", "diag", "hadamard"])
-@pytest.mark.parametrize("shape", [(37, 53, 70, 2), (200, 333, 501, 2), (520, 520, 72, 60)])
+@pytest.mark.parametrize("shape", [(37, 53, 70, 2), (200, 333, 501, 2), (520, 520, 72, 60), (300, 290, 192, 2)])
 def test_gemm16_split(epi, shape, tile16):
     """FP16 hi / lo split operands and output: about FP32 accuracy.  Plain
     FP16 inputs would be off by ~2^-11 per term (~2e-5 of sum |a b| here);

@@ -75,6 +75,13 @@ static void dispatch_reduce(DType in, DType out, GemmArgs& g, hipStream_t s) {
 // process.  Problems under 1e10 MFMA flops (a few microseconds; the many
 // distinct shapes of small fragments) keep the tile-count rule, as does
 // TINCR_TDA_AUTOTUNE=0.
+// TINCR_TDA_LDS_DMA=0: the 16-bit big tile stages its K slices through
+// VGPRs instead of loading them straight into LDS (read per call)
+bool lds_dma_on() {
+  const char* e = getenv("TINCR_TDA_LDS_DMA");
+  return !(e && e[0] == '0');
+}
+
 namespace {
 using Tune16Key = std::tuple<int, int, bool, int64_t, int64_t, int64_t, int, int64_t, int64_t>;
 std::mutex tune16_mu;

@@ -60,7 +60,12 @@ struct alignas(8) Vec4 {
   T v[4];
 };
 
-template <typename TI, typename TO, typename Cfg, int EPI, bool SPLIT>
+// DMA: the K slices go from global memory straight into LDS
+// (global_load_lds_dwordx4: each wave instruction fills 1 KiB of LDS, lane l
+// its 16-byte chunk l), with no round trip through VGPRs and no LDS stores;
+// the two LDS buffers are separate arrays, so the reads of one are not held
+// up by the transfers into the other.  Needs K a multiple of 64.
+template <typename TI, typename TO, typename Cfg, int EPI, bool SPLIT, bool DMA = false>
 __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs g) {
   using MF = Mfma16<TI>;
   using Frag = typename MF::Frag;
@@ -72,7 +77,8 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
   constexpr int A_PER = A_CH / NT, B_PER = B_CH / NT;
   static_assert(A_CH % NT == 0 && B_CH % NT == 0, "tile / thread count mismatch");
   constexpr int STAGE = (BM + BN) * BK;   // elements per LDS buffer
-  __shared__ __align__(16) TI lds[2 * STAGE];
+  __shared__ __align__(16) TI lds[DMA ? STAGE : 2 * STAGE];
+  __shared__ __align__(16) TI lds_b[DMA ? STAGE : 8];  // DMA: the second buffer
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -186,8 +192,7 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
     const int r = wn0 + j * 16 + fr, kc = ks * 4 + fq;
     return *(const Frag*)(s + BM * BK + r * BK + ((kc ^ (r & 7)) << 3));
   };
-  auto compute = [&](int buf) {
-    const TI* s = lds + buf * STAGE;
+  auto compute_at = [&](const TI* s) {
     static_assert(MI % 2 == 0, "A fragments go in pairs");
     Frag fb[NI], fa0 = frag_a(s, 0, 0), fa1 = frag_a(s, 0, 1);
 #pragma unroll
@@ -224,18 +229,74 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
       }
     }
   };
+  auto compute = [&](int buf) { compute_at(lds + buf * STAGE); };
   using I0 = std::integral_constant<int, 0>;
-  if (nkt > 0) {
-    load(0, I0{});
-    store(0, I0{});
-    __syncthreads();
-  }
-  for (int kt = 0; kt < nkt; ++kt) {
-    if (kt + 1 < nkt) load(kt + 1, I0{});
-    rescale(kt);
-    compute(kt & 1);
-    if (kt + 1 < nkt) store((kt + 1) & 1, I0{});
-    __syncthreads();
+  if constexpr (DMA) {
+    // wave w fills rows 8 (w A_INS + i) .. +7 of the A part and likewise of
+    // the B part; lane l: row +l/8, LDS chunk l%8, which holds K chunk
+    // (l%8) ^ (row & 7) (the swizzle the fragment reads undo)
+    constexpr int NW = NT / 64, A_INS = BM / 8 / NW, B_INS = BN / 8 / NW;
+    static_assert(A_INS * 8 * NW == BM && B_INS * 8 * NW == BN, "DMA row mapping");
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int lrow = lane >> 3, kch = (lane & 7) ^ lrow;
+    uint32_t da[A_INS], db[B_INS];
+#pragma unroll
+    for (int i = 0; i < A_INS; ++i)
+      da[i] = (uint32_t)(min(m0 + (wv * A_INS + i) * 8 + lrow, g.M - 1) * lda) + kch * 8;
+#pragma unroll
+    for (int i = 0; i < B_INS; ++i)
+      db[i] = (uint32_t)(min(n0 + (wv * B_INS + i) * 8 + lrow, g.N - 1) * ldb) + kch * 8;
+    auto dma = [&](int kt, TI* dst) {
+      int ks = kt;
+      uint32_t ao = 0, bo = 0;
+      if constexpr (split_in) {
+        const int seg = kt / nks;
+        ks = kt - seg * nks;
+        ao = seg == 1 ? a_lo : 0u;
+        bo = seg == 0 ? b_lo : 0u;
+      }
+      const uint32_t ka = (uint32_t)ks * BK + ao, kb = (uint32_t)ks * BK + bo;
+#pragma unroll
+      for (int i = 0; i < A_INS; ++i)
+        __builtin_amdgcn_global_load_lds((const void*)(A + (da[i] + ka)),
+                                         (__attribute__((address_space(3))) void*)(dst + (wv * A_INS + i) * 8 * BK),
+                                         16, 0, 0);
+#pragma unroll
+      for (int i = 0; i < B_INS; ++i)
+        __builtin_amdgcn_global_load_lds((const void*)(B + (db[i] + kb)),
+                                         (__attribute__((address_space(3))) void*)(dst + BM * BK +
+                                                                                   (wv * B_INS + i) * 8 * BK),
+                                         16, 0, 0);
+    };
+    // vmcnt(0): this wave's transfers into LDS are complete (the barrier
+    // after it makes every wave's visible)
+    auto dma_wait = [] { __builtin_amdgcn_s_waitcnt(0x0F70); };
+    if (nkt > 0) {
+      dma(0, lds);
+      dma_wait();
+      __syncthreads();
+    }
+    for (int kt = 0; kt < nkt; ++kt) {
+      const bool odd = kt & 1;
+      if (kt + 1 < nkt) dma(kt + 1, odd ? lds : lds_b);
+      rescale(kt);
+      compute_at(odd ? lds_b : lds);
+      dma_wait();
+      __syncthreads();
+    }
+  } else {
+    if (nkt > 0) {
+      load(0, I0{});
+      store(0, I0{});
+      __syncthreads();
+    }
+    for (int kt = 0; kt < nkt; ++kt) {
+      if (kt + 1 < nkt) load(kt + 1, I0{});
+      rescale(kt);
+      compute(kt & 1);
+      if (kt + 1 < nkt) store((kt + 1) & 1, I0{});
+      __syncthreads();
+    }
   }
 
   // ---- epilogue: lane holds C[m][n..n+3], m = row l%16 of block i, n = 4*(l/16) of block j ----
@@ -283,7 +344,7 @@ __global__ void __launch_bounds__(Cfg::NT, Cfg::MINB) gemm16_nt_kernel(GemmArgs 
     int* ckey = rkey + BM;    // [BN]
     int* tot = ckey + BN;     // [NW] slots opened per wave
     ht = (float*)(tot + 16);
-    constexpr int CAP = (int)((sizeof(TI) * 2 * STAGE) / 4) - 2 * (BM + BN) - 16;
+    constexpr int CAP = (int)((sizeof(TI) * (DMA ? 1 : 2) * STAGE) / 4) - 2 * (BM + BN) - 16;
     auto key_of = [&](int64_t x) -> int {
       return g.hidx ? g.hidx[(uint32_t)x % (uint32_t)g.hmod] : (int)x;
     };
@@ -473,12 +534,31 @@ inline bool gemm16_big(const GemmArgs& g) {
   return tiles >= 512;
 }
 
+bool lds_dma_on();  // TINCR_TDA_LDS_DMA (default on)
+
 template <typename TI, typename TO, int EPI, typename Cfg>
 void launch16_cfg(GemmArgs& g, hipStream_t s) {
   dim3 grid((unsigned)ceil_div(g.N, Cfg::BN), (unsigned)ceil_div(g.M, Cfg::BM), (unsigned)g.batch);
+  // the big tile with whole K slices loads them straight into LDS
+  constexpr bool dma_cfg = std::is_same<Cfg, Cfg16Big>::value && EPI != EPI_DIAG;
+  const bool dma = dma_cfg && g.K % 64 == 0 && lds_dma_on();
   if constexpr (std::is_same<TI, _Float16>::value) {
     if (g.a_lo) {
+      if constexpr (dma_cfg) {
+        if (dma) {
+          gemm16_nt_kernel<TI, TO, Cfg, EPI, true, true><<<grid, Cfg::NT, 0, s>>>(g);
+          TDA_LAUNCH_CHECK();
+          return;
+        }
+      }
       gemm16_nt_kernel<TI, TO, Cfg, EPI, true><<<grid, Cfg::NT, 0, s>>>(g);
+      TDA_LAUNCH_CHECK();
+      return;
+    }
+  }
+  if constexpr (dma_cfg) {
+    if (dma) {
+      gemm16_nt_kernel<TI, TO, Cfg, EPI, false, true><<<grid, Cfg::NT, 0, s>>>(g);
       TDA_LAUNCH_CHECK();
       return;
     }
