@@ -211,7 +211,7 @@ def test_gemm16_store(prec, shape, tile16):
 
 
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
-@pytest.mark.parametrize("shape", [(37, 53, 70, 2), (200, 333, 501, 2), (520, 520, 72, 60)])
+@pytest.mark.parametrize("shape", [(37, 53, 70, 2), (200, 333, 501, 2), (520, 520, 72, 60), (300, 290, 192, 2)])
 def test_gemm16_diag(prec, shape, tile16):
     nat = _nat()
     print(f"gemm16 diag {prec} {shape} tile={tile16}", flush=True)

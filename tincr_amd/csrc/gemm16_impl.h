@@ -539,9 +539,8 @@ bool lds_dma_on();  // TINCR_TDA_LDS_DMA (default on)
 template <typename TI, typename TO, int EPI, typename Cfg>
 void launch16_cfg(GemmArgs& g, hipStream_t s) {
   dim3 grid((unsigned)ceil_div(g.N, Cfg::BN), (unsigned)ceil_div(g.M, Cfg::BM), (unsigned)g.batch);
-  // with whole K slices the tiles load them straight into LDS (the FP32 /
-  // FP64-output diagonal epilogue keeps the VGPR staging)
-  constexpr bool dma_cfg = EPI != EPI_DIAG;
+  // with whole K slices the tiles load them straight into LDS
+  constexpr bool dma_cfg = true;
   const bool dma = dma_cfg && g.K % 64 == 0 && lds_dma_on();
   if constexpr (std::is_same<TI, _Float16>::value) {
     if (g.a_lo) {
